@@ -1,0 +1,170 @@
+"""Benchmark of the P2P U/J path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): the round-jet
+column of N = 1,048,576 particles, one all-pairs U+J evaluation (UJ_direct,
+Gaussian-erf kernel) per step, FP32 pair arithmetic with compensated sums,
+particle state resident in HBM.  Also measured in the same run:
+  * the FP64 GPU path (the reference's FP64 pair arithmetic) on the same input,
+  * the FP32c-vs-FP64 per-particle relative error of U and J,
+  * the adjoint/forward time ratio on the N = 32,768 cloud,
+  * the FP64 CPU oracle on a bounded sample of targets (host cores).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "UJ pair-interactions/s FP32 vs FP64 at ≤1e-5 rel err; adjoint/forward ratio"
+FLOP_PER_PAIR = {"gaussianerf": 88, "winckelmans": 97, "gaussian": 86, "singular": 79}
+PEAK_FP32 = 157.3e12
+PEAK_FP64 = 78.6e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(P, kernel, budget_s=12.0):
+    """FP64 oracle (reference-algorithm restatement) on a bounded sample of
+    targets against all sources, OpenMP over targets."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    orc.lib()
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    n = P.shape[0]
+    nt = 64
+    while True:
+        Q = P.copy()
+        t0 = time.perf_counter()
+        orc.direct(Q, kernel, t_range=(0, nt), nthreads=nthreads)
+        dt = time.perf_counter() - t0
+        if dt > budget_s / 4 or nt * 2 > n:
+            break
+        nt = min(n, int(nt * max(2.0, min(8.0, (budget_s / 4) / max(dt, 1e-3)))))
+    rate = nt * n / dt
+    return {"value": rate, "unit": "pair-interactions/s", "cores": nthreads, "kind": "port",
+            "sample": "%d targets x %d sources of the workload, FP64 oracle (oracle/vpm_oracle.c), "
+                      "%.1f s" % (nt, n, dt)}, Q[:nt], nt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--kernel", default="gaussianerf")
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-f64", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus %d is not supported (the all-pairs path "
+              "would shard targets across GPUs, see DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from __graft_entry__ import load_package
+    vpm = load_package()
+    api = vpm.api
+    kernel = args.kernel
+    log("bench: generating jet column N=%d" % args.n)
+    P = vpm.workloads.jet_column(args.n)
+    n = P.shape[0]
+    pairs = float(n) * float(n)
+
+    f = api.Field(n, 0)
+    f.set_particles(P)
+    f.profile(True)
+    log("bench: warmup %d" % args.warmup)
+    for _ in range(args.warmup):
+        f.uj_direct(kernel, "f32c")
+    f.profile(True)   # reset counters
+    f.synchronize()
+    log("bench: timing %d F32C steps" % args.steps)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        f.uj_direct(kernel, "f32c")
+    f.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = 1e3 * wall / args.steps
+    kms, kcount = f.timing(0)
+    k_avg_s = kms / 1e3 / max(kcount, 1)
+    value = pairs / (wall / args.steps)
+    achieved = pairs * FLOP_PER_PAIR[kernel] / k_avg_s
+    out32 = np.zeros_like(P)
+    f.get_particles(out32, rows=(9, 24))
+    log("bench: F32C %.1f ms/step, kernel %.1f ms, %.3e pairs/s" % (ms_step, 1e3 * k_avg_s, value))
+
+    res = {
+        "metric": METRIC, "value": value, "unit": "pair-interactions/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "jet column N=%d, one UJ_direct U+J evaluation, %s kernel, FP32 pair "
+                               "arithmetic with compensated sums, state resident in HBM" % (n, kernel),
+                   "N": n, "kernel": kernel, "pairs_per_step": pairs, "flop_per_pair": FLOP_PER_PAIR[kernel]},
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32, "traffic": None,
+                     "kernel": "uj_f32c_kernel<%s>" % kernel, "avg_launch_ms": 1e3 * k_avg_s},
+    }
+
+    if not args.no_f64:
+        log("bench: FP64 GPU evaluation")
+        f.profile(True)
+        t0 = time.perf_counter()
+        f.uj_direct(kernel, "f64")
+        f.synchronize()
+        w64 = time.perf_counter() - t0
+        k64, _ = f.timing(1)
+        out64 = np.zeros_like(P)
+        f.get_particles(out64, rows=(9, 24))
+        eu = vpm.metrics.err_stats(out32[:, 9:12], out64[:, 9:12])
+        ej = vpm.metrics.err_stats(out32[:, 15:24], out64[:, 15:24])
+        res["f64"] = {"value": pairs / w64, "unit": "pair-interactions/s", "ms_per_step": 1e3 * w64,
+                      "kernel_ms": k64, "speedup_f32c_over_f64": w64 / (wall / args.steps),
+                      "frac_of_fp64_peak": pairs * FLOP_PER_PAIR[kernel] / (k64 / 1e3) / PEAK_FP64}
+        res["accuracy_f32c_vs_f64_gpu"] = {"U": eu, "J": ej}
+        log("bench: FP64 %.1f ms; err U %s J %s" % (1e3 * w64, eu, ej))
+    f.profile(False)
+
+    if not args.no_adjoint:
+        na = 32768
+        Q = vpm.workloads.random_cloud(na, seed=1)
+        w, W = vpm.workloads.adjoint_weights(na)
+        fa = api.Field(na, 0)
+        fa.set_particles(Q)
+        fa.uj_direct(kernel, "f32c")
+        fa.profile(True)
+        t0 = time.perf_counter()
+        fa.uj_direct(kernel, "f32c")
+        tf = time.perf_counter() - t0
+        fa.close()
+        t0 = time.perf_counter()
+        api.uj_adjoint(Q, w, W, kernel=kernel)
+        ta = time.perf_counter() - t0
+        res["adjoint"] = {"N": na, "forward_ms": 1e3 * tf, "adjoint_ms": 1e3 * ta, "ratio": ta / tf,
+                          "adjoint_precision": "f64"}
+        log("bench: adjoint %.1f ms vs forward %.1f ms" % (1e3 * ta, 1e3 * tf))
+
+    if not args.no_cpu:
+        log("bench: CPU baseline")
+        cb, ref_rows, nt = cpu_baseline(P, kernel)
+        res["cpu_baseline"] = cb
+        eu = vpm.metrics.err_stats(out32[:nt, 9:12], ref_rows[:, 9:12])
+        ej = vpm.metrics.err_stats(out32[:nt, 15:24], ref_rows[:, 15:24])
+        res["accuracy_f32c_vs_cpu_oracle"] = {"targets": nt, "U": eu, "J": ej}
+    f.close()
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,331 @@
+// estr.hip -- all-pairs subfilter-scale stretching E_str
+// (Estr_direct!, src/FLOWVPM_subfilterscale_models.jl:16-92).
+//
+// SFS_i += sum_j zeta_sigma_j(r_ij) S_ij over non-static targets i and
+// non-static sources j, with S_ij = (J_i - J_j)^T Gamma_j (transposed scheme)
+// or (J_i - J_j) Gamma_j (classic) and zeta_sigma(r) = zeta(r/sigma)/sigma^3.
+//
+// F32C: write J'_k = J_k - J_ref with J_ref the J of the block's first
+// (spatially sorted) target, subtracted in FP64.  Then
+//     sum_j zeta_j (J'_i - J'_j)^T Gamma_j = J'_i^T A_i - B_i,
+//     A_i = sum_j zeta_j Gamma_j,  B_i = sum_j zeta_j J'_j^T Gamma_j,
+// so a pair costs one zeta and six FMAs, and every FP32 rounding is relative
+// to the local variation of J (not to |J|).  A and B use the same per-tile
+// TwoSum compensation as the U/J kernel; J'_i^T A - B is formed in FP64.
+//
+// F64: the reference's pair expression (subfilterscale_models.jl:22-40) in
+// storage order, contraction off -- the GPU oracle for E_str.
+#include <cfloat>
+
+#include "vpm_internal.h"
+
+namespace vpm {
+
+__device__ __forceinline__ float rsq_e(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float ex2_e(float x) { return __builtin_amdgcn_exp2f(x); }
+constexpr double kLog2eD = 1.4426950408889634;
+
+// zeta_sigma(r) from per-source (a, b) (see pack_estr_kernel)
+template <int K>
+__device__ __forceinline__ float zeta_sig(float r2, float a, float b)
+{
+    if constexpr (K == VPM_KERNEL_GAUSSIANERF) {
+        return b * ex2_e(a * r2);                    // const1 s^-3 exp(-q^2/2)
+    } else if constexpr (K == VPM_KERNEL_GAUSSIAN) {
+        const float r = r2 > 0.f ? r2 * rsq_e(r2) : 0.f;
+        return b * ex2_e(a * r2 * r);                // const3 s^-3 exp(-q^3)
+    } else if constexpr (K == VPM_KERNEL_WINCKELMANS) {
+        const float s = rsq_e(fmaf(a, r2, 1.f));     // (q^2 + 1)^-1/2
+        const float s2 = s * s;
+        return b * (s2 * s2) * (s2 * s);             // 15/(8 pi) s^-3 (q^2+1)^-3.5
+    } else {
+        return r2 == 0.f ? b : 0.f;                  // zeta_sing: 1 at r = 0
+    }
+}
+
+// Source record for E_str, in sorted order: a, b of zeta_sigma (b = 0 for
+// static sources, which the reference skips, subfilterscale_models.jl:66).
+template <int K>
+__global__ __launch_bounds__(kBlock) void pack_estr_kernel(const double* __restrict__ P, int64_t np,
+                                                           const uint32_t* __restrict__ perm,
+                                                           double* __restrict__ pos, double* __restrict__ J,
+                                                           int64_t ld, float4* __restrict__ gz)
+{
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= np) return;
+    const int64_t p = perm ? (int64_t)perm[k] : k;
+    const double* rec = P + p * NF;
+    pos[k] = rec[RX];
+    pos[ld + k] = rec[RX + 1];
+    pos[2 * ld + k] = rec[RX + 2];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) J[c * ld + k] = rec[RJ + c];
+    const double si = 1.0 / rec[RSIG];
+    const double s2 = si * si, s3 = s2 * si;
+    double a, b;
+    if (K == VPM_KERNEL_GAUSSIANERF) { a = -0.5 * kLog2eD * s2; b = kConst1 * s3; }
+    else if (K == VPM_KERNEL_GAUSSIAN) { a = -kLog2eD * s3; b = kConst3 * s3; }
+    else if (K == VPM_KERNEL_WINCKELMANS) { a = s2; b = kConst4 * 7.5 * s3; }
+    else { a = 0.0; b = s3; }
+    if (rec[RSTATIC] != 0) b = 0.0;
+    (void)a;
+    gz[k] = make_float4((float)rec[RG], (float)rec[RG + 1], (float)rec[RG + 2], (float)a);
+    // b rides in the w of a second record: store it in the last position slot
+    pos[3 * ld + k] = b;
+}
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void estr_f32c_kernel(
+    const double* __restrict__ pos, const double* __restrict__ J, int64_t ld,
+    const float4* __restrict__ gz, int64_t np, int transposed, int64_t chunk,
+    double* __restrict__ part, int64_t pld)
+{
+    __shared__ float4 lp[kTile];   // s' x, y, z, a
+    __shared__ float4 lg[kTile];   // Gamma x, y, z, b
+    __shared__ float4 lq[kTile];   // J'^T Gamma (or J' Gamma)
+
+    const int tid = threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + tid;
+    const int64_t ib = (int64_t)blockIdx.x * kBlock;
+    const double ox = pos[ib], oy = pos[ld + ib], oz = pos[2 * ld + ib];
+    double Jr[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) Jr[c] = J[c * ld + ib];
+    const int64_t it = i < np ? i : ib;
+    const float tx = (float)(pos[it] - ox), ty = (float)(pos[ld + it] - oy), tz = (float)(pos[2 * ld + it] - oz);
+
+    const int64_t sb = (int64_t)blockIdx.y * chunk;
+    const int64_t se = min(sb + chunk, np);
+    float sum[6], err[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) { sum[c] = 0.f; err[c] = 0.f; }
+
+    for (int64_t base = sb; base < se; base += kTile) {
+        const int64_t j = base + tid;
+        float4 p, g, q;
+        if (j < se) {
+            const float4 gg = gz[j];
+            p = make_float4((float)(pos[j] - ox), (float)(pos[ld + j] - oy), (float)(pos[2 * ld + j] - oz), gg.w);
+            g = make_float4(gg.x, gg.y, gg.z, (float)pos[3 * ld + j]);
+            double Jp[9];
+#pragma unroll
+            for (int c = 0; c < 9; ++c) Jp[c] = J[c * ld + j] - Jr[c];
+            const double G0 = gg.x, G1 = gg.y, G2 = gg.z;
+            if (transposed) {   // S_a = sum_c J'[3a + c] Gamma_c
+                q = make_float4((float)(Jp[0] * G0 + Jp[1] * G1 + Jp[2] * G2),
+                                (float)(Jp[3] * G0 + Jp[4] * G1 + Jp[5] * G2),
+                                (float)(Jp[6] * G0 + Jp[7] * G1 + Jp[8] * G2), 0.f);
+            } else {            // S_a = sum_c J'[3c + a] Gamma_c
+                q = make_float4((float)(Jp[0] * G0 + Jp[3] * G1 + Jp[6] * G2),
+                                (float)(Jp[1] * G0 + Jp[4] * G1 + Jp[7] * G2),
+                                (float)(Jp[2] * G0 + Jp[5] * G1 + Jp[8] * G2), 0.f);
+            }
+        } else {
+            p = make_float4(1e15f, 1e15f, 1e15f, 0.f);
+            g = make_float4(0.f, 0.f, 0.f, 0.f);
+            q = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        __syncthreads();
+        lp[tid] = p;
+        lg[tid] = g;
+        lq[tid] = q;
+        __syncthreads();
+        float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+        for (int s = 0; s < kTile; ++s) {
+            const float4 a = lp[s];
+            const float4 b = lg[s];
+            const float4 c = lq[s];
+            const float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
+            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            const float z = zeta_sig<K>(r2, a.w, b.w);
+            acc[0] = fmaf(z, b.x, acc[0]);
+            acc[1] = fmaf(z, b.y, acc[1]);
+            acc[2] = fmaf(z, b.z, acc[2]);
+            acc[3] = fmaf(z, c.x, acc[3]);
+            acc[4] = fmaf(z, c.y, acc[4]);
+            acc[5] = fmaf(z, c.z, acc[5]);
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            const float x = acc[c];
+            const float t = sum[c] + x;
+            const float bp = t - sum[c];
+            err[c] += (sum[c] - (t - bp)) + (x - bp);
+            sum[c] = t;
+        }
+    }
+    if (i < np) {
+        double* out = part + (int64_t)blockIdx.y * 6 * pld + i;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) out[c * pld] = (double)sum[c] + (double)err[c];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void estr_finalize_kernel(const double* __restrict__ part, int64_t pld, int K,
+                                                              const double* __restrict__ J, int64_t ld, int64_t np,
+                                                              const uint32_t* __restrict__ perm, int transposed,
+                                                              double* __restrict__ P)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= np) return;
+    const int64_t p = perm ? (int64_t)perm[i] : i;
+    double* rec = P + p * NF;
+    if (rec[RSTATIC] != 0) return;                      // subfilterscale_models.jl:63
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) a[c] += part[((int64_t)k * 6 + c) * pld + i];
+    const int64_t ib = (i / kBlock) * kBlock;
+    double Jp[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) Jp[c] = J[c * ld + i] - J[c * ld + ib];
+    double S[3];
+    if (transposed) {
+        for (int q = 0; q < 3; ++q) S[q] = Jp[3 * q] * a[0] + Jp[3 * q + 1] * a[1] + Jp[3 * q + 2] * a[2] - a[3 + q];
+    } else {
+        for (int q = 0; q < 3; ++q) S[q] = Jp[q] * a[0] + Jp[3 + q] * a[1] + Jp[6 + q] * a[2] - a[3 + q];
+    }
+    rec[RSFS + 0] += S[0];
+    rec[RSFS + 1] += S[1];
+    rec[RSFS + 2] += S[2];
+}
+
+// ---------------------------------------------------------------- F64
+#pragma clang fp contract(off)
+template <int K>
+__device__ __forceinline__ double zeta_ref(double r)
+{
+    if constexpr (K == VPM_KERNEL_SINGULAR) return r == 0 ? 1.0 : 0.0;
+    else if constexpr (K == VPM_KERNEL_GAUSSIAN) return kConst3 * exp(-r * r * r);
+    else if constexpr (K == VPM_KERNEL_GAUSSIANERF) return kConst1 * exp(-r * r / 2);
+    else {
+        double temp = r * r + 1;
+        const double temp2 = temp * temp * temp;
+        temp = temp2 * temp2 * temp;
+        return kConst4 * 7.5 / sqrt(temp);
+    }
+}
+
+// One block per 256 targets (storage order); sources in storage order, whole
+// range per block (E_str is not the headline path; this is the oracle).
+template <int K>
+__global__ __launch_bounds__(kBlock) void estr_f64_kernel(double* __restrict__ P, int64_t np, int transposed)
+{
+    __shared__ double lx[kTile][4];
+    __shared__ double lg[kTile][4];
+    __shared__ double lj[kTile][9];
+    const int tid = threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + tid;
+    const int64_t it = i < np ? i : 0;
+    const double* tr = P + it * NF;
+    const double tx = tr[RX], ty = tr[RX + 1], tz = tr[RX + 2];
+    double JT[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) JT[c] = tr[RJ + c];
+    const bool tstatic = tr[RSTATIC] != 0;
+    double s1 = 0, s2 = 0, s3 = 0;
+    for (int64_t base = 0; base < np; base += kTile) {
+        const int n = (int)min((int64_t)kTile, np - base);
+        __syncthreads();
+        if (tid < n) {
+            const double* sr = P + (base + tid) * NF;
+            lx[tid][0] = sr[RX]; lx[tid][1] = sr[RX + 1]; lx[tid][2] = sr[RX + 2]; lx[tid][3] = sr[RSIG];
+            lg[tid][0] = sr[RG]; lg[tid][1] = sr[RG + 1]; lg[tid][2] = sr[RG + 2]; lg[tid][3] = sr[RSTATIC];
+#pragma unroll
+            for (int c = 0; c < 9; ++c) lj[tid][c] = sr[RJ + c];
+        }
+        __syncthreads();
+        for (int s = 0; s < n; ++s) {
+            if (lg[s][3] != 0) continue;                    // static source
+            const double dx = lx[s][0] - tx, dy = lx[s][1] - ty, dz = lx[s][2] - tz;
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            const double* JS = lj[s];
+            const double* GS = lg[s];
+            double S1, S2, S3;
+            if (transposed) {
+                S1 = (JT[0] - JS[0]) * GS[0] + (JT[1] - JS[1]) * GS[1] + (JT[2] - JS[2]) * GS[2];
+                S2 = (JT[3] - JS[3]) * GS[0] + (JT[4] - JS[4]) * GS[1] + (JT[5] - JS[5]) * GS[2];
+                S3 = (JT[6] - JS[6]) * GS[0] + (JT[7] - JS[7]) * GS[1] + (JT[8] - JS[8]) * GS[2];
+            } else {
+                S1 = (JT[0] - JS[0]) * GS[0] + (JT[3] - JS[3]) * GS[1] + (JT[6] - JS[6]) * GS[2];
+                S2 = (JT[1] - JS[1]) * GS[0] + (JT[4] - JS[4]) * GS[1] + (JT[7] - JS[7]) * GS[2];
+                S3 = (JT[2] - JS[2]) * GS[0] + (JT[5] - JS[5]) * GS[1] + (JT[8] - JS[8]) * GS[2];
+            }
+            const double sigma_inv = 1.0 / lx[s][3];
+            const double zeta_sgm = zeta_ref<K>(r * sigma_inv) * sigma_inv * sigma_inv * sigma_inv;
+            s1 += zeta_sgm * S1;
+            s2 += zeta_sgm * S2;
+            s3 += zeta_sgm * S3;
+        }
+    }
+    if (i < np && !tstatic) {
+        double* rec = P + i * NF;
+        rec[RSFS] += s1;
+        rec[RSFS + 1] += s2;
+        rec[RSFS + 2] += s3;
+    }
+}
+#pragma clang fp contract(on)
+
+static inline unsigned nblk(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+int launch_estr_impl(vpm_field f, int kernel, int transposed, int precision, const uint32_t* perm)
+{
+    const int64_t np = f->np;
+    if (np == 0) return VPM_OK;
+    int rc;
+    if (precision == VPM_PREC_F64) {
+        timing_begin(f);
+        switch (kernel) {
+        case VPM_KERNEL_SINGULAR: hipLaunchKernelGGL(estr_f64_kernel<VPM_KERNEL_SINGULAR>, dim3(nblk(np)), dim3(kBlock), 0, f->stream, f->P, np, transposed); break;
+        case VPM_KERNEL_GAUSSIAN: hipLaunchKernelGGL(estr_f64_kernel<VPM_KERNEL_GAUSSIAN>, dim3(nblk(np)), dim3(kBlock), 0, f->stream, f->P, np, transposed); break;
+        case VPM_KERNEL_GAUSSIANERF: hipLaunchKernelGGL(estr_f64_kernel<VPM_KERNEL_GAUSSIANERF>, dim3(nblk(np)), dim3(kBlock), 0, f->stream, f->P, np, transposed); break;
+        case VPM_KERNEL_WINCKELMANS: hipLaunchKernelGGL(estr_f64_kernel<VPM_KERNEL_WINCKELMANS>, dim3(nblk(np)), dim3(kBlock), 0, f->stream, f->P, np, transposed); break;
+        default: return fail(VPM_EINVAL, "unknown kernel id");
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "estr_f64");
+        timing_end(f, 2);
+        return VPM_OK;
+    }
+    const int64_t ld = f->cap;
+    if ((rc = ensure(f->adj, sizeof(double) * 13 * (size_t)ld + sizeof(float4) * (size_t)ld))) return rc;
+    double* pos = (double*)f->adj.p;            // [4][ld]: x, y, z, b
+    double* J = pos + 4 * ld;                   // [9][ld]
+    float4* gz = (float4*)(J + 9 * ld);
+    switch (kernel) {
+#define VPM_PE(KID) case KID: hipLaunchKernelGGL(pack_estr_kernel<KID>, dim3(nblk(np)), dim3(kBlock), 0, f->stream, f->P, np, perm, pos, J, ld, gz); break;
+        VPM_PE(VPM_KERNEL_SINGULAR) VPM_PE(VPM_KERNEL_GAUSSIAN) VPM_PE(VPM_KERNEL_GAUSSIANERF) VPM_PE(VPM_KERNEL_WINCKELMANS)
+#undef VPM_PE
+    default: return fail(VPM_EINVAL, "unknown kernel id");
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "pack_estr");
+    // source splits, as for U/J
+    const int64_t nb = (np + kBlock - 1) / kBlock;
+    int64_t K = std::max<int64_t>(1, (2560 + nb - 1) / nb);
+    const int64_t tiles = (np + kTile - 1) / kTile;
+    K = std::min(K, tiles);
+    const int64_t chunk = ((tiles + K - 1) / K) * kTile;
+    K = (np + chunk - 1) / chunk;
+    if ((rc = ensure(f->part, sizeof(double) * 6 * (size_t)K * (size_t)np))) return rc;
+    double* part = (double*)f->part.p;
+    timing_begin(f);
+    switch (kernel) {
+#define VPM_ES(KID) case KID: hipLaunchKernelGGL(estr_f32c_kernel<KID>, dim3(nblk(np), (unsigned)K), dim3(kBlock), 0, f->stream, pos, J, ld, gz, np, transposed, chunk, part, np); break;
+        VPM_ES(VPM_KERNEL_SINGULAR) VPM_ES(VPM_KERNEL_GAUSSIAN) VPM_ES(VPM_KERNEL_GAUSSIANERF) VPM_ES(VPM_KERNEL_WINCKELMANS)
+#undef VPM_ES
+    default: return fail(VPM_EINVAL, "unknown kernel id");
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "estr_f32c");
+    timing_end(f, 2);
+    hipLaunchKernelGGL(estr_finalize_kernel, dim3(nblk(np)), dim3(kBlock), 0, f->stream, part, np, (int)K, J, ld, np,
+                       perm, transposed, f->P);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "estr_finalize");
+    return VPM_OK;
+}
+
+}  // namespace vpm

@@ -1,0 +1,270 @@
+// p2p_pair.h -- per-pair device arithmetic shared by the all-pairs and the
+// leaf-pair-list U/J kernels (src/FLOWVPM_fmm.jl:113-161).
+#pragma once
+#include <cfloat>
+
+#include "kernel_fits.h"
+#include "vpm_internal.h"
+
+namespace vpm {
+
+// ---------------------------------------------------------------- FP32 math
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float ex2(float x) { return __builtin_amdgcn_exp2f(x); }
+constexpr float kLog2e = 1.4426950408889634f;
+
+__device__ __forceinline__ float horner8(const float (&c)[8], float x)
+{
+    float a = c[7];
+#pragma unroll
+    for (int k = 6; k >= 0; --k) a = fmaf(a, x, c[k]);
+    return a;
+}
+
+// Far-field threshold q_far^2: for q >= q_far, g = 1 and q g' - 3 g = -3
+// to FP32 rounding.  Gaussian-erf: 1 - g < 2e-10 and |q g'| < 3e-9 at q = 7.
+// Gaussian: exp(-q^3) (1 + q^3) < 6e-11 at q = 3.
+template <int K> struct Far { static constexpr float q2 = 0.f; static constexpr bool has = false; };
+template <> struct Far<VPM_KERNEL_GAUSSIANERF> { static constexpr float q2 = 49.f; static constexpr bool has = true; };
+template <> struct Far<VPM_KERNEL_GAUSSIAN> { static constexpr float q2 = 9.f; static constexpr bool has = true; };
+
+// Full (any-q) evaluation of F' = g/r^3 and H'' = -(q g' - 3g)/(3 r^5).
+// Coincident pairs (r2 == 0) contribute nothing (fmm.jl:118).
+template <int K>
+__device__ __forceinline__ void coef_full(float r2, float sinv, float& Fp, float& Hpp);
+
+template <>
+__device__ __forceinline__ void coef_full<VPM_KERNEL_SINGULAR>(float r2, float, float& Fp, float& Hpp)
+{
+    const float ri = r2 > 0.f ? rsq(r2) : 0.f;
+    const float ri2 = ri * ri;
+    Fp = ri2 * ri;
+    Hpp = Fp * ri2;
+}
+
+template <>
+__device__ __forceinline__ void coef_full<VPM_KERNEL_GAUSSIANERF>(float r2, float sinv, float& Fp, float& Hpp)
+{
+    // g = erf(q/sqrt2) - sqrt(2/pi) q e^{-q^2/2},  q g' = sqrt(2/pi) q^3 e^{-q^2/2}
+    const float ri = r2 > 0.f ? rsq(r2) : 0.f;
+    const float q = r2 * ri * sinv;
+    const float t = q * q;
+    const float s3 = sinv * sinv * sinv;
+    const float s5 = s3 * sinv * sinv;
+    const float Fs = horner8(kErfPhi0, t) * s3;
+    const float Hs = horner8(kErfPhi1, t) * s5;
+    const float E = ex2(-0.5f * kLog2e * t);
+    const float c2qE = (float)kConst2 * q * E;
+    const float g = erff(q * 0.70710678118654752f) - c2qE;
+    const float hm = fmaf(-(1.f / 3.f) * t, c2qE, g);       // g - q g'/3
+    const float ri2 = ri * ri;
+    const float ri3 = ri2 * ri;
+    const bool small = t < kErfFitTMax;
+    Fp = small ? Fs : g * ri3;
+    Hpp = small ? Hs : hm * ri3 * ri2;
+    if (!(r2 > 0.f)) { Fp = 0.f; Hpp = 0.f; }
+}
+
+template <>
+__device__ __forceinline__ void coef_full<VPM_KERNEL_GAUSSIAN>(float r2, float sinv, float& Fp, float& Hpp)
+{
+    // g = 1 - e^{-u}, u = q^3;  -(q g' - 3 g)/3 = 1 - e^{-u}(1 + u)
+    const float ri = r2 > 0.f ? rsq(r2) : 0.f;
+    const float q = r2 * ri * sinv;
+    const float u = q * q * q;
+    const float s3 = sinv * sinv * sinv;
+    const float s5 = s3 * sinv * sinv;
+    const float Fs = horner8(kGauPhi0, u) * s3;
+    const float Hs = q * horner8(kGauPsi, u) * s5;
+    const float E = ex2(-kLog2e * u);
+    const float ri2 = ri * ri;
+    const float ri3 = ri2 * ri;
+    const bool small = u < kGauFitUMax;
+    Fp = small ? Fs : (1.f - E) * ri3;
+    Hpp = small ? Hs : fmaf(-E, 1.f + u, 1.f) * ri3 * ri2;
+    if (!(r2 > 0.f)) { Fp = 0.f; Hpp = 0.f; }
+}
+
+template <>
+__device__ __forceinline__ void coef_full<VPM_KERNEL_WINCKELMANS>(float r2, float sinv, float& Fp, float& Hpp)
+{
+    // phi0 = (t + 5/2)/(t + 1)^{5/2},  -phi1/3 = (t + 7/2)/(t + 1)^{7/2},  t = q^2
+    const float s2 = sinv * sinv;
+    const float t = r2 * s2;
+    const float sr = rsq(t + 1.f);
+    const float w = sr * sr;
+    const float w2s = w * w * sr;
+    const float s3 = s2 * sinv;
+    Fp = (t + 2.5f) * w2s * s3;
+    Hpp = (t + 3.5f) * w2s * w * s3 * s2;
+    if (!(r2 > 0.f)) { Fp = 0.f; Hpp = 0.f; }
+}
+
+// One source (a = (s', far threshold), b = (Gamma', 1/sigma)) acting on the
+// lane's target t' (F32C); accumulates U', W', Jhv (see p2p_uj.hip).
+template <int K>
+__device__ __forceinline__ void pair_f32c(float tx, float ty, float tz, const float4 a, const float4 b,
+                                          float (&acc)[kAccF32])
+{
+    const float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
+    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    float Fp, Hpp;
+    if constexpr (Far<K>::has) {
+        if (__any(r2 < a.w)) {
+            coef_full<K>(r2, b.w, Fp, Hpp);
+        } else {
+            const float ri = rsq(r2);
+            const float ri2 = ri * ri;
+            Fp = ri2 * ri;
+            Hpp = Fp * ri2;
+        }
+    } else {
+        coef_full<K>(r2, b.w, Fp, Hpp);
+    }
+    const float vx = fmaf(dy, b.z, -dz * b.y);
+    const float vy = fmaf(dz, b.x, -dx * b.z);
+    const float vz = fmaf(dx, b.y, -dy * b.x);
+    acc[0] = fmaf(Fp, vx, acc[0]);
+    acc[1] = fmaf(Fp, vy, acc[1]);
+    acc[2] = fmaf(Fp, vz, acc[2]);
+    acc[3] = fmaf(Fp, b.x, acc[3]);
+    acc[4] = fmaf(Fp, b.y, acc[4]);
+    acc[5] = fmaf(Fp, b.z, acc[5]);
+    const float hx = Hpp * vx, hy = Hpp * vy, hz = Hpp * vz;
+    acc[6] = fmaf(hx, dx, acc[6]);     // Jhv[0][0]
+    acc[7] = fmaf(hy, dx, acc[7]);     // Jhv[1][0]
+    acc[8] = fmaf(hz, dx, acc[8]);     // Jhv[2][0]
+    acc[9] = fmaf(hx, dy, acc[9]);     // Jhv[0][1]
+    acc[10] = fmaf(hy, dy, acc[10]);   // Jhv[1][1]
+    acc[11] = fmaf(hz, dy, acc[11]);   // Jhv[2][1]
+    acc[12] = fmaf(hx, dz, acc[12]);   // Jhv[0][2]
+    acc[13] = fmaf(hy, dz, acc[13]);   // Jhv[1][2]
+}
+
+// TwoSum fold of FP32 partials into (sum, err)
+__device__ __forceinline__ void fold_twosum(const float (&acc)[kAccF32], float (&sum)[kAccF32],
+                                            float (&err)[kAccF32])
+{
+#pragma unroll
+    for (int c = 0; c < kAccF32; ++c) {
+        const float x = acc[c];
+        const float t = sum[c] + x;
+        const float bp = t - sum[c];
+        err[c] += (sum[c] - (t - bp)) + (x - bp);
+        sum[c] = t;
+    }
+}
+
+// Adds the F32C accumulators (U', W', Jhv) of one target onto its record:
+// U += U', J = -3 Jhv + eps . W' (column-major), Jhv[2][2] = -Jhv00 - Jhv11.
+__device__ __forceinline__ void add_f32c_to_record(double* rec, const double (&a)[kAccF32])
+{
+    const double Wx = a[3], Wy = a[4], Wz = a[5];
+    const double j8 = -(a[6] + a[10]);
+    rec[RU + 0] += a[0];
+    rec[RU + 1] += a[1];
+    rec[RU + 2] += a[2];
+    rec[RJ + 0] += -3.0 * a[6];
+    rec[RJ + 1] += -3.0 * a[7] - Wz;
+    rec[RJ + 2] += -3.0 * a[8] + Wy;
+    rec[RJ + 3] += -3.0 * a[9] + Wz;
+    rec[RJ + 4] += -3.0 * a[10];
+    rec[RJ + 5] += -3.0 * a[11] - Wx;
+    rec[RJ + 6] += -3.0 * a[12] - Wy;
+    rec[RJ + 7] += -3.0 * a[13] + Wx;
+    rec[RJ + 8] += -3.0 * j8;
+}
+
+#pragma clang fp contract(off)
+// custom_erf64 (src/FLOWVPM_gpu_erf.jl:159-191): fdlibm s_erf.c branches and
+// coefficients (the coefficient table is the published fdlibm one).
+__device__ double erf64_ref(double x)
+{
+    const double xabs = fabs(x);
+    const double sgn = x > 0 ? 1.0 : (x < 0 ? -1.0 : x);
+    double val = sgn * 1.0;
+    if (xabs < 0.84375) {
+        const double z = x * x;
+        const double r = 1.28379167095512558561e-01 + z * (-3.25042107247001499370e-01 + z * (-2.84817495755985104766e-02 + z * (-5.77027029648944159157e-03 + z * -2.37630166566501626084e-05)));
+        const double s = 1.0 + z * (3.97917223959155352819e-01 + z * (6.50222499887672944485e-02 + z * (5.08130628187576562776e-03 + z * (1.32494738004321644526e-04 + z * -3.96022827877536812320e-06))));
+        const double y = r / s;
+        val = sgn * (xabs + xabs * y);
+    } else if (xabs < 1.25) {
+        const double s = xabs - 1.0;
+        const double P = -2.36211856075265944077e-03 + s * (4.14856118683748331666e-01 + s * (-3.72207876035701323847e-01 + s * (3.18346619901161753674e-01 + s * (-1.10894694282396677476e-01 + s * (3.54783043256182359371e-02 + s * -2.16637559486879084300e-03)))));
+        const double Q = 1.0 + s * (1.06420880400844228286e-01 + s * (5.40397917702171048937e-01 + s * (7.18286544141962662868e-02 + s * (1.26171219808761642112e-01 + s * (1.36370839120290507362e-02 + s * 1.19844998467991074170e-02)))));
+        val = sgn * (8.45062911510467529297e-01 + P / Q);
+    } else if (xabs < 2.857142857142857) {
+        const double s = 1.0 / (x * x);
+        const double R = -9.86494403484714822705e-03 + s * (-6.93858572707181764372e-01 + s * (-1.05586262253232909814e+01 + s * (-6.23753324503260060396e+01 + s * (-1.62396669462573470355e+02 + s * (-1.84605092906711035994e+02 + s * (-8.12874355063065934246e+01 + s * -9.81432934416914548592e+00))))));
+        const double S = 1.0 + s * (1.96512716674392571292e+01 + s * (1.37657754143519042600e+02 + s * (4.34565877475229228821e+02 + s * (6.45387271733267880336e+02 + s * (4.29008140027567833386e+02 + s * (1.08635005541779435134e+02 + s * (6.57024977031928170135e+00 + s * -6.04244152148580987438e-02)))))));
+        const double r = exp(-x * x - 0.5625 + R / S);
+        val = sgn * (1.0 - r / xabs);
+    } else if (xabs < 6.0) {
+        const double s = 1.0 / (x * x);
+        const double R = -9.86494292470009928597e-03 + s * (-7.99283237680523006574e-01 + s * (-1.77579549177547519889e+01 + s * (-1.60636384855821916062e+02 + s * (-6.37566443368389627722e+02 + s * (-1.02509513161107724954e+03 + s * -4.83519191608651397019e+02)))));
+        const double S = 1.0 + s * (3.03380607434824582924e+01 + s * (3.25792512996573918826e+02 + s * (1.53672958608443695994e+03 + s * (3.19985821950859553908e+03 + s * (2.55305040643316442583e+03 + s * (4.74528541206955367215e+02 + s * -2.24409524465858183362e+01))))));
+        const double r = exp(-x * x - 0.5625 + R / S);
+        val = sgn * (1.0 - r / xabs);
+    }
+    return val;
+}
+
+// kernel.jl:45-84, statement by statement
+template <int K>
+__device__ __forceinline__ void g_dgdr_ref(double r, double& g, double& dg)
+{
+    if constexpr (K == VPM_KERNEL_SINGULAR) {
+        g = 1.0; dg = 0.0;
+    } else if constexpr (K == VPM_KERNEL_GAUSSIAN) {
+        const double aux = exp(-r * r * r);
+        g = 1 - aux; dg = 3 * r * r * aux;
+    } else if constexpr (K == VPM_KERNEL_GAUSSIANERF) {
+        const double aux = kConst2 * r * exp(-r * r / 2);
+        g = erf64_ref(r / kSqr2) - aux; dg = r * aux;
+    } else {
+        double aux0 = r * r + 1;
+        const double aux02 = aux0 * aux0;
+        aux0 = aux02 * aux02 * aux0;
+        aux0 = sqrt(aux0);
+        g = r * r * r * (r * r + 2.5) / aux0;
+        dg = 7.5 * r * r / (aux0 * (r * r + 1));
+    }
+}
+
+// One source (a = (x, y, z, sigma), b = Gamma) acting on target t in FP64,
+// statement by statement as fmm.jl:113-161.
+template <int K>
+__device__ __forceinline__ void pair_f64(double tx, double ty, double tz, const double4 a, const double4 b,
+                                         double (&acc)[kAccF64])
+{
+    const double dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 != 0.0) {
+        const double sigma = a.w;
+        const double r = sqrt(r2);
+        double g_sgm, dg_sgmdr;
+        g_dgdr_ref<K>(r / sigma, g_sgm, dg_sgmdr);
+        const double r3inv = 1.0 / (r2 * r);
+        const double crss1 = -kConst4 * r3inv * (dy * b.z - dz * b.y);
+        const double crss2 = -kConst4 * r3inv * (dz * b.x - dx * b.z);
+        const double crss3 = -kConst4 * r3inv * (dx * b.y - dy * b.x);
+        acc[0] += g_sgm * crss1;
+        acc[1] += g_sgm * crss2;
+        acc[2] += g_sgm * crss3;
+        const double aux = dg_sgmdr / (sigma * r) - 3 * g_sgm / r2;
+        const double aux2 = -kConst4 * g_sgm * r3inv;
+        acc[3] += aux * crss1 * dx;
+        acc[4] += aux * crss2 * dx - aux2 * b.z;
+        acc[5] += aux * crss3 * dx + aux2 * b.y;
+        acc[6] += aux * crss1 * dy + aux2 * b.z;
+        acc[7] += aux * crss2 * dy;
+        acc[8] += aux * crss3 * dy - aux2 * b.x;
+        acc[9] += aux * crss1 * dz - aux2 * b.y;
+        acc[10] += aux * crss2 * dz + aux2 * b.x;
+        acc[11] += aux * crss3 * dz;
+    }
+}
+#pragma clang fp contract(on)
+
+}  // namespace vpm

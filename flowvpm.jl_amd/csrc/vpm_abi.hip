@@ -1,0 +1,474 @@
+// vpm_abi.hip -- the extern "C" entry points of include/flowvpm_hip.h.
+//
+// Host-side orchestration only: argument checks, device buffers, and the
+// order of launches on the field's stream.  The numerical work is in
+// p2p_uj.hip (U/J), estr.hip (E_str), particle_ops.hip (per-particle FP64),
+// p2p_ranges.hip (leaf-pair lists) and adjoint.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "vpm_internal.h"
+
+namespace vpm {
+
+static thread_local std::string g_last_error;
+
+int fail(int status, const std::string& msg)
+{
+    g_last_error = msg;
+    return status;
+}
+
+int hip_fail(hipError_t e, const char* what)
+{
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? VPM_ENOMEM : VPM_EDEVICE;
+}
+
+int ensure(DevBuf& b, size_t bytes)
+{
+    if (b.bytes >= bytes) return VPM_OK;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);   // synchronizes with pending work
+        b.p = nullptr;
+        b.bytes = 0;
+        if (e != hipSuccess) return hip_fail(e, "hipFree");
+    }
+    size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return hip_fail(e, "hipMalloc");
+    }
+    b.bytes = want;
+    return VPM_OK;
+}
+
+static void release(DevBuf& b)
+{
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+void timing_begin(vpm_field f)
+{
+    if (f->profiling) (void)hipEventRecord(f->ev0, f->stream);
+}
+
+void timing_end(vpm_field f, int which)
+{
+    if (!f->profiling) return;
+    (void)hipEventRecord(f->ev1, f->stream);
+    if (hipEventSynchronize(f->ev1) == hipSuccess) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, f->ev0, f->ev1) == hipSuccess) {
+            f->timing[which].ms += ms;
+            f->timing[which].count += 1;
+        }
+    }
+}
+
+double zeta0_of(int kernel)
+{
+    // kernel.zeta(0) of src/FLOWVPM_kernel.jl:45,51,60,69-74
+    switch (kernel) {
+    case VPM_KERNEL_SINGULAR: return 1.0;
+    case VPM_KERNEL_GAUSSIAN: return kConst3;
+    case VPM_KERNEL_GAUSSIANERF: return kConst1;
+    case VPM_KERNEL_WINCKELMANS: return kConst4 * 7.5;
+    }
+    return NAN;
+}
+
+static int check_cfg(const vpm_config* cfg)
+{
+    if (!cfg) return fail(VPM_EINVAL, "null vpm_config");
+    if (cfg->kernel < VPM_KERNEL_SINGULAR || cfg->kernel > VPM_KERNEL_WINCKELMANS)
+        return fail(VPM_EINVAL, "unknown kernel id " + std::to_string(cfg->kernel));
+    if (cfg->precision != VPM_PREC_F32C && cfg->precision != VPM_PREC_F64)
+        return fail(VPM_EINVAL, "unknown precision " + std::to_string(cfg->precision));
+    return VPM_OK;
+}
+
+static int set_device(vpm_field f)
+{
+    hipError_t e = hipSetDevice(f->device);
+    return e == hipSuccess ? VPM_OK : hip_fail(e, "hipSetDevice");
+}
+
+static int sync(vpm_field f)
+{
+    hipError_t e = hipStreamSynchronize(f->stream);
+    return e == hipSuccess ? VPM_OK : hip_fail(e, "hipStreamSynchronize");
+}
+
+// UJ_direct body on a locked field (src/FLOWVPM_UJ.jl:21-37)
+static int uj_direct_locked(vpm_field f, const vpm_config* cfg, uint32_t flags)
+{
+    int rc;
+    if (flags & VPM_UJ_RESET) if ((rc = launch_reset(f, 0))) return rc;
+    if (flags & VPM_UJ_RESET_SFS) if ((rc = launch_reset(f, 1))) return rc;
+    const bool sort = cfg->precision == VPM_PREC_F32C && !(flags & VPM_UJ_NO_SORT);
+    Staged st;
+    if ((rc = stage_field(f, cfg->kernel, cfg->precision, sort, st))) return rc;
+    if ((rc = launch_uj(f, st, 0, f->np, st, 0, f->np, cfg->kernel, cfg->precision))) return rc;
+    if (flags & VPM_UJ_SFS) {
+        const uint32_t* perm = nullptr;
+        if (cfg->precision == VPM_PREC_F32C && (rc = sort_field(f, perm))) return rc;
+        if ((rc = launch_estr_impl(f, cfg->kernel, cfg->transposed, cfg->precision, perm))) return rc;
+    }
+    return VPM_OK;
+}
+
+static int sfs_before_locked(vpm_field f, const vpm_config* cfg, double alpha, double minC, double maxC)
+{
+    if (minC < 0 || maxC < 0 || minC > maxC)                         // subfilterscale.jl:455-461
+        return fail(VPM_EINVAL, "Invalid C bounds");
+    int rc;
+    if ((rc = launch_scale_sigma(f, alpha, 0))) return rc;
+    if ((rc = uj_direct_locked(f, cfg, VPM_UJ_RESET | VPM_UJ_RESET_SFS | VPM_UJ_SFS))) return rc;
+    if ((rc = launch_sfs_store_test(f, cfg->transposed))) return rc;
+    return launch_scale_sigma(f, alpha, 1);
+}
+
+static int sfs_after_locked(vpm_field f, const vpm_config* cfg, double alpha, double rlxf, double minC,
+                            double maxC, int force_positive, int clip)
+{
+    if (minC < 0 || maxC < 0 || minC > maxC) return fail(VPM_EINVAL, "Invalid C bounds");
+    int nan_flag = 0;
+    int rc = launch_sfs_coefficient(f, cfg->transposed, alpha, rlxf, minC, maxC, zeta0_of(cfg->kernel),
+                                    force_positive, clip, &nan_flag);
+    if (rc) return rc;
+    if (nan_flag) return fail(VPM_ENAN, "NaN in dynamicprocedure_pseudo3level_afterUJ");
+    return VPM_OK;
+}
+
+}  // namespace vpm
+
+using namespace vpm;
+
+#define VPM_FIELD_GUARD(f)                                                     \
+    if (!(f)) return fail(VPM_EINVAL, "null vpm_field");                       \
+    std::lock_guard<std::mutex> lock_((f)->mu);                                \
+    if (int rc_ = set_device(f)) return rc_;
+
+extern "C" {
+
+const char* vpm_version(void) { return "flowvpm_hip 0.1.0 (gfx950, abi 1)"; }
+
+const char* vpm_last_error(void) { return g_last_error.c_str(); }
+
+int vpm_device_count(int* count)
+{
+    if (!count) return fail(VPM_EINVAL, "null count");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        *count = 0;
+        return hip_fail(e, "hipGetDeviceCount");
+    }
+    *count = n;
+    return VPM_OK;
+}
+
+int vpm_create(int device, int64_t maxparticles, vpm_field* out)
+{
+    if (!out) return fail(VPM_EINVAL, "null output handle");
+    *out = nullptr;
+    if (maxparticles < 1 || maxparticles > (int64_t)UINT32_MAX)
+        return fail(VPM_EINVAL, "maxparticles must be in [1, 2^32)");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+    if (device < 0 || device >= n) return fail(VPM_EDEVICE, "no such device " + std::to_string(device));
+    vpm_field f = new (std::nothrow) vpm_field_s();
+    if (!f) return fail(VPM_ENOMEM, "host allocation failed");
+    f->device = device;
+    f->cap = maxparticles;
+    if ((e = hipSetDevice(device)) != hipSuccess) { delete f; return hip_fail(e, "hipSetDevice"); }
+    if ((e = hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking)) != hipSuccess) {
+        delete f;
+        return hip_fail(e, "hipStreamCreateWithFlags");
+    }
+    if ((e = hipMalloc(&f->P, sizeof(double) * NF * (size_t)maxparticles)) != hipSuccess) {
+        (void)hipStreamDestroy(f->stream);
+        delete f;
+        return hip_fail(e, "hipMalloc(particles)");
+    }
+    if ((e = hipMemset(f->P, 0, sizeof(double) * NF * (size_t)maxparticles)) != hipSuccess ||
+        (e = hipEventCreate(&f->ev0)) != hipSuccess || (e = hipEventCreate(&f->ev1)) != hipSuccess) {
+        int rc = hip_fail(e, "field setup");
+        (void)hipFree(f->P);
+        (void)hipStreamDestroy(f->stream);
+        delete f;
+        return rc;
+    }
+    *out = f;
+    return VPM_OK;
+}
+
+int vpm_destroy(vpm_field f)
+{
+    if (!f) return VPM_OK;
+    {
+        std::lock_guard<std::mutex> lock(f->mu);
+        (void)hipSetDevice(f->device);
+        (void)hipStreamSynchronize(f->stream);
+        DevBuf* bufs[] = {&f->pos, &f->gs, &f->cut, &f->g64, &f->part, &f->keys, &f->keys2, &f->perm,
+                          &f->iota, &f->sorttmp, &f->bbox, &f->flag, &f->adj, &f->rng};
+        for (DevBuf* b : bufs) release(*b);
+        if (f->P) (void)hipFree(f->P);
+        if (f->ev0) (void)hipEventDestroy(f->ev0);
+        if (f->ev1) (void)hipEventDestroy(f->ev1);
+        if (f->stream) (void)hipStreamDestroy(f->stream);
+    }
+    delete f;
+    return VPM_OK;
+}
+
+int vpm_set_particles(vpm_field f, const double* particles, int64_t np)
+{
+    VPM_FIELD_GUARD(f);
+    if (np < 0 || (np > 0 && !particles)) return fail(VPM_EINVAL, "bad particle array");
+    if (np > f->cap) return fail(VPM_ERANGE, "PARTICLE OVERFLOW: " + std::to_string(np) + " > maxparticles " +
+                                                  std::to_string(f->cap));
+    for (int64_t i = 0; i < np; ++i) {
+        const double* p = particles + i * NF;
+        for (int k = 0; k < 7; ++k)
+            if (!std::isfinite(p[k]))
+                return fail(VPM_EINVAL, "non-finite X/Gamma/sigma at particle " + std::to_string(i));
+        if (!(p[RSIG] > 0)) return fail(VPM_EINVAL, "sigma <= 0 at particle " + std::to_string(i));
+    }
+    if (np > 0) {
+        hipError_t e = hipMemcpyAsync(f->P, particles, sizeof(double) * NF * (size_t)np, hipMemcpyHostToDevice,
+                                      f->stream);
+        if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(H2D)");
+    }
+    f->np = np;
+    f->perm_valid = false;
+    return sync(f);
+}
+
+int vpm_get_particles(vpm_field f, double* particles, int64_t np, int32_t row0, int32_t row1)
+{
+    VPM_FIELD_GUARD(f);
+    if (np < 0 || np > f->np) return fail(VPM_ERANGE, "np exceeds the field's particle count");
+    if (row0 < 0 || row1 > NF || row0 >= row1) return fail(VPM_EINVAL, "bad row range");
+    if (np == 0) return VPM_OK;
+    if (!particles) return fail(VPM_EINVAL, "null output array");
+    const size_t pitch = sizeof(double) * NF;
+    hipError_t e = hipMemcpy2DAsync(particles + row0, pitch, f->P + row0, pitch, sizeof(double) * (row1 - row0),
+                                    (size_t)np, hipMemcpyDeviceToHost, f->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy2DAsync(D2H)");
+    return sync(f);
+}
+
+int vpm_get_np(vpm_field f, int64_t* np)
+{
+    if (!f || !np) return fail(VPM_EINVAL, "null argument");
+    *np = f->np;
+    return VPM_OK;
+}
+
+int vpm_synchronize(vpm_field f)
+{
+    VPM_FIELD_GUARD(f);
+    return sync(f);
+}
+
+int vpm_uj_direct(vpm_field f, const vpm_config* cfg, uint32_t flags)
+{
+    VPM_FIELD_GUARD(f);
+    if (int rc = check_cfg(cfg)) return rc;
+    if (int rc = uj_direct_locked(f, cfg, flags)) return rc;
+    return sync(f);
+}
+
+int vpm_uj_direct_fields(vpm_field target, vpm_field source, const vpm_config* cfg)
+{
+    if (!target || !source) return fail(VPM_EINVAL, "null vpm_field");
+    if (int rc = check_cfg(cfg)) return rc;
+    if (target == source) {
+        std::lock_guard<std::mutex> lock(target->mu);
+        if (int rc = set_device(target)) return rc;
+        Staged st;
+        const bool sort = cfg->precision == VPM_PREC_F32C;
+        if (int rc = stage_field(target, cfg->kernel, cfg->precision, sort, st)) return rc;
+        if (int rc = launch_uj(target, st, 0, target->np, st, 0, target->np, cfg->kernel, cfg->precision)) return rc;
+        return sync(target);
+    }
+    if (target->device != source->device) return fail(VPM_EINVAL, "fields live on different devices");
+    // lock in address order to avoid lock inversion between two callers
+    vpm_field a = target < source ? target : source, b = target < source ? source : target;
+    std::lock_guard<std::mutex> la(a->mu), lb(b->mu);
+    if (int rc = set_device(target)) return rc;
+    const bool sort = cfg->precision == VPM_PREC_F32C;
+    Staged ss, ts;
+    if (int rc = stage_field(source, cfg->kernel, cfg->precision, sort, ss)) return rc;
+    if (int rc = sync(source)) return rc;          // source staging is on the source stream
+    if (int rc = stage_field(target, cfg->kernel, cfg->precision, sort, ts)) return rc;
+    if (int rc = launch_uj(target, ts, 0, target->np, ss, 0, source->np, cfg->kernel, cfg->precision)) return rc;
+    return sync(target);
+}
+
+int vpm_p2p_ranges(vpm_field f, const vpm_config* cfg, const int64_t* tb, const int64_t* te, const int64_t* sb,
+                   const int64_t* se, int64_t npairs)
+{
+    VPM_FIELD_GUARD(f);
+    if (int rc = check_cfg(cfg)) return rc;
+    if (npairs < 0) return fail(VPM_EINVAL, "npairs < 0");
+    if (npairs == 0) return VPM_OK;
+    if (!tb || !te || !sb || !se) return fail(VPM_EINVAL, "null range array");
+    for (int64_t k = 0; k < npairs; ++k) {
+        if (tb[k] < 0 || te[k] > f->np || tb[k] > te[k] || sb[k] < 0 || se[k] > f->np || sb[k] > se[k])
+            return fail(VPM_EINVAL, "range pair " + std::to_string(k) + " outside [0, np)");
+    }
+    if (int rc = launch_p2p_ranges(f, cfg->kernel, cfg->precision, tb, te, sb, se, npairs)) return rc;
+    return sync(f);
+}
+
+int vpm_estr_direct(vpm_field f, const vpm_config* cfg)
+{
+    VPM_FIELD_GUARD(f);
+    if (int rc = check_cfg(cfg)) return rc;
+    const uint32_t* perm = nullptr;
+    if (cfg->precision == VPM_PREC_F32C)
+        if (int rc = sort_field(f, perm)) return rc;
+    if (int rc = launch_estr_impl(f, cfg->kernel, cfg->transposed, cfg->precision, perm)) return rc;
+    return sync(f);
+}
+
+int vpm_reset_particles(vpm_field f)
+{
+    VPM_FIELD_GUARD(f);
+    if (int rc = launch_reset(f, 0)) return rc;
+    return sync(f);
+}
+
+int vpm_reset_particles_sfs(vpm_field f)
+{
+    VPM_FIELD_GUARD(f);
+    if (int rc = launch_reset(f, 1)) return rc;
+    return sync(f);
+}
+
+int vpm_update_particle_states(vpm_field f, double a, double b, double dt, const double Uinf[3], double fr,
+                               double gr, double zeta0, int32_t transposed)
+{
+    VPM_FIELD_GUARD(f);
+    if (!Uinf) return fail(VPM_EINVAL, "null Uinf");
+    if (int rc = launch_update_states(f, a, b, dt, Uinf, fr, gr, zeta0, transposed)) return rc;
+    return sync(f);
+}
+
+int vpm_relax(vpm_field f, double rlxf, int32_t corrected)
+{
+    VPM_FIELD_GUARD(f);
+    if (int rc = launch_relax(f, rlxf, corrected)) return rc;
+    return sync(f);
+}
+
+int vpm_core_spreading(vpm_field f, double nu, double a, double b, double dt)
+{
+    VPM_FIELD_GUARD(f);
+    if (int rc = launch_core_spreading(f, nu, a, b, dt)) return rc;
+    return sync(f);
+}
+
+int vpm_sfs_dynamic_before(vpm_field f, const vpm_config* cfg, double alpha, double minC, double maxC)
+{
+    VPM_FIELD_GUARD(f);
+    if (int rc = check_cfg(cfg)) return rc;
+    if (int rc = sfs_before_locked(f, cfg, alpha, minC, maxC)) return rc;
+    return sync(f);
+}
+
+int vpm_sfs_dynamic_after(vpm_field f, const vpm_config* cfg, double alpha, double rlxf, double minC, double maxC,
+                          int32_t force_positive, int32_t clip)
+{
+    VPM_FIELD_GUARD(f);
+    if (int rc = check_cfg(cfg)) return rc;
+    return sfs_after_locked(f, cfg, alpha, rlxf, minC, maxC, force_positive, clip);
+}
+
+int vpm_rk3_step(vpm_field f, vpm_step_config* s, double dt, int32_t relax_now)
+{
+    VPM_FIELD_GUARD(f);
+    if (!s) return fail(VPM_EINVAL, "null vpm_step_config");
+    const vpm_config* cfg = &s->uj;
+    if (int rc = check_cfg(cfg)) return rc;
+    if (s->viscous == 1 && cfg->kernel != VPM_KERNEL_GAUSSIANERF)   // FLOWVPM.jl:265-267, utils.jl:58-64
+        return fail(VPM_EINVAL, "CoreSpreading requires the Gaussian-erf kernel");
+    const double zeta0 = zeta0_of(cfg->kernel);                    // timeintegration.jl:398
+    const bool sfs = s->sfs != 0;
+    int rc;
+    bool reset_needed = false;
+    if ((rc = launch_reset(f, 2))) return rc;                      // :401-414
+    const double ab[3][2] = {{0.0, 1.0 / 3.0}, {-5.0 / 9.0, 15.0 / 16.0}, {-153.0 / 128.0, 8.0 / 15.0}};
+    for (int st = 0; st < 3; ++st) {
+        const double a = ab[st][0], b = ab[st][1];
+        const bool first = (a == 1 || a == 0);                      // subfilterscale.jl:207,219
+        if (sfs && first)
+            if ((rc = sfs_before_locked(f, cfg, s->sfs_alpha, s->sfs_minC, s->sfs_maxC))) return rc;
+        const uint32_t fl = VPM_UJ_RESET | (sfs ? (VPM_UJ_RESET_SFS | VPM_UJ_SFS) : 0u);
+        if ((rc = uj_direct_locked(f, cfg, fl))) return rc;        // :422
+        if (sfs && first)
+            if ((rc = sfs_after_locked(f, cfg, s->sfs_alpha, s->sfs_rlxf, s->sfs_minC, s->sfs_maxC,
+                                       s->sfs_force_positive, s->sfs_clip)))
+                return rc;
+        if ((rc = launch_update_states(f, a, b, dt, s->Uinf, s->f, s->g, zeta0, cfg->transposed))) return rc;
+        if (s->viscous == 1) {                                      // viscous.jl:167-221
+            if ((rc = launch_core_spreading(f, s->nu, a, b, dt))) return rc;
+            if (std::fabs(b - 8.0 / 15.0) <= 1e-7) {
+                s->t_sgm += dt;
+                const double beta_cur = std::sqrt(2 * s->nu * s->t_sgm / (s->sgm0 * s->sgm0) + 1);
+                if (beta_cur >= s->beta) reset_needed = true;
+            }
+        }
+    }
+    if (relax_now && s->relax) {                                    // :437-458
+        if ((rc = launch_reset(f, 0))) return rc;
+        if ((rc = uj_direct_locked(f, cfg, VPM_UJ_RESET))) return rc;
+        if ((rc = launch_relax(f, s->rlxf, s->relax == 2))) return rc;
+    }
+    if ((rc = launch_update_uprev(f))) return rc;                   // particlefield.jl:443-455
+    if ((rc = sync(f))) return rc;
+    if (reset_needed)
+        return fail(VPM_ERESET, "core-spreading reset required (sigma growth >= beta); RBF reset not provided");
+    return VPM_OK;
+}
+
+int vpm_uj_adjoint(vpm_field f, const vpm_config* cfg, const double* Ubar, const double* Jbar, double* Xbar,
+                   double* Gbar, double* Sbar)
+{
+    VPM_FIELD_GUARD(f);
+    if (int rc = check_cfg(cfg)) return rc;
+    if (f->np > 0 && (!Ubar || !Jbar || !Xbar || !Gbar || !Sbar)) return fail(VPM_EINVAL, "null adjoint array");
+    if (int rc = launch_uj_adjoint(f, cfg->kernel, Ubar, Jbar, Xbar, Gbar, Sbar)) return rc;
+    return sync(f);
+}
+
+int vpm_profile(vpm_field f, int32_t enable)
+{
+    VPM_FIELD_GUARD(f);
+    f->profiling = enable != 0;
+    for (auto& t : f->timing) t = Timing();
+    return VPM_OK;
+}
+
+int vpm_timing(vpm_field f, int32_t which, double* total_ms, int64_t* count)
+{
+    if (!f || !total_ms || !count) return fail(VPM_EINVAL, "null argument");
+    if (which < 0 || which > 3) return fail(VPM_EINVAL, "timing class must be 0..3");
+    std::lock_guard<std::mutex> lock(f->mu);
+    *total_ms = f->timing[which].ms;
+    *count = f->timing[which].count;
+    return VPM_OK;
+}
+
+}  // extern "C"

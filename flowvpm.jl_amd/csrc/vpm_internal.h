@@ -1,0 +1,113 @@
+// vpm_internal.h -- shared declarations of the gfx950 P2P library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+
+#include "../../include/flowvpm_hip.h"
+
+namespace vpm {
+
+// Rows of the 46-double particle record (src/FLOWVPM_particlefield.jl:239-252)
+enum : int {
+    RX = 0, RG = 3, RSIG = 6, RVOL = 7, RCIRC = 8, RU = 9, RW = 12, RJ = 15,
+    RPSE = 24, RM = 27, RC = 36, RSFS = 39, RSTATIC = 42, RUPREV = 43, NF = 46
+};
+
+constexpr double kPi = 3.141592653589793;
+// Constants of src/FLOWVPM.jl:62-66, as the exact doubles Julia computes.
+constexpr double kConst1 = 0x1.0411e71d77959p-4;    // 1/(2*pi)^1.5
+constexpr double kConst2 = 0x1.9884533d43651p-1;    // sqrt(2/pi)
+constexpr double kConst3 = 0x1.e8ec8a4aeacc4p-3;    // 3/(4*pi)
+constexpr double kConst4 = 0x1.45f306dc9c883p-4;    // 1/(4*pi)
+constexpr double kSqr2 = 0x1.6a09e667f3bcdp+0;      // sqrt(2)
+
+constexpr int kBlock = 256;      // threads per block (4 waves of 64)
+constexpr int kTile = 256;       // sources per LDS tile
+constexpr int kAccF32 = 14;      // U'(3) W'(3) Jhv(8) per target, F32C path
+constexpr int kAccF64 = 12;      // U(3) J(9) per target, F64 path
+
+// Grow-only device buffer owned by a field.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Timing {
+    double ms = 0.0;
+    int64_t count = 0;
+};
+
+}  // namespace vpm
+
+struct vpm_field_s {
+    int device = 0;
+    int64_t cap = 0;
+    int64_t np = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    double* P = nullptr;             // cap x 46 particle records (FP64 master state)
+    // staging buffers (sorted or storage order), grown on demand
+    vpm::DevBuf pos;                 // double [3][cap]  positions, SoA
+    vpm::DevBuf gs;                  // float4 [cap]     (-Gamma/(4 pi), 1/sigma)
+    vpm::DevBuf cut;                 // float  [cap]     far-field r^2 threshold
+    vpm::DevBuf g64;                 // double4 [cap]    (Gamma, sigma)
+    vpm::DevBuf part;                // double partial sums [K][acc][ld]
+    vpm::DevBuf keys, keys2, perm, iota, sorttmp;
+    bool perm_valid = false;         // perm matches the current positions
+    vpm::DevBuf bbox;                // double [6 + partials]
+    vpm::DevBuf flag;                // int device error flag
+    vpm::DevBuf adj;                 // adjoint / E_str staging scratch
+    vpm::DevBuf rng;                 // leaf-pair range lists
+    bool profiling = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    vpm::Timing timing[4];
+};
+
+namespace vpm {
+
+// thread-local error text, set by fail()
+int fail(int status, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+// ensure a grow-only buffer holds at least `bytes`
+int ensure(DevBuf& b, size_t bytes);
+
+// Launchers (stream-ordered on f->stream).  Return a vpm_status.
+struct Staged {
+    const double* pos;   // SoA positions, stride ld
+    int64_t ld;
+    const float4* gs;
+    const float* cut;
+    const double4* g64;
+    const uint32_t* perm;  // sorted index -> particle index, or nullptr
+};
+
+int sort_field(vpm_field f, const uint32_t*& perm);
+int stage_field(vpm_field f, int kernel, int precision, bool sort, Staged& out);
+int launch_uj(vpm_field ft, const Staged& T, int64_t t0, int64_t nt,
+              const Staged& S, int64_t s0, int64_t s1, int kernel, int precision);
+int launch_estr_impl(vpm_field f, int kernel, int transposed, int precision, const uint32_t* perm);
+int launch_reset(vpm_field f, int which);
+int launch_update_states(vpm_field f, double a, double b, double dt, const double Uinf[3],
+                         double fr, double gr, double zeta0, int transposed);
+int launch_relax(vpm_field f, double rlxf, int corrected);
+int launch_core_spreading(vpm_field f, double nu, double a, double b, double dt);
+int launch_scale_sigma(vpm_field f, double alpha, int divide);
+int launch_sfs_store_test(vpm_field f, int transposed);
+int launch_sfs_coefficient(vpm_field f, int transposed, double alpha, double rlxf, double minC,
+                           double maxC, double zeta0, int force_positive, int clip, int* nan_flag);
+int launch_update_uprev(vpm_field f);
+int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb, const int64_t* te,
+                      const int64_t* sb, const int64_t* se, int64_t npairs);
+int launch_uj_adjoint(vpm_field f, int kernel, const double* Ubar, const double* Jbar,
+                      double* Xbar, double* Gbar, double* Sbar);
+
+double zeta0_of(int kernel);
+
+// kernel timing helpers
+void timing_begin(vpm_field f);
+void timing_end(vpm_field f, int which);
+
+}  // namespace vpm

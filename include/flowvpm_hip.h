@@ -1,0 +1,170 @@
+/*
+ * flowvpm_hip.h -- C ABI of the MI355X (gfx950) particle-to-particle library.
+ *
+ * Drop-in boundary for FLOWVPM's P2P path.  The caller keeps its particle
+ * matrix exactly as ParticleField stores it: Float64, particle-major, one
+ * 46-double record per particle (src/FLOWVPM_particlefield.jl:11,134), rows
+ * (0-based) X 0-2, Gamma 3-5, sigma 6, vol 7, circulation 8, U 9-11,
+ * vorticity 12-14, J 15-23, PSE 24-26, M 27-35, C 36-38, SFS 39-41,
+ * static 42, U_prev 43 (particlefield.jl:239-252).  J is column-major,
+ * J[3*b + a] = du_a/dx_b (src/FLOWVPM_fmm.jl:160).
+ *
+ * A vpm_field is a device-resident copy of one particle matrix on one GPU;
+ * it persists across calls, so a time step never re-uploads state.  Every
+ * function returns a vpm_status; on failure vpm_last_error() (thread-local)
+ * describes it.  No function keeps a caller pointer past its return.  Calls
+ * on one vpm_field are serialized by an internal mutex; distinct fields are
+ * independent.  Index ranges are 0-based and half-open.
+ */
+#ifndef FLOWVPM_HIP_H
+#define FLOWVPM_HIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VPM_NFIELDS 46
+#define VPM_ABI_VERSION 1
+
+typedef enum {
+    VPM_OK = 0,
+    VPM_EINVAL = -1,    /* bad argument (null pointer, bad id, bad range)      */
+    VPM_ERANGE = -2,    /* particle count above the field's capacity           */
+    VPM_ENOMEM = -3,    /* device allocation failed                            */
+    VPM_EDEVICE = -4,   /* no such device or a HIP runtime error               */
+    VPM_ENAN = -5,      /* NaN in the dynamic SFS coefficient (subfilterscale.jl:645-652) */
+    VPM_ERESET = -6     /* core-spreading reset required (viscous.jl:202); not provided */
+} vpm_status;
+
+/* Regularization kernels (src/FLOWVPM.jl:129-133, src/FLOWVPM_kernel.jl:45-84) */
+enum { VPM_KERNEL_SINGULAR = 0, VPM_KERNEL_GAUSSIAN = 1,
+       VPM_KERNEL_GAUSSIANERF = 2, VPM_KERNEL_WINCKELMANS = 3 };
+
+/* Pair-arithmetic precision.  F32C: FP32 pair arithmetic, compensated
+ * per-target sums, results folded into the FP64 state.  F64: the reference's
+ * FP64 pair arithmetic (fmm.jl:113-161 with custom_erf64). */
+enum { VPM_PREC_F32C = 0, VPM_PREC_F64 = 1 };
+
+/* flags of vpm_uj_direct: the keywords of UJ_direct (src/FLOWVPM_UJ.jl:21-25) */
+enum { VPM_UJ_RESET = 1, VPM_UJ_RESET_SFS = 2, VPM_UJ_SFS = 4,
+       VPM_UJ_NO_SORT = 8 /* evaluate in storage order (no spatial sort) */ };
+
+typedef struct {
+    int32_t kernel;       /* VPM_KERNEL_*                                    */
+    int32_t precision;    /* VPM_PREC_*                                      */
+    int32_t transposed;   /* 1: transposed stretching (ParticleField.transposed) */
+    int32_t reserved;
+} vpm_config;
+
+/* Settings of one rungekutta3 step (src/FLOWVPM_timeintegration.jl:388-461) */
+typedef struct {
+    vpm_config uj;
+    double f, g;               /* ReformulatedVPM{f,g} (src/FLOWVPM_formulation.jl:31-37) */
+    double Uinf[3];            /* pfield.Uinf(t) */
+    int32_t sfs;               /* 0 NoSFS, 1 DynamicSFS pseudo-3-level (subfilterscale.jl:172-268) */
+    int32_t sfs_force_positive;
+    int32_t sfs_clip;          /* clipping_backscatter */
+    int32_t relax;             /* 0 none, 1 pedrizzetti, 2 correctedpedrizzetti */
+    double sfs_alpha, sfs_rlxf, sfs_minC, sfs_maxC;
+    double rlxf;
+    int32_t viscous;           /* 0 Inviscid, 1 CoreSpreading (viscous.jl:65-223) */
+    int32_t reserved;
+    double nu, sgm0, beta;
+    double t_sgm;              /* core-spreading timer; updated in place */
+} vpm_step_config;
+
+typedef struct vpm_field_s* vpm_field;
+
+/* ---- library ---------------------------------------------------------- */
+const char* vpm_version(void);
+const char* vpm_last_error(void);
+int vpm_device_count(int* count);
+
+/* ---- particle fields: ParticleField storage (particlefield.jl:67-141) --- */
+/* Allocate a field for up to maxparticles particles on `device`. */
+int vpm_create(int device, int64_t maxparticles, vpm_field* out);
+int vpm_destroy(vpm_field f);
+/* Copy np whole 46-row records in (np <= maxparticles).  Rejects sigma <= 0
+ * or non-finite X, Gamma, sigma (the reference divides by sigma, fmm.jl:122). */
+int vpm_set_particles(vpm_field f, const double* particles, int64_t np);
+/* Copy rows [row0, row1) of the first np records out, into a 46-stride array
+ * (other rows of `particles` are left untouched). */
+int vpm_get_particles(vpm_field f, double* particles, int64_t np, int32_t row0, int32_t row1);
+int vpm_get_np(vpm_field f, int64_t* np);
+int vpm_synchronize(vpm_field f);
+
+/* ---- boundary #1: the UJ functor ---------------------------------------- */
+/* UJ_direct(pfield; reset, reset_sfs, sfs) (src/FLOWVPM_UJ.jl:21-37):
+ * reset zeroes U, vorticity, J, PSE of non-static particles
+ * (particlefield.jl:464-490); reset_sfs zeroes SFS of non-static particles
+ * (:492-507); then all-pairs U/J accumulate (fmm.jl:102-176); sfs then adds
+ * E_str (Estr_direct!, subfilterscale_models.jl:43-92). */
+int vpm_uj_direct(vpm_field f, const vpm_config* cfg, uint32_t flags);
+/* UJ_direct(source, target) (src/FLOWVPM_UJ.jl:48-50): every particle of
+ * `source` acts on every particle of `target`; accumulates, no reset.  Both
+ * fields must live on the same device. */
+int vpm_uj_direct_fields(vpm_field target, vpm_field source, const vpm_config* cfg);
+
+/* ---- boundary #2: FastMultipole leaf-pair P2P ---------------------------- */
+/* fmm.direct!(target_buffer, target_index, switch, source_system,
+ * source_buffer, source_index) (src/FLOWVPM_fmm.jl:102-168), batched: for
+ * k < npairs, sources [sb[k], se[k]) act on targets [tb[k], te[k]) of the
+ * same field, accumulated into U and J like buffer_to_target_system!
+ * (fmm.jl:170-176).  Host arrays; ranges must lie within [0, np). */
+int vpm_p2p_ranges(vpm_field f, const vpm_config* cfg,
+                   const int64_t* tb, const int64_t* te,
+                   const int64_t* sb, const int64_t* se, int64_t npairs);
+
+/* ---- terms built from J ------------------------------------------------- */
+/* Estr_direct! (src/FLOWVPM_subfilterscale_models.jl:16-92): SFS += sum over
+ * non-static sources of zeta_sigma(r) (Gamma_s . grad)(U_t - U_s), for
+ * non-static targets. */
+int vpm_estr_direct(vpm_field f, const vpm_config* cfg);
+int vpm_reset_particles(vpm_field f);          /* particlefield.jl:464-490 */
+int vpm_reset_particles_sfs(vpm_field f);      /* particlefield.jl:492-507 */
+
+/* ---- per-particle FP64 schemes ------------------------------------------ */
+/* update_particle_states for ReformulatedVPM (timeintegration.jl:463-532) */
+int vpm_update_particle_states(vpm_field f, double a, double b, double dt,
+                               const double Uinf[3], double fr, double gr,
+                               double zeta0, int32_t transposed);
+/* relax_pedrizzetti / relax_correctedpedrizzetti (relaxation.jl:62-81,117-142) */
+int vpm_relax(vpm_field f, double rlxf, int32_t corrected);
+/* CoreSpreading sigma update, rungekutta3 branch (viscous.jl:170-175) */
+int vpm_core_spreading(vpm_field f, double nu, double a, double b, double dt);
+/* dynamicprocedure_pseudo3level_beforeUJ (subfilterscale.jl:447-539),
+ * including its test-filter UJ + E_str evaluation */
+int vpm_sfs_dynamic_before(vpm_field f, const vpm_config* cfg, double alpha,
+                           double minC, double maxC);
+/* dynamicprocedure_pseudo3level_afterUJ (subfilterscale.jl:541-673) followed
+ * by clipping_backscatter when clip != 0 (subfilterscale.jl:225-247).
+ * Returns VPM_ENAN when a coefficient is NaN (subfilterscale.jl:645-652). */
+int vpm_sfs_dynamic_after(vpm_field f, const vpm_config* cfg, double alpha, double rlxf,
+                          double minC, double maxC, int32_t force_positive, int32_t clip);
+/* One rungekutta3 step for ReformulatedVPM with UJ = UJ_direct
+ * (timeintegration.jl:388-461) and nextstep's U_prev update
+ * (particlefield.jl:443-455).  relax_now mirrors run_vpm!'s relax flag
+ * (utils.jl:94-96).  Returns VPM_ERESET (after completing the step) when
+ * CoreSpreading would reset core sizes (viscous.jl:202-220, not provided). */
+int vpm_rk3_step(vpm_field f, vpm_step_config* s, double dt, int32_t relax_now);
+
+/* ---- adjoint (replaces src/FLOWVPM_rrules.jl:179-330, UJS_pullback) ------ */
+/* For L = sum_i Ubar_i . U_i + sum_i sum_k Jbar_i[k] J_i[k] with U, J from one
+ * all-pairs evaluation of the field (fmm.jl:102-168), writes dL/dX (3 np),
+ * dL/dGamma (3 np) and dL/dsigma (np), particle-major.  Host arrays;
+ * Ubar is 3 np, Jbar 9 np in the J storage order. */
+int vpm_uj_adjoint(vpm_field f, const vpm_config* cfg, const double* Ubar, const double* Jbar,
+                   double* Xbar, double* Gbar, double* Sbar);
+
+/* ---- measurement -------------------------------------------------------- */
+/* When enabled, the library brackets each all-pairs kernel launch with HIP
+ * events on the field's stream.  vpm_timing returns the summed duration (ms)
+ * and the launch count of one kernel class since the last reset:
+ * 0 U/J FP32c, 1 U/J FP64, 2 E_str, 3 adjoint. */
+int vpm_profile(vpm_field f, int32_t enable);
+int vpm_timing(vpm_field f, int32_t which, double* total_ms, int64_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

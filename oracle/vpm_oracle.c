@@ -1,0 +1,684 @@
+/*
+ * vpm_oracle.c -- FP64 CPU restatement of FLOWVPM's particle-to-particle path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library (see the header).
+ * Built with -ffp-contract=off so that every expression rounds as the Julia
+ * reference does (Julia does not fuse a*b+c unless asked to).  Operation
+ * order inside each expression follows the cited reference line, including
+ * Julia's left-to-right evaluation of chained * and /, and x^3 == x*x*x
+ * (Base.literal_pow).
+ *
+ * Pinning (what checks each function; tests/test_oracle.py):
+ *   vpmo_erf64         mpmath erf at 50 digits over all five fdlibm branches
+ *   vpmo_g_dgdr/zeta   mpmath closed forms; g' = 4 pi q^2 zeta identity
+ *   vpmo_g_derivs      mpmath derivative of the closed form g'
+ *   vpmo_direct2       mpmath brute force (N<=64, four kernels), 2-body
+ *                      singular closed form (scripts/check_fmm.jl:39-71),
+ *                      tr J = 0, rotation/scaling/translation invariance
+ *   vpmo_direct_ranges equals vpmo_direct2 over the union of the ranges
+ *   vpmo_estr_direct   mpmath brute force; zero when all J are equal
+ *   vpmo_update_particle_states / vpmo_relax / vpmo_core_spreading_rk3 /
+ *   vpmo_sfs_before / vpmo_sfs_after
+ *                      exact-arithmetic (fractions) restatements on small
+ *                      inputs, plus the single-ring translation speed test
+ *                      of test/runtests_singlevortexring.jl:22-23,143
+ *   vpmo_uj_adjoint    central finite differences of vpmo_loss_local
+ */
+#include "vpm_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Row offsets in the 46-row particle record (particlefield.jl:239-252). */
+enum {
+    RX = 0, RG = 3, RSIG = 6, RVOL = 7, RCIRC = 8, RU = 9, RW = 12, RJ = 15,
+    RPSE = 24, RM = 27, RC = 36, RSFS = 39, RSTATIC = 42, RUPREV = 43
+};
+
+/* src/FLOWVPM.jl:62-66 */
+#define PI_D 3.141592653589793
+static double const1(void) { return 1.0 / pow(2.0 * PI_D, 1.5); }
+static double const2(void) { return sqrt(2.0 / PI_D); }
+static double const3(void) { return 3.0 / (4.0 * PI_D); }
+static double const4(void) { return 1.0 / (4.0 * PI_D); }
+static double sqr2(void) { return sqrt(2.0); }
+
+/* ---- erf: the fdlibm s_erf.c method as restated by custom_erf64
+ *      (src/FLOWVPM_gpu_erf.jl:62-121 coefficients, :159-191 branches). */
+static const double e_erx = 8.45062911510467529297e-01;
+static const double e_pp[5] = { 1.28379167095512558561e-01, -3.25042107247001499370e-01,
+    -2.84817495755985104766e-02, -5.77027029648944159157e-03, -2.37630166566501626084e-05 };
+static const double e_qq[5] = { 3.97917223959155352819e-01, 6.50222499887672944485e-02,
+    5.08130628187576562776e-03, 1.32494738004321644526e-04, -3.96022827877536812320e-06 };
+static const double e_pa[7] = { -2.36211856075265944077e-03, 4.14856118683748331666e-01,
+    -3.72207876035701323847e-01, 3.18346619901161753674e-01, -1.10894694282396677476e-01,
+    3.54783043256182359371e-02, -2.16637559486879084300e-03 };
+static const double e_qa[6] = { 1.06420880400844228286e-01, 5.40397917702171048937e-01,
+    7.18286544141962662868e-02, 1.26171219808761642112e-01, 1.36370839120290507362e-02,
+    1.19844998467991074170e-02 };
+static const double e_ra[8] = { -9.86494403484714822705e-03, -6.93858572707181764372e-01,
+    -1.05586262253232909814e+01, -6.23753324503260060396e+01, -1.62396669462573470355e+02,
+    -1.84605092906711035994e+02, -8.12874355063065934246e+01, -9.81432934416914548592e+00 };
+static const double e_sa[8] = { 1.96512716674392571292e+01, 1.37657754143519042600e+02,
+    4.34565877475229228821e+02, 6.45387271733267880336e+02, 4.29008140027567833386e+02,
+    1.08635005541779435134e+02, 6.57024977031928170135e+00, -6.04244152148580987438e-02 };
+static const double e_rb[7] = { -9.86494292470009928597e-03, -7.99283237680523006574e-01,
+    -1.77579549177547519889e+01, -1.60636384855821916062e+02, -6.37566443368389627722e+02,
+    -1.02509513161107724954e+03, -4.83519191608651397019e+02 };
+static const double e_sb[7] = { 3.03380607434824582924e+01, 3.25792512996573918826e+02,
+    1.53672958608443695994e+03, 3.19985821950859553908e+03, 2.55305040643316442583e+03,
+    4.74528541206955367215e+02, -2.24409524465858183362e+01 };
+
+/* Julia's sign(): +-1 for nonzero, the argument itself for +-0 and NaN. */
+static double jl_sign(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : x); }
+
+double vpmo_erf64(double x)
+{
+    const double xabs = fabs(x), sgn = jl_sign(x), one = 1.0;
+    double val = sgn * one;
+    if (xabs < 0.84375) {
+        double z = x * x;
+        double r = e_pp[0] + z * (e_pp[1] + z * (e_pp[2] + z * (e_pp[3] + z * e_pp[4])));
+        double s = one + z * (e_qq[0] + z * (e_qq[1] + z * (e_qq[2] + z * (e_qq[3] + z * e_qq[4]))));
+        double y = r / s;
+        val = sgn * (xabs + xabs * y);
+    } else if (xabs < 1.25) {
+        double s = xabs - one;
+        double P = e_pa[0] + s * (e_pa[1] + s * (e_pa[2] + s * (e_pa[3] + s * (e_pa[4] + s * (e_pa[5] + s * e_pa[6])))));
+        double Q = one + s * (e_qa[0] + s * (e_qa[1] + s * (e_qa[2] + s * (e_qa[3] + s * (e_qa[4] + s * e_qa[5])))));
+        val = sgn * (e_erx + P / Q);
+    } else if (xabs < 2.857142857142857) {
+        double s = one / (x * x);
+        double R = e_ra[0] + s * (e_ra[1] + s * (e_ra[2] + s * (e_ra[3] + s * (e_ra[4] + s * (e_ra[5] + s * (e_ra[6] + s * e_ra[7]))))));
+        double S = one + s * (e_sa[0] + s * (e_sa[1] + s * (e_sa[2] + s * (e_sa[3] + s * (e_sa[4] + s * (e_sa[5] + s * (e_sa[6] + s * e_sa[7])))))));
+        double r = exp(-x * x - 0.5625 + R / S);
+        val = sgn * (one - r / xabs);
+    } else if (xabs < 6.0) {
+        double s = one / (x * x);
+        double R = e_rb[0] + s * (e_rb[1] + s * (e_rb[2] + s * (e_rb[3] + s * (e_rb[4] + s * (e_rb[5] + s * e_rb[6])))));
+        double S = one + s * (e_sb[0] + s * (e_sb[1] + s * (e_sb[2] + s * (e_sb[3] + s * (e_sb[4] + s * (e_sb[5] + s * e_sb[6]))))));
+        double r = exp(-x * x - 0.5625 + R / S);
+        val = sgn * (one - r / xabs);
+    }
+    return val;
+}
+
+/* ---- regularization kernels (src/FLOWVPM_kernel.jl:45-84) ---- */
+void vpmo_g_dgdr(int kernel, double r, double* g, double* dg)
+{
+    switch (kernel) {
+    case VPMO_SINGULAR:                                   /* kernel.jl:48 */
+        *g = 1.0; *dg = 0.0; return;
+    case VPMO_GAUSSIAN: {                                 /* kernel.jl:63-66 */
+        double aux = exp(-r * r * r);
+        *g = 1 - aux; *dg = 3 * r * r * aux; return;
+    }
+    case VPMO_GAUSSIANERF: {                              /* kernel.jl:54-57 */
+        double aux = const2() * r * exp(-r * r / 2);
+        *g = vpmo_erf64(r / sqr2()) - aux; *dg = r * aux; return;
+    }
+    case VPMO_WINCKELMANS: {                              /* kernel.jl:77-84 */
+        double aux0 = r * r + 1;
+        double aux02 = aux0 * aux0;
+        aux0 = aux02 * aux02 * aux0;
+        aux0 = sqrt(aux0);
+        *g = r * r * r * (r * r + 2.5) / aux0;
+        *dg = 7.5 * r * r / (aux0 * (r * r + 1));
+        return;
+    }
+    }
+    *g = NAN; *dg = NAN;
+}
+
+double vpmo_zeta(int kernel, double r)
+{
+    switch (kernel) {
+    case VPMO_SINGULAR: return r == 0 ? 1.0 : 0.0;                 /* kernel.jl:45 */
+    case VPMO_GAUSSIAN: return const3() * exp(-r * r * r);         /* kernel.jl:60 */
+    case VPMO_GAUSSIANERF: return const1() * exp(-r * r / 2);      /* kernel.jl:51 */
+    case VPMO_WINCKELMANS: {                                       /* kernel.jl:69-74 */
+        double temp = r * r + 1;
+        double temp2 = temp * temp * temp;
+        temp = temp2 * temp2 * temp;
+        return const4() * 7.5 / sqrt(temp);
+    }
+    }
+    return NAN;
+}
+
+/* g, g' and g'' (the second derivative is not in the reference; it is the
+ * derivative of the closed-form dgdr_* of kernel.jl:47,53,62,76). */
+void vpmo_g_derivs(int kernel, double q, double* g, double* dg, double* d2g)
+{
+    vpmo_g_dgdr(kernel, q, g, dg);
+    switch (kernel) {
+    case VPMO_SINGULAR: *d2g = 0.0; return;
+    case VPMO_GAUSSIAN: *d2g = (6 * q - 9 * q * q * q * q) * exp(-q * q * q); return;
+    case VPMO_GAUSSIANERF: *d2g = const2() * (2 * q - q * q * q) * exp(-q * q / 2); return;
+    case VPMO_WINCKELMANS: {
+        double t = q * q + 1;
+        *d2g = 7.5 * q * (2 - 5 * q * q) / pow(t, 4.5);
+        return;
+    }
+    }
+    *d2g = NAN;
+}
+
+/* One source -> one target pair, the body of fmm.jl:113-161.  Adds into
+ * acc[0..2] (U) and acc[3..11] (J, column-major). */
+static inline void pair_uj(int kernel, double tx, double ty, double tz,
+                           const double* src, double* acc)
+{
+    const double c4 = const4();
+    const double gamma_x = src[RG], gamma_y = src[RG + 1], gamma_z = src[RG + 2];
+    const double sigma = src[RSIG];
+    double dx = tx - src[RX], dy = ty - src[RX + 1], dz = tz - src[RX + 2];
+    double r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 == 0.0) return;                                   /* fmm.jl:118 */
+    double r = sqrt(r2);
+    double g_sgm, dg_sgmdr;
+    vpmo_g_dgdr(kernel, r / sigma, &g_sgm, &dg_sgmdr);
+    double r3inv = 1.0 / (r2 * r);
+    double crss1 = -c4 * r3inv * (dy * gamma_z - dz * gamma_y);
+    double crss2 = -c4 * r3inv * (dz * gamma_x - dx * gamma_z);
+    double crss3 = -c4 * r3inv * (dx * gamma_y - dy * gamma_x);
+    acc[0] += g_sgm * crss1;
+    acc[1] += g_sgm * crss2;
+    acc[2] += g_sgm * crss3;
+    double aux = dg_sgmdr / (sigma * r) - 3 * g_sgm / r2;
+    double aux2 = -c4 * g_sgm * r3inv;
+    acc[3 + 0] += aux * crss1 * dx;
+    acc[3 + 1] += aux * crss2 * dx - aux2 * gamma_z;
+    acc[3 + 2] += aux * crss3 * dx + aux2 * gamma_y;
+    acc[3 + 3] += aux * crss1 * dy + aux2 * gamma_z;
+    acc[3 + 4] += aux * crss2 * dy;
+    acc[3 + 5] += aux * crss3 * dy - aux2 * gamma_x;
+    acc[3 + 6] += aux * crss1 * dz - aux2 * gamma_y;
+    acc[3 + 7] += aux * crss2 * dz + aux2 * gamma_x;
+    acc[3 + 8] += aux * crss3 * dz;
+}
+
+static void add_acc(double* tp, const double* acc)
+{
+    for (int k = 0; k < 3; k++) tp[RU + k] += acc[k];            /* fmm.jl:171 */
+    for (int k = 0; k < 9; k++) tp[RJ + k] += acc[3 + k];        /* fmm.jl:172-175 */
+}
+
+void vpmo_direct2(double* Pt, int64_t st, int64_t t0, int64_t t1,
+                  const double* Ps, int64_t ss, int64_t s0, int64_t s1,
+                  int kernel, int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
+    for (int64_t i = t0; i < t1; i++) {
+        double* tp = Pt + i * st;
+        double acc[12] = {0};
+        const double tx = tp[RX], ty = tp[RX + 1], tz = tp[RX + 2];
+        for (int64_t j = s0; j < s1; j++) pair_uj(kernel, tx, ty, tz, Ps + j * ss, acc);
+        add_acc(tp, acc);
+    }
+}
+
+void vpmo_direct_ranges(double* P, int64_t stride, int kernel,
+                        const int64_t* tb, const int64_t* te,
+                        const int64_t* sb, const int64_t* se, int64_t npairs)
+{
+    /* buffer over every particle touched, summed in list order */
+    int64_t lo = -1, hi = -1;
+    for (int64_t k = 0; k < npairs; k++) {
+        if (lo < 0 || tb[k] < lo) lo = tb[k];
+        if (te[k] > hi) hi = te[k];
+    }
+    if (lo < 0 || hi <= lo) return;
+    double* buf = (double*)calloc((size_t)(hi - lo) * 12, sizeof(double));
+    /* read targets' positions before any write: buffers are added at the end */
+    for (int64_t k = 0; k < npairs; k++)
+        for (int64_t i = tb[k]; i < te[k]; i++) {
+            const double* tp = P + i * stride;
+            for (int64_t j = sb[k]; j < se[k]; j++)
+                pair_uj(kernel, tp[RX], tp[RX + 1], tp[RX + 2], P + j * stride, buf + (i - lo) * 12);
+        }
+    for (int64_t i = lo; i < hi; i++) add_acc(P + i * stride, buf + (i - lo) * 12);
+    free(buf);
+}
+
+static int is_static(const double* p) { return p[RSTATIC] != 0; }
+
+void vpmo_reset_particles(double* P, int64_t np)
+{
+    for (int64_t i = 0; i < np; i++) {
+        double* p = P + i * VPMO_NFIELDS;
+        if (p[RSTATIC] == 0)                                   /* particlefield.jl:468 */
+            for (int k = RU; k < RPSE + 3; k++) p[k] = 0.0;    /* U, vorticity, J, PSE */
+    }
+}
+
+void vpmo_reset_particles_sfs(double* P, int64_t np)
+{
+    for (int64_t i = 0; i < np; i++) {
+        double* p = P + i * VPMO_NFIELDS;
+        if (p[RSTATIC] == 0)
+            for (int k = 0; k < 3; k++) p[RSFS + k] = 0.0;
+    }
+}
+
+/* subfilterscale_models.jl:16-41 for one (target, source) pair */
+static inline void estr_pair(int kernel, int transposed, double* tp, const double* sp, double r)
+{
+    const double* GS = sp + RG;
+    const double* JS = sp + RJ;
+    const double* JT = tp + RJ;
+    double S1, S2, S3;
+    if (transposed) {
+        S1 = (JT[0] - JS[0]) * GS[0] + (JT[1] - JS[1]) * GS[1] + (JT[2] - JS[2]) * GS[2];
+        S2 = (JT[3] - JS[3]) * GS[0] + (JT[4] - JS[4]) * GS[1] + (JT[5] - JS[5]) * GS[2];
+        S3 = (JT[6] - JS[6]) * GS[0] + (JT[7] - JS[7]) * GS[1] + (JT[8] - JS[8]) * GS[2];
+    } else {
+        S1 = (JT[0] - JS[0]) * GS[0] + (JT[3] - JS[3]) * GS[1] + (JT[6] - JS[6]) * GS[2];
+        S2 = (JT[1] - JS[1]) * GS[0] + (JT[4] - JS[4]) * GS[1] + (JT[7] - JS[7]) * GS[2];
+        S3 = (JT[2] - JS[2]) * GS[0] + (JT[5] - JS[5]) * GS[1] + (JT[8] - JS[8]) * GS[2];
+    }
+    double sigma_inv = 1.0 / sp[RSIG];
+    double zeta_sgm = vpmo_zeta(kernel, r * sigma_inv) * sigma_inv * sigma_inv * sigma_inv;
+    tp[RSFS + 0] += zeta_sgm * S1;
+    tp[RSFS + 1] += zeta_sgm * S2;
+    tp[RSFS + 2] += zeta_sgm * S3;
+}
+
+void vpmo_estr_direct(double* P, int64_t np, int kernel, int transposed, int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+    /* The SFS rows of a target are only written by that target's iteration,
+     * and J/Gamma/X are only read, so targets are independent. */
+#pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
+    for (int64_t i = 0; i < np; i++) {
+        double* tp = P + i * VPMO_NFIELDS;
+        if (is_static(tp)) continue;                                /* :63 */
+        const double tx = tp[RX], ty = tp[RX + 1], tz = tp[RX + 2];
+        for (int64_t j = 0; j < np; j++) {
+            const double* sp = P + j * VPMO_NFIELDS;
+            if (is_static(sp)) continue;                            /* iterator() */
+            double dx = sp[RX] - tx, dy = sp[RX + 1] - ty, dz = sp[RX + 2] - tz;
+            double r = sqrt(dx * dx + dy * dy + dz * dz);
+            estr_pair(kernel, transposed, tp, sp, r);
+        }
+    }
+}
+
+void vpmo_uj_direct(double* P, int64_t np, int kernel, int reset, int reset_sfs,
+                    int sfs, int transposed, int nthreads)
+{
+    if (reset) vpmo_reset_particles(P, np);                 /* UJ.jl:28-30 */
+    if (reset_sfs) vpmo_reset_particles_sfs(P, np);         /* UJ.jl:31-33 */
+    vpmo_direct2(P, VPMO_NFIELDS, 0, np, P, VPMO_NFIELDS, 0, np, kernel, nthreads);
+    if (sfs) vpmo_estr_direct(P, np, kernel, transposed, nthreads);   /* UJ.jl:36 */
+}
+
+/* ---- per-particle schemes ---- */
+static inline double cube(double x) { return x * x * x; }
+
+static inline void stretching(const double* J, const double* G, int transposed, double S[3])
+{
+    if (transposed) {                                   /* timeintegration.jl:489-493 */
+        S[0] = J[0] * G[0] + J[1] * G[1] + J[2] * G[2];
+        S[1] = J[3] * G[0] + J[4] * G[1] + J[5] * G[2];
+        S[2] = J[6] * G[0] + J[7] * G[1] + J[8] * G[2];
+    } else {                                            /* :494-499 */
+        S[0] = J[0] * G[0] + J[3] * G[1] + J[6] * G[2];
+        S[1] = J[1] * G[0] + J[4] * G[1] + J[7] * G[2];
+        S[2] = J[2] * G[0] + J[5] * G[1] + J[8] * G[2];
+    }
+}
+
+void vpmo_update_particle_states(double* P, int64_t np, double a, double b, double dt,
+                                 const double Uinf[3], double f, double g,
+                                 double zeta0, int transposed)
+{
+    for (int64_t i = 0; i < np; i++) {
+        double* p = P + i * VPMO_NFIELDS;
+        if (is_static(p)) continue;                                /* :472 */
+        double C = p[RC];
+        double* M = p + RM;   /* M[k-1] here is M[k] in the reference */
+        double* G = p + RG;
+        const double* J = p + RJ;
+        M[0] = a * M[0] + dt * (p[RU + 0] + Uinf[0]);
+        M[1] = a * M[1] + dt * (p[RU + 1] + Uinf[1]);
+        M[2] = a * M[2] + dt * (p[RU + 2] + Uinf[2]);
+        p[RX + 0] += b * M[0];
+        p[RX + 1] += b * M[1];
+        p[RX + 2] += b * M[2];
+        double MM[3];
+        stretching(J, G, transposed, MM);
+        double Gnorm2 = G[0] * G[0] + G[1] * G[1] + G[2] * G[2];
+        double MM4;
+        double sgm = p[RSIG];
+        if (Gnorm2 > 0) {                                         /* :503-509 */
+            MM4 = (f + g) / (1 + 3 * f) * (MM[0] * G[0] + MM[1] * G[1] + MM[2] * G[2]);
+            MM4 -= f / (1 + 3 * f) * (C * p[RSFS] * G[0] + C * p[RSFS + 1] * G[1] + C * p[RSFS + 2] * G[2]) * cube(sgm) / zeta0;
+            MM4 /= Gnorm2;
+        } else {
+            MM4 = 0.0;
+        }
+        M[3] = a * M[3] + dt * (MM[0] - 3 * MM4 * G[0] - C * p[RSFS + 0] * cube(sgm) / zeta0);
+        M[4] = a * M[4] + dt * (MM[1] - 3 * MM4 * G[1] - C * p[RSFS + 1] * cube(sgm) / zeta0);
+        M[5] = a * M[5] + dt * (MM[2] - 3 * MM4 * G[2] - C * p[RSFS + 2] * cube(sgm) / zeta0);
+        M[7] = a * M[7] - dt * (sgm * MM4);                       /* :518 */
+        G[0] += b * M[3];
+        G[1] += b * M[4];
+        G[2] += b * M[5];
+        p[RSIG] += b * M[7];                                      /* :526 */
+    }
+}
+
+void vpmo_relax(double* P, int64_t np, double rlxf, int corrected)
+{
+    for (int64_t i = 0; i < np; i++) {
+        double* p = P + i * VPMO_NFIELDS;
+        if (is_static(p)) continue;                   /* timeintegration.jl:447 */
+        const double* J = p + RJ;
+        double* G = p + RG;
+        double nrmw = sqrt((J[5] - J[7]) * (J[5] - J[7]) +
+                           (J[6] - J[2]) * (J[6] - J[2]) +
+                           (J[1] - J[3]) * (J[1] - J[3]));
+        if (nrmw != 0) {
+            double nrmGamma = sqrt(G[0] * G[0] + G[1] * G[1] + G[2] * G[2]);
+            double b2 = 0;
+            if (corrected)                              /* relaxation.jl:129-131 */
+                b2 = 1 - 2 * (1 - rlxf) * rlxf * (1 - (G[0] * (J[5] - J[7]) +
+                                                     G[1] * (J[6] - J[2]) +
+                                                     G[2] * (J[1] - J[3])) / (nrmGamma * nrmw));
+            G[0] = (1 - rlxf) * G[0] + rlxf * nrmGamma * (J[5] - J[7]) / nrmw;
+            G[1] = (1 - rlxf) * G[1] + rlxf * nrmGamma * (J[6] - J[2]) / nrmw;
+            G[2] = (1 - rlxf) * G[2] + rlxf * nrmGamma * (J[1] - J[3]) / nrmw;
+            if (corrected) {
+                double sb2 = sqrt(b2);
+                G[0] /= sb2; G[1] /= sb2; G[2] /= sb2;
+            }
+        }
+    }
+}
+
+int vpmo_core_spreading_rk3(double* P, int64_t np, double nu, double sgm0, double beta,
+                            double* t_sgm, double a, double b, double dt)
+{
+    for (int64_t i = 0; i < np; i++) {
+        double* p = P + i * VPMO_NFIELDS;
+        if (is_static(p)) continue;                         /* iterator() */
+        p[RM + 6] = a * p[RM + 6] + dt * 2 * nu;            /* viscous.jl:173 */
+        p[RSIG] = sqrt(p[RSIG] * p[RSIG] + b * p[RM + 6]);  /* viscous.jl:174 */
+    }
+    if (fabs(b - 8.0 / 15.0) <= 1e-7) {                     /* viscous.jl:178 */
+        *t_sgm += dt;                                       /* :191 */
+        double beta_cur = sqrt(2 * nu * (*t_sgm) / (sgm0 * sgm0) + 1);
+        if (beta_cur >= beta) return 1;                     /* :202 */
+    }
+    return 0;
+}
+
+static int check_bounds(double minC, double maxC)
+{
+    if (minC < 0 || maxC < 0 || minC > maxC) return -1;     /* subfilterscale.jl:455-461 */
+    return 0;
+}
+
+int vpmo_sfs_before(double* P, int64_t np, int kernel, int transposed, double alpha,
+                    double minC, double maxC, int nthreads)
+{
+    if (check_bounds(minC, maxC)) return -1;
+    for (int64_t i = 0; i < np; i++) {                      /* :465-475 */
+        double* p = P + i * VPMO_NFIELDS;
+        if (is_static(p)) continue;
+        p[RSIG] *= alpha;
+    }
+    vpmo_uj_direct(P, np, kernel, 1, 1, 1, transposed, nthreads);   /* :478 */
+    for (int64_t i = 0; i < np; i++) {                      /* :481-521 */
+        double* p = P + i * VPMO_NFIELDS;
+        if (is_static(p)) continue;
+        for (int k = 0; k < 9; k++) p[RM + k] = 0.0;
+        double S[3];
+        stretching(p + RJ, p + RG, transposed, S);
+        p[RM + 0] = S[0]; p[RM + 1] = S[1]; p[RM + 2] = S[2];
+        p[RM + 3] = p[RSFS]; p[RM + 4] = p[RSFS + 1]; p[RM + 5] = p[RSFS + 2];
+    }
+    for (int64_t i = 0; i < np; i++) {                      /* :526-536 */
+        double* p = P + i * VPMO_NFIELDS;
+        if (is_static(p)) continue;
+        p[RSIG] /= alpha;
+    }
+    return 0;
+}
+
+int vpmo_sfs_after(double* P, int64_t np, int kernel, int transposed, double alpha,
+                   double rlxf, double minC, double maxC, int force_positive, int clip)
+{
+    if (check_bounds(minC, maxC)) return -1;
+    for (int64_t i = 0; i < np; i++) {                      /* :559-586 */
+        double* p = P + i * VPMO_NFIELDS;
+        if (is_static(p)) continue;
+        double* M = p + RM;
+        const double* J = p + RJ;
+        const double* G = p + RG;
+        if (transposed) {
+            M[0] -= J[0] * G[0] + J[1] * G[1] + J[2] * G[2];
+            M[1] -= J[3] * G[0] + J[4] * G[1] + J[5] * G[2];
+            M[2] -= J[6] * G[0] + J[7] * G[1] + J[8] * G[2];
+        } else {
+            M[0] -= J[0] * G[0] + J[3] * G[1] + J[6] * G[2];
+            M[1] -= J[1] * G[0] + J[4] * G[1] + J[7] * G[2];
+            M[2] -= J[2] * G[0] + J[5] * G[1] + J[8] * G[2];
+        }
+        M[3] -= p[RSFS]; M[4] -= p[RSFS + 1]; M[5] -= p[RSFS + 2];
+    }
+    const double zeta0 = vpmo_zeta(kernel, 0.0);            /* :590 */
+    int status = 0;
+    for (int64_t i = 0; i < np; i++) {                      /* :592-656 */
+        double* p = P + i * VPMO_NFIELDS;
+        if (is_static(p)) continue;
+        const double* M = p + RM;
+        double* C = p + RC;
+        const double* G = p + RG;
+        double nume = M[0] * G[0] + M[1] * G[1] + M[2] * G[2];
+        nume *= 3 * alpha - 2;
+        double deno = M[3] * G[0] + M[4] * G[1] + M[5] * G[2];
+        deno /= zeta0 / cube(p[RSIG]);
+        if (C[2] == 0) {
+            C[2] = deno;
+            if (C[2] == 0) C[2] = 2.220446049250313e-16;    /* eps() */
+        }
+        nume = rlxf * nume + (1 - rlxf) * C[1];
+        deno = rlxf * deno + (1 - rlxf) * C[2];
+        if (fabs(nume / deno) > maxC) {
+            if (fabs(deno) < fabs(C[2])) deno = jl_sign(deno) * fabs(C[2]);
+            if (fabs(nume / deno) >= maxC) nume = jl_sign(nume) * fabs(deno) * maxC;
+        } else if (fabs(nume / deno) < minC) {
+            nume = jl_sign(nume) * fabs(deno) * minC;
+        }
+        C[1] = nume;
+        C[2] = deno;
+        C[0] = C[1] / C[2];
+        if (isnan(C[0])) status = -2;                       /* :645-652 */
+        if (force_positive) C[0] *= jl_sign(C[0]);          /* :655 */
+    }
+    for (int64_t i = 0; i < np; i++) {                      /* :659-670 */
+        double* p = P + i * VPMO_NFIELDS;
+        if (is_static(p)) continue;
+        for (int k = 0; k < 9; k++) p[RM + k] = 0.0;
+    }
+    if (status) return status;
+    if (clip) {                                             /* :225-247, 287-296 */
+        for (int64_t i = 0; i < np; i++) {
+            double* p = P + i * VPMO_NFIELDS;
+            if (is_static(p)) continue;
+            double Cc = p[RC];
+            if (Cc * (p[RG] * p[RSFS] + p[RG + 1] * p[RSFS + 1] + p[RG + 2] * p[RSFS + 2]) < 0)
+                p[RC] *= 0;
+        }
+    }
+    return 0;
+}
+
+int vpmo_rk3_step(double* P, int64_t np, vpmo_settings* s, double dt, int relax_now)
+{
+    const double zeta0 = vpmo_zeta(s->kernel, 0.0);                 /* :398 */
+    const int sfs_on = s->sfs != 0;
+    int status = 0;
+    for (int64_t i = 0; i < np; i++) {                              /* :401-414 */
+        double* p = P + i * VPMO_NFIELDS;
+        if (p[RSTATIC] == 0) for (int k = 0; k < 9; k++) p[RM + k] = 0.0;
+    }
+    const double ab[3][2] = { {0.0, 1.0 / 3.0}, {-5.0 / 9.0, 15.0 / 16.0},
+                              {-153.0 / 128.0, 8.0 / 15.0} };       /* :417 */
+    for (int st = 0; st < 3; st++) {
+        double a = ab[st][0], b = ab[st][1];
+        int first = (a == 1 || a == 0);                             /* subfilterscale.jl:207 */
+        if (sfs_on && first) {
+            int rc = vpmo_sfs_before(P, np, s->kernel, s->transposed, s->sfs_alpha,
+                                     s->sfs_minC, s->sfs_maxC, s->nthreads);
+            if (rc) return rc;
+        }
+        vpmo_uj_direct(P, np, s->kernel, 1, sfs_on, sfs_on, s->transposed, s->nthreads);  /* :422 */
+        if (sfs_on && first) {
+            int rc = vpmo_sfs_after(P, np, s->kernel, s->transposed, s->sfs_alpha, s->sfs_rlxf,
+                                    s->sfs_minC, s->sfs_maxC, s->sfs_force_positive, s->sfs_clip);
+            if (rc) return rc;
+        }
+        vpmo_update_particle_states(P, np, a, b, dt, s->Uinf, s->f, s->g, zeta0, s->transposed);
+        if (s->viscous == 1) {
+            if (vpmo_core_spreading_rk3(P, np, s->nu, s->sgm0, s->beta, &s->t_sgm, a, b, dt))
+                status = 1;
+        }
+    }
+    if (relax_now && s->relax) {                                    /* :437-458 */
+        vpmo_reset_particles(P, np);
+        vpmo_uj_direct(P, np, s->kernel, 1, 0, 0, s->transposed, s->nthreads);
+        vpmo_relax(P, np, s->rlxf, s->relax == 2);
+    }
+    /* nextstep: U_prev = |U| for every particle (particlefield.jl:443-455) */
+    for (int64_t i = 0; i < np; i++) {
+        double* p = P + i * VPMO_NFIELDS;
+        p[RUPREV] = sqrt(p[RU] * p[RU] + p[RU + 1] * p[RU + 1] + p[RU + 2] * p[RU + 2]);
+    }
+    return status;
+}
+
+/* ---- adjoint ----
+ * Pair contribution  l = F (A + C) + H B  with d = x_i - x_j, v = d x Gamma_j,
+ * A = ubar . v, B = v . (Jbar d), C = tau . Gamma_j,
+ * tau = (Jbar_23 - Jbar_32, Jbar_31 - Jbar_13, Jbar_12 - Jbar_21),
+ * F = -c g/r^3, H = F'(r)/r, K = H'(r)/r, F_s = dF/dsigma, H_s = dH/dsigma.
+ * (Derived from the forward expressions of fmm.jl:125-161.) */
+typedef struct { double F, H, K, Fs, Hs; } pair_coef;
+
+static inline pair_coef coefs(int kernel, double r, double sigma)
+{
+    const double c = const4();
+    double q = r / sigma, g, dg, d2g;
+    vpmo_g_derivs(kernel, q, &g, &dg, &d2g);
+    double r2 = r * r, r3 = r2 * r, r5 = r3 * r2, r7 = r5 * r2;
+    pair_coef k;
+    k.F = -c * g / r3;
+    k.H = -c * (q * dg - 3 * g) / r5;
+    k.K = -c * (q * q * d2g - 7 * q * dg + 15 * g) / r7;
+    k.Fs = c * dg * q / (sigma * r3);
+    k.Hs = c * q * (q * d2g - 2 * dg) / (sigma * r5);
+    return k;
+}
+
+static inline void cross3(const double* a, const double* b, double* o)
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* Jbar flat f = 3b + a is the cotangent of J[a][b]. */
+static inline void pair_adjoint(int kernel, const double* ti, const double* sj,
+                                const double* ub, const double* Jb,
+                                double gd[3], double gG[3], double* gs)
+{
+    double d[3] = { ti[RX] - sj[RX], ti[RX + 1] - sj[RX + 1], ti[RX + 2] - sj[RX + 2] };
+    double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    gd[0] = gd[1] = gd[2] = gG[0] = gG[1] = gG[2] = 0; *gs = 0;
+    if (r2 == 0) return;
+    double r = sqrt(r2);
+    const double* G = sj + RG;
+    pair_coef k = coefs(kernel, r, sj[RSIG]);
+    double v[3]; cross3(d, G, v);
+    double w[3], JTv[3];
+    for (int a = 0; a < 3; a++) {
+        w[a] = Jb[a] * d[0] + Jb[3 + a] * d[1] + Jb[6 + a] * d[2];
+        JTv[a] = Jb[3 * a + 0] * v[0] + Jb[3 * a + 1] * v[1] + Jb[3 * a + 2] * v[2];
+    }
+    double tau[3] = { Jb[3 * 2 + 1] - Jb[3 * 1 + 2], Jb[3 * 0 + 2] - Jb[3 * 2 + 0],
+                      Jb[3 * 1 + 0] - Jb[3 * 0 + 1] };
+    double A = ub[0] * v[0] + ub[1] * v[1] + ub[2] * v[2];
+    double B = v[0] * w[0] + v[1] * w[1] + v[2] * w[2];
+    double C = tau[0] * G[0] + tau[1] * G[1] + tau[2] * G[2];
+    double Gxu[3], Gxw[3], uxd[3], wxd[3];
+    cross3(G, ub, Gxu); cross3(G, w, Gxw); cross3(ub, d, uxd); cross3(w, d, wxd);
+    double s = k.H * (A + C) + k.K * B;
+    for (int a = 0; a < 3; a++) {
+        gd[a] = s * d[a] + k.F * Gxu[a] + k.H * (Gxw[a] + JTv[a]);
+        gG[a] = k.F * (uxd[a] + tau[a]) + k.H * wxd[a];
+    }
+    *gs = k.Fs * (A + C) + k.Hs * B;
+}
+
+void vpmo_uj_adjoint(const double* P, int64_t np, int kernel,
+                     const double* Ubar, const double* Jbar,
+                     double* Xbar, double* Gbar, double* Sbar, int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+    /* target role: d(l)/d(x_i) = +gd */
+#pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
+    for (int64_t i = 0; i < np; i++) {
+        const double* ti = P + i * VPMO_NFIELDS;
+        double ax[3] = {0, 0, 0};
+        for (int64_t j = 0; j < np; j++) {
+            double gd[3], gG[3], gs;
+            pair_adjoint(kernel, ti, P + j * VPMO_NFIELDS, Ubar + 3 * i, Jbar + 9 * i, gd, gG, &gs);
+            ax[0] += gd[0]; ax[1] += gd[1]; ax[2] += gd[2];
+        }
+        Xbar[3 * i] = ax[0]; Xbar[3 * i + 1] = ax[1]; Xbar[3 * i + 2] = ax[2];
+    }
+    /* source role: d(l)/d(x_j) = -gd, plus Gamma_j and sigma_j */
+#pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
+    for (int64_t j = 0; j < np; j++) {
+        const double* sj = P + j * VPMO_NFIELDS;
+        double ax[3] = {0, 0, 0}, aG[3] = {0, 0, 0}, as = 0;
+        for (int64_t i = 0; i < np; i++) {
+            double gd[3], gG[3], gs;
+            pair_adjoint(kernel, P + i * VPMO_NFIELDS, sj, Ubar + 3 * i, Jbar + 9 * i, gd, gG, &gs);
+            for (int a = 0; a < 3; a++) { ax[a] -= gd[a]; aG[a] += gG[a]; }
+            as += gs;
+        }
+        for (int a = 0; a < 3; a++) { Xbar[3 * j + a] += ax[a]; Gbar[3 * j + a] = aG[a]; }
+        Sbar[j] = as;
+    }
+}
+
+static long double pair_loss(int kernel, const double* ti, const double* sj,
+                             const double* ub, const double* Jb)
+{
+    double acc[12] = {0};
+    pair_uj(kernel, ti[RX], ti[RX + 1], ti[RX + 2], sj, acc);
+    long double l = 0;
+    for (int a = 0; a < 3; a++) l += (long double)ub[a] * acc[a];
+    for (int f = 0; f < 9; f++) l += (long double)Jb[f] * acc[3 + f];
+    return l;
+}
+
+double vpmo_loss_local(const double* P, int64_t np, int kernel,
+                       const double* Ubar, const double* Jbar, int64_t k)
+{
+    long double L = 0;
+    const double* pk = P + k * VPMO_NFIELDS;
+    for (int64_t j = 0; j < np; j++)
+        if (j != k) L += pair_loss(kernel, pk, P + j * VPMO_NFIELDS, Ubar + 3 * k, Jbar + 9 * k);
+    for (int64_t i = 0; i < np; i++)
+        if (i != k) L += pair_loss(kernel, P + i * VPMO_NFIELDS, pk, Ubar + 3 * i, Jbar + 9 * i);
+    return (double)L;
+}

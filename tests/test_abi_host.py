@@ -1,0 +1,128 @@
+"""CPU-only checks of the C-ABI library and the host-side logic.
+
+No call here launches GPU work: the library is loaded, its exports are
+compared with include/flowvpm_hip.h, and only calls that fail before any
+device work (argument checks, no device present) are made.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+
+def test_library_exports_header(vpm):
+    L = vpm.api.lib()
+    names = vpm._lib.header_symbols()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+    assert L.vpm_version().startswith(b"flowvpm_hip")
+
+
+def test_error_paths_without_device(vpm):
+    L = vpm.api.lib()
+    assert L.vpm_device_count(None) == vpm._lib.VPM_EINVAL
+    assert b"null" in L.vpm_last_error()
+    h = C.c_void_p()
+    assert L.vpm_create(0, 0, C.byref(h)) == vpm._lib.VPM_EINVAL
+    assert L.vpm_create(0, 10, None) == vpm._lib.VPM_EINVAL
+    if vpm._lib.device_count() == 0:
+        assert L.vpm_create(0, 10, C.byref(h)) == vpm._lib.VPM_EDEVICE
+    assert L.vpm_destroy(None) == vpm._lib.VPM_OK
+    assert L.vpm_uj_direct(None, None, 0) == vpm._lib.VPM_EINVAL
+    assert L.vpm_get_np(None, None) == vpm._lib.VPM_EINVAL
+
+
+def test_no_cpu_fallback(vpm, tmp_path, monkeypatch):
+    """The product path fails loudly when the library is missing."""
+    monkeypatch.setattr(vpm._lib, "_lib", None)
+    monkeypatch.setattr(vpm._lib, "LIB_PATH", str(tmp_path / "missing.so"))
+    with pytest.raises(OSError):
+        vpm._lib.lib()
+
+
+def test_particle_layout(vpm):
+    P = vpm.fields.make_particles([[1, 2, 3]], [[4, 5, 6]], 0.5, static=[1])
+    assert P.shape == (1, 46)
+    assert list(P[0, 0:7]) == [1, 2, 3, 4, 5, 6, 0.5]
+    assert P[0, 8] == 1.0 and P[0, 42] == 1.0
+    J = np.zeros((1, 46))
+    J[0, 15:24] = np.arange(9)
+    Jm = vpm.fields.J_matrix(J)
+    # column-major: J[3b + a] = du_a/dx_b
+    assert Jm[0, 1, 0] == 1 and Jm[0, 0, 1] == 3
+    W = vpm.fields.get_W(J)
+    assert list(W[0]) == [5 - 7, 6 - 2, 1 - 3]
+
+
+def test_particlefield_add_remove(vpm):
+    api = vpm.api
+    pf = api.ParticleField(3)
+    for k in range(3):
+        api.add_particle(pf, (k, 0, 0), (0, 0, 1), 0.1, static=(k == 1))
+    with pytest.raises(api.VPMError):
+        api.add_particle(pf, (9, 0, 0), (0, 0, 1), 0.1)
+    assert api.get_np(pf) == 3 and pf.particles[1, 42] == 1.0
+    pf.particles[2, 9:27] = 7.0
+    api.remove_particle(pf, 0)            # last particle moves into slot 0
+    assert api.get_np(pf) == 2
+    assert pf.particles[0, 0] == 2.0 and np.all(pf.particles[0, 9:27] == 7.0)
+    assert np.all(pf.particles[2, 9:27] == 0.0)
+    with pytest.raises(api.VPMError):
+        api.remove_particle(pf, 5)
+
+
+def test_options_mirror_reference(vpm):
+    api = vpm.api
+    assert api.rVPM.f == 0 and api.rVPM.g == pytest.approx(0.2) and api.rVPM.h == pytest.approx(0.4)
+    assert api.cVPM.g == 0
+    assert api.SFS_Cd_twolevel_nobackscatter.alpha == 0.999
+    assert api.SFS_Cd_twolevel_nobackscatter.force_positive
+    assert api.pedrizzetti.rlxf == 0.3 and api.pedrizzetti.nsteps_relax == 1
+    assert api.zeta0("gaussianerf") == pytest.approx((2 * np.pi) ** -1.5)
+
+
+def test_workloads_shapes(vpm):
+    w = vpm.workloads
+    P0 = w.ring_cfg0()
+    assert P0.shape == (8000, 46)
+    np.testing.assert_allclose(P0[:, 6], 1.5 * 2 * np.pi / 200)
+    # total circulation of a station: Gamma0 * 2 pi R / Nphi along t_hat
+    np.testing.assert_allclose(np.linalg.norm(P0[:40, 3:6].sum(axis=0)), 2 * np.pi / 200, rtol=1e-12)
+    P1 = w.random_cloud(4096)
+    assert P1.shape == (4096, 46) and P1[:, 6].min() >= 0.02 and P1[:, 6].max() <= 0.04
+    P2 = w.leapfrog_cfg2(Nphi=64)
+    assert P2.shape == (2 * 64 * 64, 46)
+    Pr = w.ring_test()
+    assert Pr.shape == (100, 46)
+    np.testing.assert_allclose(np.linalg.norm(Pr[:, 3:6], axis=1), 2 * np.pi / 100)
+
+
+@pytest.mark.slow
+def test_jet_column_size(vpm):
+    P = vpm.workloads.jet_column()
+    assert P.shape == (1 << 20, 46)
+    r = np.hypot(P[:, 0], P[:, 1])
+    assert r.max() < 0.5 + 0.01 and abs(P[:, 2]).max() < 2.01
+    assert np.all(P[:, 6] == 0.015)
+
+
+def test_metrics(vpm):
+    R = np.array([[1.0, 0, 0], [0, 2.0, 0], [0, 0, 1e-9]])
+    F = R + np.array([[1e-6, 0, 0], [0, 0, 0], [0, 0, 1e-9]])
+    e = vpm.metrics.rel_err(F, R)
+    assert e[0] == pytest.approx(1e-6) and e[1] == 0
+    # the floor: 1e-6 * max norm
+    assert e[2] == pytest.approx(1e-9 / (1e-6 * 2.0))
+
+
+def test_gen_kernel_fits_reproducible(vpm):
+    """kernel_fits.h is what its generator writes."""
+    import subprocess
+    import sys
+    csrc = os.path.join(os.path.dirname(vpm.__file__), "csrc")
+    out = subprocess.run([sys.executable, os.path.join(csrc, "gen_kernel_fits.py")], check=True,
+                         capture_output=True, text=True).stdout
+    with open(os.path.join(csrc, "kernel_fits.h")) as fh:
+        assert fh.read().strip() == out.strip()

@@ -1,0 +1,255 @@
+"""GPU parity of the U/J, E_str and leaf-pair paths, through the C-ABI.
+
+Tolerances (per-particle relative error, see flowvpm.jl_amd/metrics.py):
+  F32C vs oracle/golden: max <= 1e-5 (the stated FP32c bound)
+  F64  vs oracle:        max <= 1e-12 (same FP64 arithmetic, other summation order)
+"""
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+KERNELS = ["singular", "gaussian", "gaussianerf", "winckelmans"]
+F32_TOL = 1e-5
+F64_TOL = 1e-12
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _uj_err(vpm, A, R):
+    eu = vpm.metrics.err_stats(A[:, 9:12], R[:, 9:12])
+    ej = vpm.metrics.err_stats(A[:, 15:24], R[:, 15:24])
+    return eu, ej
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("precision", ["f32c", "f64"])
+def test_golden_uj(vpm, kernel, precision):
+    _say("golden", kernel, precision)
+    g = golden("uj_%s.json" % kernel)
+    P = vpm.fields.make_particles(g["X"], g["Gamma"], g["sigma"])
+    vpm.UJ_hip(P, kernel=kernel, precision=precision)
+    U, J = np.array(g["U"]), np.array(g["J"])
+    eu = vpm.metrics.err_stats(P[:, 9:12], U)
+    ej = vpm.metrics.err_stats(P[:, 15:24], J)
+    _say("  U", eu, "J", ej)
+    if precision == "f32c":
+        # series forms keep FP32 accuracy at q = 1e-4 (the reference's FP64
+        # formula itself loses digits there for the Gaussian kernels)
+        assert eu["max"] < F32_TOL and ej["max"] < F32_TOL
+    else:
+        loose = {"gaussian": 3e-5, "gaussianerf": 3e-7}.get(kernel, 1e-11)
+        assert eu["max"] < loose and ej["max"] < loose
+        eu2 = vpm.metrics.err_stats(P[16:, 9:12], U[16:])
+        ej2 = vpm.metrics.err_stats(P[16:, 15:24], J[16:])
+        assert eu2["max"] < 1e-12 and ej2["max"] < 1e-12
+
+
+def test_two_body_check_fmm(vpm):
+    g = golden("check_fmm.json")
+    P = vpm.fields.make_particles(g["X"], g["Gamma"], g["sigma"])
+    for prec, tol in (("f64", 1e-13), ("f32c", 1e-6)):
+        A = P.copy()
+        vpm.UJ_hip(A, kernel="singular", precision=prec)
+        np.testing.assert_allclose(A[:, 9:12], g["U"], rtol=tol)
+        np.testing.assert_allclose(A[:, 15:24], g["J"], rtol=tol, atol=tol * np.abs(g["J"]).max())
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_cloud_vs_oracle(vpm, orc, kernel):
+    P = vpm.workloads.random_cloud(3000, seed=11)
+    ref = P.copy()
+    orc.uj_direct(ref, kernel)
+    for prec in ("f64", "f32c"):
+        _say("cloud N=3000", kernel, prec)
+        A = P.copy()
+        vpm.UJ_hip(A, kernel=kernel, precision=prec)
+        eu, ej = _uj_err(vpm, A, ref)
+        _say("  U", eu, "J", ej)
+        tol = F64_TOL if prec == "f64" else F32_TOL
+        assert eu["max"] < tol and ej["max"] < tol
+
+
+def test_ring_cfg0(vpm, orc):
+    """Config 0: the N=8,000 ring, Gaussian-erf, one U+J evaluation."""
+    P = vpm.workloads.ring_cfg0()
+    ref = P.copy()
+    orc.uj_direct(ref, "gaussianerf")
+    for prec in ("f64", "f32c"):
+        A = P.copy()
+        vpm.UJ_hip(A, kernel="gaussianerf", precision=prec)
+        eu, ej = _uj_err(vpm, A, ref)
+        _say("ring cfg0", prec, "U", eu, "J", ej)
+        tol = F64_TOL if prec == "f64" else F32_TOL
+        assert eu["max"] < tol and ej["max"] < tol
+
+
+def test_sort_and_storage_order_agree(vpm, orc):
+    P = vpm.workloads.random_cloud(1500, seed=2)
+    ref = P.copy()
+    orc.uj_direct(ref, "gaussianerf")
+    A, B = P.copy(), P.copy()
+    vpm.UJ_hip(A, kernel="gaussianerf", sort=True)
+    vpm.UJ_hip(B, kernel="gaussianerf", sort=False)
+    for X in (A, B):
+        eu, ej = _uj_err(vpm, X, ref)
+        assert eu["max"] < F32_TOL and ej["max"] < F32_TOL
+
+
+def test_edge_sizes(vpm, orc):
+    for n in (1, 2, 63, 255, 257, 700):
+        P = vpm.workloads.random_cloud(n, seed=n)
+        ref = P.copy()
+        orc.uj_direct(ref, "winckelmans")
+        for prec, tol in (("f64", F64_TOL), ("f32c", F32_TOL)):
+            A = P.copy()
+            vpm.UJ_hip(A, kernel="winckelmans", precision=prec)
+            if n == 1:
+                assert np.all(A[:, 9:24] == 0)
+                continue
+            eu, ej = _uj_err(vpm, A, ref)
+            assert eu["max"] < tol and ej["max"] < tol, (n, prec, eu, ej)
+
+
+def test_reset_accumulate_static_coincident(vpm, orc):
+    """UJ_direct semantics: reset skips static particles; reset=false
+    accumulates; coincident pairs are skipped (fmm.jl:118)."""
+    P = vpm.workloads.random_cloud(600, seed=5)
+    P[7, 0:3] = P[3, 0:3]                  # coincident pair
+    P[[10, 20], 42] = 1.0                  # static
+    P[:, 9:27] = 0.25                      # stale U/J/vorticity/PSE
+    for prec, tol in (("f64", F64_TOL), ("f32c", F32_TOL)):
+        ref = P.copy()
+        orc.uj_direct(ref, "gaussianerf")
+        orc.uj_direct(ref, "gaussianerf", reset=False)
+        A = P.copy()
+        vpm.UJ_hip(A, kernel="gaussianerf", precision=prec)
+        vpm.UJ_hip(A, kernel="gaussianerf", precision=prec, reset=False)
+        eu, ej = _uj_err(vpm, A, ref)
+        assert eu["max"] < tol and ej["max"] < tol, (prec, eu, ej)
+        np.testing.assert_array_equal(A[:, 12:15], ref[:, 12:15])   # vorticity rows
+        np.testing.assert_array_equal(A[:, 24:27], ref[:, 24:27])   # PSE rows
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("transposed", [True, False])
+def test_estr_vs_oracle(vpm, orc, kernel, transposed):
+    P = vpm.workloads.random_cloud(2000, seed=8)
+    P[:, 6] *= 3.0                         # overlapping cores: E_str is local
+    P[[5, 6], 42] = 1.0
+    orc.uj_direct(P, kernel)               # J from the oracle, shared by both
+    ref = P.copy()
+    orc.estr_direct(ref, kernel, transposed=transposed)
+    for prec, tol in (("f64", F64_TOL), ("f32c", F32_TOL)):
+        A = P.copy()
+        vpm.Estr_hip(A, kernel=kernel, precision=prec, transposed=transposed)
+        st = vpm.metrics.err_stats(A[:, 39:42], ref[:, 39:42])
+        _say("estr", kernel, transposed, prec, st)
+        assert st["max"] < tol, st
+        assert np.all(A[[5, 6], 39:42] == P[[5, 6], 39:42])
+
+
+def test_estr_golden(vpm):
+    g = golden("estr.json")
+    for kernel in KERNELS:
+        for tr in (True, False):
+            P = vpm.fields.make_particles(g["X"], g["Gamma"], g["sigma"], static=g["static"])
+            P[:, 15:24] = np.array(g["J"])
+            vpm.Estr_hip(P, kernel=kernel, precision="f32c", transposed=tr)
+            ref = np.array(g["%s_%s" % (kernel, "T" if tr else "C")])
+            st = vpm.metrics.err_stats(P[:, 39:42], ref)
+            assert st["max"] < F32_TOL, (kernel, tr, st)
+
+
+def test_uj_with_sfs_flag(vpm, orc):
+    P = vpm.workloads.random_cloud(1200, seed=4)
+    P[:, 6] *= 3.0
+    ref = P.copy()
+    orc.uj_direct(ref, "gaussianerf", reset_sfs=True, sfs=True)
+    A = P.copy()
+    vpm.UJ_hip(A, kernel="gaussianerf", reset_sfs=True, sfs=True)
+    st = vpm.metrics.err_stats(A[:, 39:42], ref[:, 39:42])
+    assert st["max"] < 1e-4, st     # E_str inherits the F32C error of J
+
+
+@pytest.mark.parametrize("precision", ["f32c", "f64"])
+def test_p2p_ranges_vs_oracle(vpm, orc, precision):
+    """Leaf-pair lists, including overlapping target ranges and repeats."""
+    P = vpm.workloads.random_cloud(900, seed=6)
+    rng = np.random.default_rng(3)
+    cuts = np.sort(np.concatenate([[0, 900], rng.choice(np.arange(1, 900), 20, replace=False)]))
+    leaves = list(zip(cuts[:-1], cuts[1:]))
+    tb, te, sb, se = [], [], [], []
+    for (a, b) in leaves:
+        for (c, d) in leaves:
+            if rng.random() < 0.4:
+                tb.append(a); te.append(b); sb.append(c); se.append(d)
+    tb += [5, 100]; te += [300, 120]; sb += [0, 40]; se += [50, 41]     # overlaps
+    ref = P.copy()
+    orc.direct_ranges(ref, "gaussianerf", tb, te, sb, se)
+    A = P.copy()
+    vpm.p2p_ranges(A, tb, te, sb, se, kernel="gaussianerf", precision=precision)
+    eu, ej = _uj_err(vpm, A, ref)
+    tol = F64_TOL if precision == "f64" else F32_TOL
+    assert eu["max"] < tol and ej["max"] < tol, (eu, ej)
+
+
+def test_ranges_reject_out_of_bounds(vpm):
+    P = vpm.workloads.random_cloud(100, seed=1)
+    with pytest.raises(vpm.VPMError):
+        vpm.p2p_ranges(P, [0], [101], [0], [10])
+    with pytest.raises(vpm.VPMError):
+        vpm.p2p_ranges(P, [5], [3], [0], [10])
+
+
+def test_two_field_form(vpm, orc):
+    S = vpm.workloads.random_cloud(700, seed=9)
+    T = vpm.workloads.random_cloud(500, seed=10)
+    T[:, 9:24] = 0.5
+    ref = T.copy()
+    orc.direct(ref, "gaussianerf", source=S)
+    A = T.copy()
+    vpm.UJ_hip_fields(S, A, kernel="gaussianerf")
+    eu, ej = _uj_err(vpm, A, ref)
+    assert eu["max"] < F32_TOL and ej["max"] < F32_TOL
+
+
+def test_rejects_bad_sigma(vpm):
+    P = vpm.workloads.random_cloud(10, seed=1)
+    P[3, 6] = 0.0
+    with pytest.raises(vpm.VPMError):
+        vpm.UJ_hip(P)
+    P[3, 6] = np.nan
+    with pytest.raises(vpm.VPMError):
+        vpm.UJ_hip(P)
+
+
+def test_deterministic(vpm):
+    P = vpm.workloads.random_cloud(5000, seed=12)
+    A, B = P.copy(), P.copy()
+    vpm.UJ_hip(A)
+    vpm.UJ_hip(B)
+    np.testing.assert_array_equal(A, B)
+
+
+def test_cfg1_full_size_sample(vpm, orc):
+    """Config 1 at full size (N=65,536), Gaussian-erf and Winckelmans, FP32c
+    and FP64, against the oracle on a sample of 1,024 targets."""
+    P = vpm.workloads.random_cloud(65536, seed=1)
+    idx = np.arange(0, 65536, 64)
+    for kernel in ("gaussianerf", "winckelmans"):
+        ref = P.copy()
+        ref[:, 9:24] = 0
+        sub = ref[idx].copy()
+        orc.direct(sub, kernel, source=P)
+        for prec in ("f32c", "f64"):
+            A = P.copy()
+            vpm.UJ_hip(A, kernel=kernel, precision=prec)
+            eu, ej = _uj_err(vpm, A[idx], sub)
+            _say("cfg1", kernel, prec, "U", eu, "J", ej)
+            tol = F64_TOL if prec == "f64" else F32_TOL
+            assert eu["max"] < tol and ej["max"] < tol
+            assert np.all(np.isfinite(A[:, 9:24]))
