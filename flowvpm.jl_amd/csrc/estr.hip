@@ -10,8 +10,12 @@
 //     sum_j zeta_j (J'_i - J'_j)^T Gamma_j = J'_i^T A_i - B_i,
 //     A_i = sum_j zeta_j Gamma_j,  B_i = sum_j zeta_j J'_j^T Gamma_j,
 // so a pair costs one zeta and six FMAs, and every FP32 rounding is relative
-// to the local variation of J (not to |J|).  A and B use the same per-tile
-// TwoSum compensation as the U/J kernel; J'_i^T A - B is formed in FP64.
+// to the local variation of J (not to |J|).  Distances use the hi/lo split of
+// positions relative to the block origin (zeta is steep, so r needs full
+// FP32 relative accuracy).  Sources beyond q_far, where zeta is below FP32
+// resolution of the near sum, are skipped per wave.  A and B use the same
+// per-tile TwoSum compensation as the U/J kernel; J'_i^T A - B is formed in
+// FP64.  The singular kernel (zeta nonzero only at r = 0) uses the F64 path.
 //
 // F64: the reference's pair expression (subfilterscale_models.jl:22-40) in
 // storage order, contraction off -- the GPU oracle for E_str.
@@ -34,22 +38,34 @@ __device__ __forceinline__ float zeta_sig(float r2, float a, float b)
     } else if constexpr (K == VPM_KERNEL_GAUSSIAN) {
         const float r = r2 > 0.f ? r2 * rsq_e(r2) : 0.f;
         return b * ex2_e(a * r2 * r);                // const3 s^-3 exp(-q^3)
-    } else if constexpr (K == VPM_KERNEL_WINCKELMANS) {
+    } else {
         const float s = rsq_e(fmaf(a, r2, 1.f));     // (q^2 + 1)^-1/2
         const float s2 = s * s;
         return b * (s2 * s2) * (s2 * s);             // 15/(8 pi) s^-3 (q^2+1)^-3.5
-    } else {
-        return r2 == 0.f ? b : 0.f;                  // zeta_sing: 1 at r = 0
     }
 }
 
-// Source record for E_str, in sorted order: a, b of zeta_sigma (b = 0 for
-// static sources, which the reference skips, subfilterscale_models.jl:66).
+// Sources beyond q_far contribute below FP32 resolution of the near sum:
+// Gaussian-erf zeta(7)/zeta(0) = 2.3e-11, Gaussian zeta(3)/zeta(0) = 1.9e-12.
+// Winckelmans decays only like q^-7, so every source is evaluated.
+template <int K>
+__device__ __forceinline__ float estr_cut(double sigma)
+{
+    if (K == VPM_KERNEL_GAUSSIANERF) return (float)(49.0 * sigma * sigma);
+    if (K == VPM_KERNEL_GAUSSIAN) return (float)(9.0 * sigma * sigma);
+    return FLT_MAX;
+}
+
+// Source records for E_str in sorted order: positions and J in FP64 (the
+// kernel subtracts the block origin and J_ref in FP64), (Gamma, a) and
+// (b, cut) in FP32.  b = 0 for static sources, which the reference skips
+// (subfilterscale_models.jl:66).
 template <int K>
 __global__ __launch_bounds__(kBlock) void pack_estr_kernel(const double* __restrict__ P, int64_t np,
                                                            const uint32_t* __restrict__ perm,
                                                            double* __restrict__ pos, double* __restrict__ J,
-                                                           int64_t ld, float4* __restrict__ gz)
+                                                           int64_t ld, float4* __restrict__ gz,
+                                                           float2* __restrict__ bc)
 {
     const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (k >= np) return;
@@ -60,29 +76,35 @@ __global__ __launch_bounds__(kBlock) void pack_estr_kernel(const double* __restr
     pos[2 * ld + k] = rec[RX + 2];
 #pragma unroll
     for (int c = 0; c < 9; ++c) J[c * ld + k] = rec[RJ + c];
-    const double si = 1.0 / rec[RSIG];
+    const double sg = rec[RSIG];
+    const double si = 1.0 / sg;
     const double s2 = si * si, s3 = s2 * si;
     double a, b;
     if (K == VPM_KERNEL_GAUSSIANERF) { a = -0.5 * kLog2eD * s2; b = kConst1 * s3; }
     else if (K == VPM_KERNEL_GAUSSIAN) { a = -kLog2eD * s3; b = kConst3 * s3; }
-    else if (K == VPM_KERNEL_WINCKELMANS) { a = s2; b = kConst4 * 7.5 * s3; }
-    else { a = 0.0; b = s3; }
+    else { a = s2; b = kConst4 * 7.5 * s3; }
     if (rec[RSTATIC] != 0) b = 0.0;
-    (void)a;
     gz[k] = make_float4((float)rec[RG], (float)rec[RG + 1], (float)rec[RG + 2], (float)a);
-    // b rides in the w of a second record: store it in the last position slot
-    pos[3 * ld + k] = b;
+    bc[k] = make_float2((float)b, estr_cut<K>(sg));
+}
+
+__device__ __forceinline__ void split_rel_e(double x, double o, float& hi, float& lo)
+{
+    const double d = x - o;
+    hi = (float)d;
+    lo = (float)(d - (double)hi);
 }
 
 template <int K>
 __global__ __launch_bounds__(kBlock) void estr_f32c_kernel(
     const double* __restrict__ pos, const double* __restrict__ J, int64_t ld,
-    const float4* __restrict__ gz, int64_t np, int transposed, int64_t chunk,
-    double* __restrict__ part, int64_t pld)
+    const float4* __restrict__ gz, const float2* __restrict__ bc, int64_t np, int transposed,
+    int64_t chunk, double* __restrict__ part, int64_t pld)
 {
-    __shared__ float4 lp[kTile];   // s' x, y, z, a
+    __shared__ float4 lp[kTile];   // s'_hi x, y, z, a
     __shared__ float4 lg[kTile];   // Gamma x, y, z, b
-    __shared__ float4 lq[kTile];   // J'^T Gamma (or J' Gamma)
+    __shared__ float4 lq[kTile];   // J'^T Gamma (or J' Gamma), cut
+    __shared__ float4 ll[kTile];   // s'_lo x, y, z
 
     const int tid = threadIdx.x;
     const int64_t i = (int64_t)blockIdx.x * kBlock + tid;
@@ -92,7 +114,10 @@ __global__ __launch_bounds__(kBlock) void estr_f32c_kernel(
 #pragma unroll
     for (int c = 0; c < 9; ++c) Jr[c] = J[c * ld + ib];
     const int64_t it = i < np ? i : ib;
-    const float tx = (float)(pos[it] - ox), ty = (float)(pos[ld + it] - oy), tz = (float)(pos[2 * ld + it] - oz);
+    float tx, ty, tz, tlx, tly, tlz;
+    split_rel_e(pos[it], ox, tx, tlx);
+    split_rel_e(pos[ld + it], oy, ty, tly);
+    split_rel_e(pos[2 * ld + it], oz, tz, tlz);
 
     const int64_t sb = (int64_t)blockIdx.y * chunk;
     const int64_t se = min(sb + chunk, np);
@@ -102,11 +127,16 @@ __global__ __launch_bounds__(kBlock) void estr_f32c_kernel(
 
     for (int64_t base = sb; base < se; base += kTile) {
         const int64_t j = base + tid;
-        float4 p, g, q;
+        float4 p, g, q, l;
         if (j < se) {
             const float4 gg = gz[j];
-            p = make_float4((float)(pos[j] - ox), (float)(pos[ld + j] - oy), (float)(pos[2 * ld + j] - oz), gg.w);
-            g = make_float4(gg.x, gg.y, gg.z, (float)pos[3 * ld + j]);
+            const float2 b = bc[j];
+            split_rel_e(pos[j], ox, p.x, l.x);
+            split_rel_e(pos[ld + j], oy, p.y, l.y);
+            split_rel_e(pos[2 * ld + j], oz, p.z, l.z);
+            p.w = gg.w;
+            l.w = 0.f;
+            g = make_float4(gg.x, gg.y, gg.z, b.x);
             double Jp[9];
 #pragma unroll
             for (int c = 0; c < 9; ++c) Jp[c] = J[c * ld + j] - Jr[c];
@@ -114,37 +144,46 @@ __global__ __launch_bounds__(kBlock) void estr_f32c_kernel(
             if (transposed) {   // S_a = sum_c J'[3a + c] Gamma_c
                 q = make_float4((float)(Jp[0] * G0 + Jp[1] * G1 + Jp[2] * G2),
                                 (float)(Jp[3] * G0 + Jp[4] * G1 + Jp[5] * G2),
-                                (float)(Jp[6] * G0 + Jp[7] * G1 + Jp[8] * G2), 0.f);
+                                (float)(Jp[6] * G0 + Jp[7] * G1 + Jp[8] * G2), b.y);
             } else {            // S_a = sum_c J'[3c + a] Gamma_c
                 q = make_float4((float)(Jp[0] * G0 + Jp[3] * G1 + Jp[6] * G2),
                                 (float)(Jp[1] * G0 + Jp[4] * G1 + Jp[7] * G2),
-                                (float)(Jp[2] * G0 + Jp[5] * G1 + Jp[8] * G2), 0.f);
+                                (float)(Jp[2] * G0 + Jp[5] * G1 + Jp[8] * G2), b.y);
             }
         } else {
             p = make_float4(1e15f, 1e15f, 1e15f, 0.f);
             g = make_float4(0.f, 0.f, 0.f, 0.f);
             q = make_float4(0.f, 0.f, 0.f, 0.f);
+            l = make_float4(0.f, 0.f, 0.f, 0.f);
         }
         __syncthreads();
         lp[tid] = p;
         lg[tid] = g;
         lq[tid] = q;
+        ll[tid] = l;
         __syncthreads();
         float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
+#pragma unroll 2
         for (int s = 0; s < kTile; ++s) {
             const float4 a = lp[s];
-            const float4 b = lg[s];
             const float4 c = lq[s];
-            const float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
-            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            const float z = zeta_sig<K>(r2, a.w, b.w);
-            acc[0] = fmaf(z, b.x, acc[0]);
-            acc[1] = fmaf(z, b.y, acc[1]);
-            acc[2] = fmaf(z, b.z, acc[2]);
-            acc[3] = fmaf(z, c.x, acc[3]);
-            acc[4] = fmaf(z, c.y, acc[4]);
-            acc[5] = fmaf(z, c.z, acc[5]);
+            float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
+            float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            if (__any(r2 < c.w)) {
+                const float4 lo = ll[s];
+                const float4 b = lg[s];
+                dx += tlx - lo.x;
+                dy += tly - lo.y;
+                dz += tlz - lo.z;
+                r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                const float z = zeta_sig<K>(r2, a.w, b.w);
+                acc[0] = fmaf(z, b.x, acc[0]);
+                acc[1] = fmaf(z, b.y, acc[1]);
+                acc[2] = fmaf(z, b.z, acc[2]);
+                acc[3] = fmaf(z, c.x, acc[3]);
+                acc[4] = fmaf(z, c.y, acc[4]);
+                acc[5] = fmaf(z, c.z, acc[5]);
+            }
         }
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
@@ -275,7 +314,9 @@ int launch_estr_impl(vpm_field f, int kernel, int transposed, int precision, con
     const int64_t np = f->np;
     if (np == 0) return VPM_OK;
     int rc;
-    if (precision == VPM_PREC_F64) {
+    // The singular kernel's zeta is nonzero only at r = 0 (coincident pairs),
+    // where the reference's (J_t - J_s) difference is exact: use FP64.
+    if (precision == VPM_PREC_F64 || kernel == VPM_KERNEL_SINGULAR) {
         timing_begin(f);
         switch (kernel) {
         case VPM_KERNEL_SINGULAR: hipLaunchKernelGGL(estr_f64_kernel<VPM_KERNEL_SINGULAR>, dim3(nblk(np)), dim3(kBlock), 0, f->stream, f->P, np, transposed); break;
@@ -290,13 +331,17 @@ int launch_estr_impl(vpm_field f, int kernel, int transposed, int precision, con
         return VPM_OK;
     }
     const int64_t ld = f->cap;
-    if ((rc = ensure(f->adj, sizeof(double) * 13 * (size_t)ld + sizeof(float4) * (size_t)ld))) return rc;
-    double* pos = (double*)f->adj.p;            // [4][ld]: x, y, z, b
-    double* J = pos + 4 * ld;                   // [9][ld]
+    // [3][ld] positions, [9][ld] J (double), [ld] float4, [ld] float2
+    if ((rc = ensure(f->adj, sizeof(double) * 12 * (size_t)ld + sizeof(float4) * (size_t)ld +
+                                 sizeof(float2) * (size_t)ld)))
+        return rc;
+    double* pos = (double*)f->adj.p;
+    double* J = pos + 3 * ld;
     float4* gz = (float4*)(J + 9 * ld);
+    float2* bc = (float2*)(gz + ld);
     switch (kernel) {
-#define VPM_PE(KID) case KID: hipLaunchKernelGGL(pack_estr_kernel<KID>, dim3(nblk(np)), dim3(kBlock), 0, f->stream, f->P, np, perm, pos, J, ld, gz); break;
-        VPM_PE(VPM_KERNEL_SINGULAR) VPM_PE(VPM_KERNEL_GAUSSIAN) VPM_PE(VPM_KERNEL_GAUSSIANERF) VPM_PE(VPM_KERNEL_WINCKELMANS)
+#define VPM_PE(KID) case KID: hipLaunchKernelGGL(pack_estr_kernel<KID>, dim3(nblk(np)), dim3(kBlock), 0, f->stream, f->P, np, perm, pos, J, ld, gz, bc); break;
+        VPM_PE(VPM_KERNEL_GAUSSIAN) VPM_PE(VPM_KERNEL_GAUSSIANERF) VPM_PE(VPM_KERNEL_WINCKELMANS)
 #undef VPM_PE
     default: return fail(VPM_EINVAL, "unknown kernel id");
     }
@@ -313,8 +358,8 @@ int launch_estr_impl(vpm_field f, int kernel, int transposed, int precision, con
     double* part = (double*)f->part.p;
     timing_begin(f);
     switch (kernel) {
-#define VPM_ES(KID) case KID: hipLaunchKernelGGL(estr_f32c_kernel<KID>, dim3(nblk(np), (unsigned)K), dim3(kBlock), 0, f->stream, pos, J, ld, gz, np, transposed, chunk, part, np); break;
-        VPM_ES(VPM_KERNEL_SINGULAR) VPM_ES(VPM_KERNEL_GAUSSIAN) VPM_ES(VPM_KERNEL_GAUSSIANERF) VPM_ES(VPM_KERNEL_WINCKELMANS)
+#define VPM_ES(KID) case KID: hipLaunchKernelGGL(estr_f32c_kernel<KID>, dim3(nblk(np), (unsigned)K), dim3(kBlock), 0, f->stream, pos, J, ld, gz, bc, np, transposed, chunk, part, np); break;
+        VPM_ES(VPM_KERNEL_GAUSSIAN) VPM_ES(VPM_KERNEL_GAUSSIANERF) VPM_ES(VPM_KERNEL_WINCKELMANS)
 #undef VPM_ES
     default: return fail(VPM_EINVAL, "unknown kernel id");
     }

@@ -21,12 +21,29 @@ __device__ __forceinline__ float horner8(const float (&c)[8], float x)
     return a;
 }
 
-// Far-field threshold q_far^2: for q >= q_far, g = 1 and q g' - 3 g = -3
-// to FP32 rounding.  Gaussian-erf: 1 - g < 2e-10 and |q g'| < 3e-9 at q = 7.
-// Gaussian: exp(-q^3) (1 + q^3) < 6e-11 at q = 3.
-template <int K> struct Far { static constexpr float q2 = 0.f; static constexpr bool has = false; };
-template <> struct Far<VPM_KERNEL_GAUSSIANERF> { static constexpr float q2 = 49.f; static constexpr bool has = true; };
-template <> struct Far<VPM_KERNEL_GAUSSIAN> { static constexpr float q2 = 9.f; static constexpr bool has = true; };
+// Near/far split per source: a source is "near" a wave when some lane has
+// r^2 < q_far^2 sigma^2.  Near sources take the full kernel evaluation and
+// the hi/lo position correction; far sources use the cheap form.
+//  * Gaussian-erf: for q >= 7, 1 - g < 2e-10 and |q g'| < 3e-9, so g = 1 and
+//    q g' - 3 g = -3 to FP32 rounding: F' = r^-3, H'' = r^-5.
+//  * Gaussian: for q >= 3, e^{-q^3} (1 + q^3) < 6e-11: same far form.
+//  * Winckelmans: g -> 1 only as 1 - 1.9/q^4, so the far form is the full
+//    formula; q_far = 7 only decides where the hi/lo correction is applied.
+//  * singular: every source is near (its pair error grows like 1/r).
+template <int K> struct Far;
+template <> struct Far<VPM_KERNEL_GAUSSIANERF> { static constexpr float q2 = 49.f; static constexpr bool cheap = true; };
+template <> struct Far<VPM_KERNEL_GAUSSIAN> { static constexpr float q2 = 9.f; static constexpr bool cheap = true; };
+template <> struct Far<VPM_KERNEL_WINCKELMANS> { static constexpr float q2 = 49.f; static constexpr bool cheap = false; };
+template <> struct Far<VPM_KERNEL_SINGULAR> { static constexpr float q2 = 0.f; static constexpr bool cheap = false; };
+
+// far-field r^2 threshold of a source of core size sigma
+template <int K>
+__host__ __device__ __forceinline__ float far_cut(double sigma)
+{
+    if (K == VPM_KERNEL_SINGULAR) return FLT_MAX;
+    const float c = (float)((double)Far<K>::q2 * sigma * sigma);
+    return c > FLT_MIN ? c : FLT_MIN;
+}
 
 // Full (any-q) evaluation of F' = g/r^3 and H'' = -(q g' - 3g)/(3 r^5).
 // Coincident pairs (r2 == 0) contribute nothing (fmm.jl:118).
@@ -100,24 +117,31 @@ __device__ __forceinline__ void coef_full<VPM_KERNEL_WINCKELMANS>(float r2, floa
     if (!(r2 > 0.f)) { Fp = 0.f; Hpp = 0.f; }
 }
 
-// One source (a = (s', far threshold), b = (Gamma', 1/sigma)) acting on the
-// lane's target t' (F32C); accumulates U', W', Jhv (see p2p_uj.hip).
+// One source acting on the lane's target (F32C); accumulates U', W', Jhv
+// (see p2p_uj.hip).  a = (s'_hi, far threshold), b = (Gamma', 1/sigma),
+// lo = &s'_lo (read only when the source is near), tl = t'_lo.
+// Positions are relative to the block origin O: t - O = t'_hi + t'_lo
+// exactly to ~2^-48, so near pairs get d to FP32 relative accuracy.
 template <int K>
-__device__ __forceinline__ void pair_f32c(float tx, float ty, float tz, const float4 a, const float4 b,
+__device__ __forceinline__ void pair_f32c(float tx, float ty, float tz, float tlx, float tly, float tlz,
+                                          const float4 a, const float4 b, const float4* lo,
                                           float (&acc)[kAccF32])
 {
-    const float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
-    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
+    float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
     float Fp, Hpp;
-    if constexpr (Far<K>::has) {
-        if (__any(r2 < a.w)) {
-            coef_full<K>(r2, b.w, Fp, Hpp);
-        } else {
-            const float ri = rsq(r2);
-            const float ri2 = ri * ri;
-            Fp = ri2 * ri;
-            Hpp = Fp * ri2;
-        }
+    if (__any(r2 < a.w)) {
+        const float4 l = *lo;
+        dx += tlx - l.x;
+        dy += tly - l.y;
+        dz += tlz - l.z;
+        r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        coef_full<K>(r2, b.w, Fp, Hpp);
+    } else if constexpr (Far<K>::cheap) {
+        const float ri = rsq(r2);
+        const float ri2 = ri * ri;
+        Fp = ri2 * ri;
+        Hpp = Fp * ri2;
     } else {
         coef_full<K>(r2, b.w, Fp, Hpp);
     }
@@ -139,6 +163,14 @@ __device__ __forceinline__ void pair_f32c(float tx, float ty, float tz, const fl
     acc[11] = fmaf(hz, dy, acc[11]);   // Jhv[2][1]
     acc[12] = fmaf(hx, dz, acc[12]);   // Jhv[0][2]
     acc[13] = fmaf(hy, dz, acc[13]);   // Jhv[1][2]
+}
+
+// hi/lo split of a position relative to the block origin
+__device__ __forceinline__ void split_rel(double x, double o, float& hi, float& lo)
+{
+    const double d = x - o;
+    hi = (float)d;
+    lo = (float)(d - (double)hi);
 }
 
 // TwoSum fold of FP32 partials into (sum, err)

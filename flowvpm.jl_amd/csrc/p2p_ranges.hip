@@ -32,6 +32,7 @@ __global__ __launch_bounds__(kBlock) void ranges_f32c_kernel(const RangeItem* __
 {
     __shared__ float4 lpos[kTile];
     __shared__ float4 lgs[kTile];
+    __shared__ float4 llo[kTile];
     const RangeItem it = items[blockIdx.x];
     const int tid = threadIdx.x;
     const int64_t ib = it.t_begin;
@@ -39,7 +40,10 @@ __global__ __launch_bounds__(kBlock) void ranges_f32c_kernel(const RangeItem* __
     const int64_t t = it.t_begin + tid;
     const bool active = t < it.t_end;
     const int64_t tt = active ? t : ib;
-    const float tx = (float)(pos[tt] - ox), ty = (float)(pos[ld + tt] - oy), tz = (float)(pos[2 * ld + tt] - oz);
+    float tx, ty, tz, tlx, tly, tlz;
+    split_rel(pos[tt], ox, tx, tlx);
+    split_rel(pos[ld + tt], oy, ty, tly);
+    split_rel(pos[2 * ld + tt], oz, tz, tlz);
     float sum[kAccF32], err[kAccF32];
 #pragma unroll
     for (int c = 0; c < kAccF32; ++c) { sum[c] = 0.f; err[c] = 0.f; }
@@ -50,15 +54,21 @@ __global__ __launch_bounds__(kBlock) void ranges_f32c_kernel(const RangeItem* __
             __syncthreads();
             if (tid < n) {
                 const int64_t j = base + tid;
-                lpos[tid] = make_float4((float)(pos[j] - ox), (float)(pos[ld + j] - oy), (float)(pos[2 * ld + j] - oz),
-                                        Far<K>::has ? cut[j] : 0.f);
+                float4 p, q;
+                split_rel(pos[j], ox, p.x, q.x);
+                split_rel(pos[ld + j], oy, p.y, q.y);
+                split_rel(pos[2 * ld + j], oz, p.z, q.z);
+                p.w = cut[j];
+                q.w = 0.f;
+                lpos[tid] = p;
+                llo[tid] = q;
                 lgs[tid] = gs[j];
             }
             __syncthreads();
             float acc[kAccF32];
 #pragma unroll
             for (int c = 0; c < kAccF32; ++c) acc[c] = 0.f;
-            for (int s = 0; s < n; ++s) pair_f32c<K>(tx, ty, tz, lpos[s], lgs[s], acc);
+            for (int s = 0; s < n; ++s) pair_f32c<K>(tx, ty, tz, tlx, tly, tlz, lpos[s], lgs[s], &llo[s], acc);
             fold_twosum(acc, sum, err);
         }
     }

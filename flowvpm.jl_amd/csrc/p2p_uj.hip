@@ -47,8 +47,9 @@ __global__ __launch_bounds__(kBlock) void uj_f32c_kernel(
     const float* __restrict__ scut, int64_t s0, int64_t s1, int64_t chunk,
     double* __restrict__ part, int64_t pld)
 {
-    __shared__ float4 lpos[kTile];   // s' x, y, z, far-field r^2 threshold
+    __shared__ float4 lpos[kTile];   // s'_hi x, y, z, far-field r^2 threshold
     __shared__ float4 lgs[kTile];    // Gamma' x, y, z, 1/sigma
+    __shared__ float4 llo[kTile];    // s'_lo x, y, z
 
     const int tid = threadIdx.x;
     const int64_t i = (int64_t)blockIdx.x * kBlock + tid;
@@ -56,9 +57,10 @@ __global__ __launch_bounds__(kBlock) void uj_f32c_kernel(
     const double ox = tpos[ib], oy = tpos[tld + ib], oz = tpos[2 * tld + ib];
     const bool active = i < nt;
     const int64_t it = active ? t0 + i : ib;
-    const float tx = (float)(tpos[it] - ox);
-    const float ty = (float)(tpos[tld + it] - oy);
-    const float tz = (float)(tpos[2 * tld + it] - oz);
+    float tx, ty, tz, tlx, tly, tlz;
+    split_rel(tpos[it], ox, tx, tlx);
+    split_rel(tpos[tld + it], oy, ty, tly);
+    split_rel(tpos[2 * tld + it], oz, tz, tlz);
 
     const int64_t sb = s0 + (int64_t)blockIdx.y * chunk;
     const int64_t se = min(sb + chunk, s1);
@@ -69,27 +71,30 @@ __global__ __launch_bounds__(kBlock) void uj_f32c_kernel(
 
     for (int64_t base = sb; base < se; base += kTile) {
         const int64_t j = base + tid;
-        float4 p, g;
+        float4 p, g, l;
         if (j < se) {
-            p.x = (float)(spos[j] - ox);
-            p.y = (float)(spos[sld + j] - oy);
-            p.z = (float)(spos[2 * sld + j] - oz);
-            p.w = Far<K>::has ? scut[j] : 0.f;
+            split_rel(spos[j], ox, p.x, l.x);
+            split_rel(spos[sld + j], oy, p.y, l.y);
+            split_rel(spos[2 * sld + j], oz, p.z, l.z);
+            p.w = scut[j];
+            l.w = 0.f;
             g = sgs[j];
-        } else {   // padding: zero strength, far away, never "near"
+        } else {   // padding: zero strength, far away, never near
             p = make_float4(1e15f, 1e15f, 1e15f, 0.f);
             g = make_float4(0.f, 0.f, 0.f, 0.f);
+            l = make_float4(0.f, 0.f, 0.f, 0.f);
         }
         __syncthreads();
         lpos[tid] = p;
         lgs[tid] = g;
+        llo[tid] = l;
         __syncthreads();
 
         float acc[kAccF32];
 #pragma unroll
         for (int c = 0; c < kAccF32; ++c) acc[c] = 0.f;
 #pragma unroll 2
-        for (int s = 0; s < kTile; ++s) pair_f32c<K>(tx, ty, tz, lpos[s], lgs[s], acc);
+        for (int s = 0; s < kTile; ++s) pair_f32c<K>(tx, ty, tz, tlx, tly, tlz, lpos[s], lgs[s], &llo[s], acc);
         fold_twosum(acc, sum, err);
     }
     if (active) {
@@ -275,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void pack_f32c_kernel(const double* __restr
     const double sg = rec[RSIG];
     gs[k] = make_float4((float)(-kConst4 * rec[RG]), (float)(-kConst4 * rec[RG + 1]),
                         (float)(-kConst4 * rec[RG + 2]), (float)(1.0 / sg));
-    if (Far<K>::has) cut[k] = fmaxf((float)((double)Far<K>::q2 * sg * sg), FLT_MIN);
+    cut[k] = far_cut<K>(sg);
 }
 
 __global__ __launch_bounds__(kBlock) void pack_f64_kernel(const double* __restrict__ P, int64_t np,

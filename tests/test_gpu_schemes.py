@@ -1,0 +1,186 @@
+"""GPU parity of the per-particle FP64 schemes, the RK3 step and the adjoint.
+
+The per-particle updates run in FP64 on the device, so with the F64 pair
+path a whole rungekutta3 step must agree with the oracle to FP64 rounding
+(1e-10 on the state after one step); with F32C pairs the state carries the
+FP32c U/J error (bounded here by 1e-5 relative per particle, vector norms).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _state_err(vpm, A, R):
+    out = {}
+    for name, sl in (("X", slice(0, 3)), ("Gamma", slice(3, 6)), ("sigma", slice(6, 7))):
+        out[name] = vpm.metrics.err_stats(A[:, sl], R[:, sl])["max"]
+    return out
+
+
+def _ring_pair(vpm, Nphi=128):
+    P = vpm.workloads.leapfrog_cfg2(Nphi=Nphi)
+    return P
+
+
+def _pfield(vpm, P, **kw):
+    api = vpm.api
+    pf = api.ParticleField(P.shape[0], **kw)
+    api.add_particles(pf, P)
+    return pf
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32c"])
+def test_rk3_nosfs_relax_vs_oracle(vpm, orc, precision):
+    api = vpm.api
+    P = vpm.workloads.ring_cfg0()[::4].copy()          # 2,000 particles
+    P[[3, 50], 42] = 1.0                               # two static particles
+    pf = _pfield(vpm, P, kernel="gaussianerf", formulation=api.rVPM, relaxation=api.pedrizzetti,
+                 precision=precision)
+    s = orc.make_settings(kernel="gaussianerf", f=0.0, g=0.2, relax=1, rlxf=0.3)
+    R = P.copy()
+    dt = 0.01
+    for step in range(2):
+        orc.rk3_step(R, s, dt, relax_now=True)
+        api.nextstep(pf, dt, relax=True)
+    e = _state_err(vpm, pf.particles[:pf.np], R)
+    _say("rk3 nosfs", precision, e)
+    tol = 1e-10 if precision == "f64" else 1e-5
+    assert max(e.values()) < tol, e
+    # nextstep's U_prev = |U|
+    np.testing.assert_allclose(pf.particles[:pf.np, 43], np.linalg.norm(pf.particles[:pf.np, 9:12], axis=1))
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32c"])
+def test_rk3_dynamic_sfs_core_spreading_vs_oracle(vpm, orc, precision):
+    """Config-2 physics (rVPM, SFS_Cd_twolevel_nobackscatter, CoreSpreading,
+    pedrizzetti) on a reduced leapfrog pair."""
+    api = vpm.api
+    P = _ring_pair(vpm, Nphi=64)                        # 8,192 particles
+    sg0 = float(P[0, 6])
+    pf = _pfield(vpm, P, kernel="gaussianerf", formulation=api.rVPM, relaxation=api.pedrizzetti,
+                 SFS=api.SFS_Cd_twolevel_nobackscatter, viscous=api.CoreSpreading(1e-4, sg0),
+                 precision=precision)
+    s = orc.make_settings(kernel="gaussianerf", f=0.0, g=0.2, relax=1, rlxf=0.3, sfs=1, sfs_alpha=0.999,
+                          sfs_rlxf=0.005, sfs_minC=0.0, sfs_maxC=1.0, sfs_force_positive=True,
+                          sfs_clip=True, viscous=1, nu=1e-4, sgm0=sg0, beta=1.5)
+    R = P.copy()
+    dt = 0.01
+    for step in range(2):
+        rc = orc.rk3_step(R, s, dt, relax_now=True)
+        assert rc == 0
+        api.nextstep(pf, dt, relax=True)
+    e = _state_err(vpm, pf.particles[:pf.np], R)
+    C = vpm.metrics.err_stats(pf.particles[:pf.np, 36:39], R[:, 36:39])["max"]
+    _say("rk3 dynsfs+cs", precision, e, "C", C)
+    assert pf.viscous.t_sgm == pytest.approx(s.t_sgm)
+    if precision == "f64":
+        assert max(e.values()) < 1e-10, e
+        assert C < 1e-8
+    else:
+        assert max(e.values()) < 1e-5, e
+
+
+def test_per_particle_kernels_vs_oracle(vpm, orc):
+    """update_particle_states, both relaxations and core spreading, one call each."""
+    api = vpm.api
+    rng = np.random.default_rng(0)
+    n = 777
+    P = vpm.workloads.random_cloud(n, seed=3)
+    P[:, 9:12] = rng.normal(size=(n, 3))
+    P[:, 15:24] = rng.normal(size=(n, 9))
+    P[:, 27:36] = rng.normal(size=(n, 9))
+    P[:, 36] = rng.uniform(0, 1, n)
+    P[:, 39:42] = rng.normal(size=(n, 3))
+    P[5, 3:6] = 0.0                                    # |Gamma| = 0 branch
+    P[9, 42] = 1.0
+    f = api.Field(n, 0)
+    for transposed in (True, False):
+        for (a, b) in ((0.0, 1 / 3), (-5 / 9, 15 / 16)):
+            R = P.copy()
+            orc.update_particle_states(R, a, b, 0.01, [0.1, 0.2, 0.3], 0.1, 0.2, 0.06, transposed)
+            f.set_particles(P)
+            api.check(api.lib().vpm_update_particle_states(f.handle, a, b, 0.01,
+                                                           np.array([0.1, 0.2, 0.3]).ctypes.data_as(api._lib._dptr),
+                                                           0.1, 0.2, 0.06, int(transposed)))
+            A = np.zeros_like(P)
+            f.get_particles(A)
+            np.testing.assert_array_equal(A, R)
+    for corrected in (0, 1):
+        R = P.copy()
+        orc.relax(R, 0.3, corrected)
+        f.set_particles(P)
+        api.check(api.lib().vpm_relax(f.handle, 0.3, corrected))
+        A = np.zeros_like(P)
+        f.get_particles(A)
+        np.testing.assert_allclose(A, R, rtol=1e-14, atol=0)
+    R = P.copy()
+    orc.core_spreading_rk3(R, 1e-4, 0.03, 1.5, 0.0, -5 / 9, 15 / 16, 0.01)
+    f.set_particles(P)
+    api.check(api.lib().vpm_core_spreading(f.handle, 1e-4, -5 / 9, 15 / 16, 0.01))
+    A = np.zeros_like(P)
+    f.get_particles(A)
+    np.testing.assert_array_equal(A, R)
+    f.close()
+
+
+def test_single_ring_translation(vpm, orc):
+    """test/runtests_singlevortexring.jl:22-23 with UJ_direct and RK3: nc=0
+    ring of 100 particles, Winckelmans, cVPM, pedrizzetti every step, 50
+    steps over two ring radii; the |Gamma|-weighted centroid speed must be
+    within 2% of Uring = Gamma/(4 pi R) (ln(8R/Rcross) - 1/2) (:143)."""
+    api = vpm.api
+    P = vpm.workloads.ring_test()
+    Uana = vpm.workloads.Uring(1.0, 1.0, 0.15, 0.5)
+    nsteps = 50
+    dt = (2.0 / Uana) / nsteps
+    for precision in ("f64", "f32c"):
+        pf = _pfield(vpm, P, kernel="winckelmans", formulation=api.cVPM, relaxation=api.pedrizzetti,
+                     precision=precision)
+        api.run_vpm(pf, dt, nsteps)
+        Z = vpm.workloads.ring_centroid(pf.particles[:pf.np])
+        U = np.linalg.norm(Z) / pf.t
+        err = (U - Uana) / Uana
+        _say("ring", precision, "U_vpm %.5f U_ana %.5f err %.4f" % (U, Uana, err))
+        assert abs(err) < 0.02
+    # and the GPU rollout matches the oracle's FP64 rollout
+    R = P.copy()
+    s = orc.make_settings(kernel="winckelmans", f=0.0, g=0.0, relax=1, rlxf=0.3)
+    for _ in range(nsteps):
+        orc.rk3_step(R, s, dt, relax_now=True)
+    pf = _pfield(vpm, P, kernel="winckelmans", formulation=api.cVPM, relaxation=api.pedrizzetti,
+                 precision="f64")
+    api.run_vpm(pf, dt, nsteps)
+    e = _state_err(vpm, pf.particles[:pf.np], R)
+    assert max(e.values()) < 1e-9, e
+
+
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans", "gaussian", "singular"])
+def test_adjoint_vs_oracle(vpm, orc, kernel):
+    P = vpm.workloads.random_cloud(1500, seed=21)
+    w, W = vpm.workloads.adjoint_weights(1500)
+    Xr, Gr, Sr = orc.uj_adjoint(P, kernel, w, W)
+    Xb, Gb, Sb = vpm.uj_adjoint(P, w, W, kernel=kernel)
+    for name, a, r in (("X", Xb, Xr), ("Gamma", Gb, Gr), ("sigma", Sb[:, None], Sr[:, None])):
+        st = vpm.metrics.err_stats(a, r)
+        _say("adjoint", kernel, name, st)
+        assert st["max"] < 1e-9, (name, st)
+
+
+def test_adjoint_golden(vpm):
+    g = golden("adjoint.json")
+    P = vpm.fields.make_particles(g["X"], g["Gamma"], g["sigma"])
+    for kernel in ("gaussianerf", "winckelmans", "gaussian", "singular"):
+        Xb, Gb, Sb = vpm.uj_adjoint(P, np.array(g["Ubar"]), np.array(g["Jbar"]), kernel=kernel)
+        ref = g[kernel]
+        np.testing.assert_allclose(Xb, ref["Xbar"], rtol=1e-9, atol=1e-12 * np.abs(ref["Xbar"]).max())
+        np.testing.assert_allclose(Gb, ref["Gbar"], rtol=1e-9, atol=1e-12 * np.abs(ref["Gbar"]).max())
+        np.testing.assert_allclose(Sb, ref["Sbar"], rtol=1e-9, atol=1e-12 * max(np.abs(ref["Sbar"]).max(), 1e-30))
