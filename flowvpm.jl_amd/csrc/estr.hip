@@ -162,36 +162,40 @@ __global__ __launch_bounds__(kBlock) void estr_f32c_kernel(
         lq[tid] = q;
         ll[tid] = l;
         __syncthreads();
-        float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        // fold into the compensated sums every 32 sources: E_str sums cancel
+        // strongly, so the FP32 tile sums are kept short
+        for (int s0 = 0; s0 < kTile; s0 += 32) {
+            float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll 2
-        for (int s = 0; s < kTile; ++s) {
-            const float4 a = lp[s];
-            const float4 c = lq[s];
-            float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
-            float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            if (__any(r2 < c.w)) {
-                const float4 lo = ll[s];
-                const float4 b = lg[s];
-                dx += tlx - lo.x;
-                dy += tly - lo.y;
-                dz += tlz - lo.z;
-                r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                const float z = zeta_sig<K>(r2, a.w, b.w);
-                acc[0] = fmaf(z, b.x, acc[0]);
-                acc[1] = fmaf(z, b.y, acc[1]);
-                acc[2] = fmaf(z, b.z, acc[2]);
-                acc[3] = fmaf(z, c.x, acc[3]);
-                acc[4] = fmaf(z, c.y, acc[4]);
-                acc[5] = fmaf(z, c.z, acc[5]);
+            for (int s = s0; s < s0 + 32; ++s) {
+                const float4 a = lp[s];
+                const float4 c = lq[s];
+                float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
+                float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                if (__any(r2 < c.w)) {
+                    const float4 lo = ll[s];
+                    const float4 b = lg[s];
+                    dx += tlx - lo.x;
+                    dy += tly - lo.y;
+                    dz += tlz - lo.z;
+                    r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    const float z = zeta_sig<K>(r2, a.w, b.w);
+                    acc[0] = fmaf(z, b.x, acc[0]);
+                    acc[1] = fmaf(z, b.y, acc[1]);
+                    acc[2] = fmaf(z, b.z, acc[2]);
+                    acc[3] = fmaf(z, c.x, acc[3]);
+                    acc[4] = fmaf(z, c.y, acc[4]);
+                    acc[5] = fmaf(z, c.z, acc[5]);
+                }
             }
-        }
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            const float x = acc[c];
-            const float t = sum[c] + x;
-            const float bp = t - sum[c];
-            err[c] += (sum[c] - (t - bp)) + (x - bp);
-            sum[c] = t;
+            for (int c = 0; c < 6; ++c) {
+                const float x = acc[c];
+                const float t = sum[c] + x;
+                const float bp = t - sum[c];
+                err[c] += (sum[c] - (t - bp)) + (x - bp);
+                sum[c] = t;
+            }
         }
     }
     if (i < np) {
