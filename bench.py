@@ -69,8 +69,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; --gpus %d is not supported (the all-pairs path "
-              "would shard targets across GPUs, see DESIGN.md)" % args.gpus, file=sys.stderr)
+        print("bench.py: one GPU only; --gpus %d is not supported (the all-pairs path would shard "
+              "targets across GPUs with an all-gather of the source records)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     from __graft_entry__ import load_package

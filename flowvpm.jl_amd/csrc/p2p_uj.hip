@@ -37,6 +37,11 @@
 
 #include "p2p_pair.h"
 
+// Source-loop unroll of the F32C kernel (register pressure vs. issue overlap)
+#ifndef VPM_UJ_UNROLL
+#define VPM_UJ_UNROLL 2
+#endif
+
 namespace vpm {
 
 // ------------------------------------------------------------ F32C kernel
@@ -93,7 +98,7 @@ __global__ __launch_bounds__(kBlock) void uj_f32c_kernel(
         float acc[kAccF32];
 #pragma unroll
         for (int c = 0; c < kAccF32; ++c) acc[c] = 0.f;
-#pragma unroll 2
+#pragma unroll VPM_UJ_UNROLL
         for (int s = 0; s < kTile; ++s) pair_f32c<K>(tx, ty, tz, tlx, tly, tlz, lpos[s], lgs[s], &llo[s], acc);
         fold_twosum(acc, sum, err);
     }

@@ -183,7 +183,8 @@ def test_uj_with_sfs_flag(vpm, orc):
     A = P.copy()
     vpm.UJ_hip(A, kernel="gaussianerf", reset_sfs=True, sfs=True)
     st = vpm.metrics.err_stats(A[:, 39:42], ref[:, 39:42])
-    assert st["max"] < 1e-4, st     # E_str inherits the F32C error of J
+    # E_str here also inherits the FP32c error of J from the same call
+    assert st["max"] < ESTR_MAX_TOL, st
 
 
 @pytest.mark.parametrize("precision", ["f32c", "f64"])
