@@ -117,34 +117,10 @@ __device__ __forceinline__ void coef_full<VPM_KERNEL_WINCKELMANS>(float r2, floa
     if (!(r2 > 0.f)) { Fp = 0.f; Hpp = 0.f; }
 }
 
-// One source acting on the lane's target (F32C); accumulates U', W', Jhv
-// (see p2p_uj.hip).  a = (s'_hi, far threshold), b = (Gamma', 1/sigma),
-// lo = &s'_lo (read only when the source is near), tl = t'_lo.
-// Positions are relative to the block origin O: t - O = t'_hi + t'_lo
-// exactly to ~2^-48, so near pairs get d to FP32 relative accuracy.
-template <int K>
-__device__ __forceinline__ void pair_f32c(float tx, float ty, float tz, float tlx, float tly, float tlz,
-                                          const float4 a, const float4 b, const float4* lo,
-                                          float (&acc)[kAccF32])
+// Accumulates one pair given its separation d and coefficients F', H''.
+__device__ __forceinline__ void accum_pair(float dx, float dy, float dz, float Fp, float Hpp, const float4 b,
+                                           float (&acc)[kAccF32])
 {
-    float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
-    float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-    float Fp, Hpp;
-    if (__any(r2 < a.w)) {
-        const float4 l = *lo;
-        dx += tlx - l.x;
-        dy += tly - l.y;
-        dz += tlz - l.z;
-        r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        coef_full<K>(r2, b.w, Fp, Hpp);
-    } else if constexpr (Far<K>::cheap) {
-        const float ri = rsq(r2);
-        const float ri2 = ri * ri;
-        Fp = ri2 * ri;
-        Hpp = Fp * ri2;
-    } else {
-        coef_full<K>(r2, b.w, Fp, Hpp);
-    }
     const float vx = fmaf(dy, b.z, -dz * b.y);
     const float vy = fmaf(dz, b.x, -dx * b.z);
     const float vz = fmaf(dx, b.y, -dy * b.x);
@@ -163,6 +139,117 @@ __device__ __forceinline__ void pair_f32c(float tx, float ty, float tz, float tl
     acc[11] = fmaf(hz, dy, acc[11]);   // Jhv[2][1]
     acc[12] = fmaf(hx, dz, acc[12]);   // Jhv[0][2]
     acc[13] = fmaf(hy, dz, acc[13]);   // Jhv[1][2]
+}
+
+// Far form of one pair: F' = r^-3, H'' = r^-5 (kernels with Far<K>::cheap),
+// or the full formula without the hi/lo correction (Winckelmans).
+template <int K>
+__device__ __forceinline__ void far_pair(float tx, float ty, float tz, const float4 a, const float4 b,
+                                         float (&acc)[kAccF32])
+{
+    const float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
+    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    float Fp, Hpp;
+    if constexpr (Far<K>::cheap) {
+        const float ri = rsq(r2);
+        const float ri2 = ri * ri;
+        Fp = ri2 * ri;
+        Hpp = Fp * ri2;
+    } else {
+        coef_full<K>(r2, b.w, Fp, Hpp);
+    }
+    accum_pair(dx, dy, dz, Fp, Hpp, b, acc);
+}
+
+// Near form: d from the hi/lo split (t - O = t'_hi + t'_lo to ~2^-48, so d
+// keeps FP32 relative accuracy however close the pair), full kernel formula.
+template <int K>
+__device__ __forceinline__ void near_pair(float tx, float ty, float tz, float tlx, float tly, float tlz,
+                                          const float4 a, const float4 b, const float4 l, float (&acc)[kAccF32])
+{
+    const float dx = (tx - a.x) + (tlx - l.x);
+    const float dy = (ty - a.y) + (tly - l.y);
+    const float dz = (tz - a.z) + (tlz - l.z);
+    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    float Fp, Hpp;
+    coef_full<K>(r2, b.w, Fp, Hpp);
+    accum_pair(dx, dy, dz, Fp, Hpp, b, acc);
+}
+
+// Bounding box of the wave's targets (t'_hi), computed once per block; the
+// result is wave-uniform and is kept in scalar registers.
+struct WaveBox { float lo[3], hi[3]; };
+
+__device__ __forceinline__ float uniform_f(float x)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+
+__device__ __forceinline__ WaveBox wave_box(float tx, float ty, float tz)
+{
+    WaveBox b;
+    float v[3] = {tx, ty, tz};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float mn = v[c], mx = v[c];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            mn = fminf(mn, __shfl_xor(mn, off));
+            mx = fmaxf(mx, __shfl_xor(mx, off));
+        }
+        b.lo[c] = uniform_f(mn);
+        b.hi[c] = uniform_f(mx);
+    }
+    return b;
+}
+
+// One LDS tile of n <= 256 sources against the lane's target (F32C).
+// a = (s'_hi, far threshold), b = (Gamma', 1/sigma), l = s'_lo.
+// Each wave first splits the tile with one ballot per 64 sources: a source
+// is near when its distance to the wave's target box is below q_far sigma
+// (then some lane may be near; the test is conservative).  The far and the
+// near sources are then visited by iterating over the set bits of the two
+// masks (scalar loop control), so the far loop body is straight-line.
+template <int K>
+__device__ __forceinline__ void tile_f32c(int n, const float4* lpos, const float4* lgs, const float4* llo,
+                                          float tx, float ty, float tz, float tlx, float tly, float tlz,
+                                          const WaveBox& box, float (&acc)[kAccF32])
+{
+    const int lane = threadIdx.x & 63;
+    uint64_t near[kTile / 64], far[kTile / 64];
+#pragma unroll
+    for (int g = 0; g < kTile / 64; ++g) {
+        const int s = g * 64 + lane;
+        bool nr = false, ok = s < n;
+        if (ok) {
+            const float4 a = lpos[s];
+            const float ex = fmaxf(fmaxf(box.lo[0] - a.x, a.x - box.hi[0]), 0.f);
+            const float ey = fmaxf(fmaxf(box.lo[1] - a.y, a.y - box.hi[1]), 0.f);
+            const float ez = fmaxf(fmaxf(box.lo[2] - a.z, a.z - box.hi[2]), 0.f);
+            // 1e-4 margin covers the FP32 rounding of the box test
+            nr = fmaf(ez, ez, fmaf(ey, ey, ex * ex)) < a.w * 1.0001f;
+        }
+        near[g] = __ballot(nr);
+        far[g] = __ballot(ok && !nr);
+    }
+#pragma unroll
+    for (int g = 0; g < kTile / 64; ++g) {
+        uint64_t fb = far[g];
+        while (fb) {
+            const int s = g * 64 + __builtin_ctzll(fb);
+            fb &= fb - 1;
+            far_pair<K>(tx, ty, tz, lpos[s], lgs[s], acc);
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < kTile / 64; ++g) {
+        uint64_t nb = near[g];
+        while (nb) {
+            const int s = g * 64 + __builtin_ctzll(nb);
+            nb &= nb - 1;
+            near_pair<K>(tx, ty, tz, tlx, tly, tlz, lpos[s], lgs[s], llo[s], acc);
+        }
+    }
 }
 
 // hi/lo split of a position relative to the block origin

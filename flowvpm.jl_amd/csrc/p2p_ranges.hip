@@ -44,6 +44,7 @@ __global__ __launch_bounds__(kBlock) void ranges_f32c_kernel(const RangeItem* __
     split_rel(pos[tt], ox, tx, tlx);
     split_rel(pos[ld + tt], oy, ty, tly);
     split_rel(pos[2 * ld + tt], oz, tz, tlz);
+    const WaveBox box = wave_box(tx, ty, tz);
     float sum[kAccF32], err[kAccF32];
 #pragma unroll
     for (int c = 0; c < kAccF32; ++c) { sum[c] = 0.f; err[c] = 0.f; }
@@ -68,7 +69,7 @@ __global__ __launch_bounds__(kBlock) void ranges_f32c_kernel(const RangeItem* __
             float acc[kAccF32];
 #pragma unroll
             for (int c = 0; c < kAccF32; ++c) acc[c] = 0.f;
-            for (int s = 0; s < n; ++s) pair_f32c<K>(tx, ty, tz, tlx, tly, tlz, lpos[s], lgs[s], &llo[s], acc);
+            tile_f32c<K>(n, lpos, lgs, llo, tx, ty, tz, tlx, tly, tlz, box, acc);
             fold_twosum(acc, sum, err);
         }
     }

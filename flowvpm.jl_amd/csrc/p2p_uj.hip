@@ -37,10 +37,6 @@
 
 #include "p2p_pair.h"
 
-// Source-loop unroll of the F32C kernel (register pressure vs. issue overlap)
-#ifndef VPM_UJ_UNROLL
-#define VPM_UJ_UNROLL 2
-#endif
 
 namespace vpm {
 
@@ -67,6 +63,7 @@ __global__ __launch_bounds__(kBlock) void uj_f32c_kernel(
     split_rel(tpos[tld + it], oy, ty, tly);
     split_rel(tpos[2 * tld + it], oz, tz, tlz);
 
+    const WaveBox box = wave_box(tx, ty, tz);
     const int64_t sb = s0 + (int64_t)blockIdx.y * chunk;
     const int64_t se = min(sb + chunk, s1);
 
@@ -98,8 +95,7 @@ __global__ __launch_bounds__(kBlock) void uj_f32c_kernel(
         float acc[kAccF32];
 #pragma unroll
         for (int c = 0; c < kAccF32; ++c) acc[c] = 0.f;
-#pragma unroll VPM_UJ_UNROLL
-        for (int s = 0; s < kTile; ++s) pair_f32c<K>(tx, ty, tz, tlx, tly, tlz, lpos[s], lgs[s], &llo[s], acc);
+        tile_f32c<K>((int)min((int64_t)kTile, se - base), lpos, lgs, llo, tx, ty, tz, tlx, tly, tlz, box, acc);
         fold_twosum(acc, sum, err);
     }
     if (active) {
