@@ -57,6 +57,23 @@ def cpu_baseline(P, kernel, budget_s=12.0):
                       "%.1f s" % (nt, n, dt)}, Q[:nt], nt
 
 
+def copy_calibration(mib):
+    """One device-to-device copy of `mib` MiB through the HIP runtime: a
+    plain copy kernel of known traffic (reads and writes mib MiB), used to
+    scale the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = C.c_size_t(int(mib) << 20)
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), nbytes) == 0 and hip.hipMalloc(C.byref(b), nbytes) == 0
+    assert hip.hipMemset(a, 1, nbytes) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, nbytes, 3) == 0          # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +84,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f64", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run one device copy of this size (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: one GPU only; --gpus %d is not supported (the all-pairs path would shard "
@@ -163,6 +182,9 @@ def main():
         ej = vpm.metrics.err_stats(out32[:nt, 15:24], ref_rows[:, 15:24])
         res["accuracy_f32c_vs_cpu_oracle"] = {"targets": nt, "U": eu, "J": ej}
     f.close()
+    if args.copy_mib:
+        copy_calibration(args.copy_mib)
+        res["copy_calibration_mib"] = args.copy_mib
     print(json.dumps(res))
 
 
