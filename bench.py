@@ -28,6 +28,10 @@ METRIC = "UJ pair-interactions/s FP32 vs FP64 at ≤1e-5 rel err; adjoint/forwar
 FLOP_PER_PAIR = {"gaussianerf": 88, "winckelmans": 97, "gaussian": 86, "singular": 79}
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
+# HBM bytes per U/J launch from two rocprofv3 --pmc runs of this script
+# (FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0, the factors a 256 MiB
+# device copy reads back), keyed by (N, kernel)
+TRAFFIC_BYTES = {(1 << 20, "gaussianerf"): (1.158e7 * 2.0 + 1.147e5) * 1024}
 
 
 def log(*a):
@@ -132,8 +136,10 @@ def main():
                                "arithmetic with compensated sums, state resident in HBM" % (n, kernel),
                    "N": n, "kernel": kernel, "pairs_per_step": pairs, "flop_per_pair": FLOP_PER_PAIR[kernel]},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32, "traffic": None,
-                     "kernel": "uj_f32c_kernel<%s>" % kernel, "avg_launch_ms": 1e3 * k_avg_s},
+                     "frac": achieved / PEAK_FP32, "traffic": TRAFFIC_BYTES.get((n, kernel)),
+                     "kernel": "uj_f32c_kernel<%s>" % kernel, "avg_launch_ms": 1e3 * k_avg_s,
+                     "traffic_source": "profiles/uj_f32c_gaussianerf_jet_n1048576.md (rocprofv3 FETCH_SIZE x2.0 + "
+                                       "WRITE_SIZE x1.0, copy-kernel calibrated)"},
     }
 
     if not args.no_f64:
