@@ -1,0 +1,171 @@
+#=
+FLOWVPMHIP -- Julia binding of libflowvpm_hip.so (include/flowvpm_hip.h) for
+FLOWVPM.jl.  Thin ccall layer: every numerical call runs on the MI355X.
+
+    using FLOWVPM, FLOWVPMHIP
+    pfield = FLOWVPM.ParticleField(n; UJ=FLOWVPMHIP.UJ_hip, kernel=FLOWVPM.gaussianerf)
+
+`UJ_hip` has the keyword signature of FLOWVPM.UJ_direct (src/FLOWVPM_UJ.jl:21-37)
+and mutates pfield.particles in place; `direct_hip!` is the FastMultipole
+leaf-pair hook (src/FLOWVPM_fmm.jl:102-168); `rungekutta3_hip` runs one full
+rungekutta3 step on device-resident state (src/FLOWVPM_timeintegration.jl:388-461);
+the rrule of `uj_hip` replaces the ChainRules pullback of src/FLOWVPM_rrules.jl.
+=#
+module FLOWVPMHIP
+
+import FLOWVPM
+import ChainRulesCore
+
+const vpm = FLOWVPM
+const libvpm = get(ENV, "FLOWVPM_HIP_LIB", joinpath(@__DIR__, "..", "lib", "libflowvpm_hip.so"))
+
+const VPM_OK = Cint(0)
+const VPM_UJ_RESET, VPM_UJ_RESET_SFS, VPM_UJ_SFS = UInt32(1), UInt32(2), UInt32(4)
+
+struct VpmConfig
+    kernel::Int32
+    precision::Int32      # 0 FP32 pairs + compensated sums, 1 FP64
+    transposed::Int32
+    reserved::Int32
+end
+
+mutable struct VpmStepConfig
+    uj::VpmConfig
+    f::Float64; g::Float64; Uinf::NTuple{3,Float64}
+    sfs::Int32; sfs_force_positive::Int32; sfs_clip::Int32; relax::Int32
+    sfs_alpha::Float64; sfs_rlxf::Float64; sfs_minC::Float64; sfs_maxC::Float64
+    rlxf::Float64
+    viscous::Int32; reserved::Int32
+    nu::Float64; sgm0::Float64; beta::Float64; t_sgm::Float64
+end
+
+check(rc) = rc == VPM_OK ? nothing :
+    error("libflowvpm_hip: ", unsafe_string(ccall((:vpm_last_error, libvpm), Cstring, ())))
+
+function kernel_id(k)
+    k === vpm.kernel_singular    && return Int32(0)
+    k === vpm.kernel_gaussian    && return Int32(1)
+    k === vpm.kernel_gaussianerf && return Int32(2)
+    k === vpm.kernel_winckelmans && return Int32(3)
+    error("kernel not provided by libflowvpm_hip")
+end
+
+config(pfield; precision=0) = VpmConfig(kernel_id(pfield.kernel), Int32(precision), Int32(pfield.transposed), 0)
+
+# One device field per ParticleField, kept across calls (no re-allocation).
+const FIELDS = IdDict{Any,Ptr{Cvoid}}()
+
+function device_field(pfield)
+    h = get(FIELDS, pfield, C_NULL)
+    if h == C_NULL
+        r = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:vpm_create, libvpm), Cint, (Cint, Int64, Ref{Ptr{Cvoid}}),
+                    pfield.useGPU > 0 ? pfield.useGPU - 1 : 0, pfield.maxparticles, r))
+        h = r[]
+        FIELDS[pfield] = h
+        finalizer(p -> (ccall((:vpm_destroy, libvpm), Cint, (Ptr{Cvoid},), FIELDS[p]); delete!(FIELDS, p)), pfield)
+    end
+    return h
+end
+
+upload(pfield) = GC.@preserve pfield check(ccall((:vpm_set_particles, libvpm), Cint,
+    (Ptr{Cvoid}, Ptr{Float64}, Int64), device_field(pfield), pfield.particles, pfield.np))
+
+download(pfield, row0, row1) = GC.@preserve pfield check(ccall((:vpm_get_particles, libvpm), Cint,
+    (Ptr{Cvoid}, Ptr{Float64}, Int64, Int32, Int32), device_field(pfield), pfield.particles, pfield.np, row0, row1))
+
+"""
+    UJ_hip(pfield; rbf=false, sfs=false, reset=true, reset_sfs=false, precision=0)
+
+UJ_direct semantics on the GPU: reset U, vorticity, J, PSE of non-static
+particles; SFS reset if `reset_sfs`; all-pairs U and J; E_str if `sfs`.
+"""
+function UJ_hip(pfield::vpm.ParticleField; rbf::Bool=false, sfs::Bool=false,
+                reset=true, reset_sfs=false, precision=0, optargs...)
+    pfield.np == 0 && return nothing
+    upload(pfield)
+    flags = (reset ? VPM_UJ_RESET : 0x0) | (reset_sfs ? VPM_UJ_RESET_SFS : 0x0) | (sfs ? VPM_UJ_SFS : 0x0)
+    cfg = Ref(config(pfield; precision))
+    check(ccall((:vpm_uj_direct, libvpm), Cint, (Ptr{Cvoid}, Ref{VpmConfig}, UInt32),
+                device_field(pfield), cfg, flags))
+    download(pfield, Int32(9), Int32(27))                # U, vorticity, J, PSE (0-based rows)
+    (sfs || reset_sfs) && download(pfield, Int32(39), Int32(42))
+    return nothing
+end
+
+"""
+    direct_hip!(pfield, target_ranges, source_ranges; precision=0)
+
+FastMultipole leaf-pair P2P: for each k, the particles in source_ranges[k]
+act on the particles in target_ranges[k] (1-based UnitRanges), accumulating
+U and J as fmm.direct! + buffer_to_target_system! do.
+"""
+function direct_hip!(pfield::vpm.ParticleField, target_ranges, source_ranges; precision=0)
+    n = length(target_ranges)
+    tb = Int64[first(r) - 1 for r in target_ranges]; te = Int64[last(r) for r in target_ranges]
+    sb = Int64[first(r) - 1 for r in source_ranges]; se = Int64[last(r) for r in source_ranges]
+    upload(pfield)
+    cfg = Ref(config(pfield; precision))
+    check(ccall((:vpm_p2p_ranges, libvpm), Cint,
+                (Ptr{Cvoid}, Ref{VpmConfig}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Int64),
+                device_field(pfield), cfg, tb, te, sb, se, n))
+    download(pfield, Int32(9), Int32(24))
+    return nothing
+end
+
+"""
+    rungekutta3_hip(pfield, dt; relax=false, precision=0)
+
+One rungekutta3 step of ReformulatedVPM with UJ = UJ_direct, entirely on the
+device (pfield.particles is uploaded once and downloaded once).
+"""
+function rungekutta3_hip(pfield::vpm.ParticleField, dt::Real; relax::Bool=false, precision=0, optargs...)
+    upload(pfield)
+    sfs = pfield.SFS isa vpm.DynamicSFS
+    visc = pfield.viscous isa vpm.CoreSpreading
+    rl = pfield.relaxation
+    method = rl.relax === vpm.relax_pedrizzetti ? 1 : rl.relax === vpm.relax_correctedpedrizzetti ? 2 : 0
+    s = Ref(VpmStepConfig(config(pfield; precision), pfield.formulation.f, pfield.formulation.g,
+            Tuple(pfield.Uinf(pfield.t)),
+            Int32(sfs), Int32(sfs), Int32(sfs && !isempty(pfield.SFS.clippings)), Int32(method),
+            sfs ? pfield.SFS.alpha : 0.0, sfs ? pfield.SFS.rlxf : 0.0,
+            sfs ? pfield.SFS.minC : 0.0, sfs ? pfield.SFS.maxC : 1.0, rl.rlxf,
+            Int32(visc), 0, visc ? pfield.viscous.nu : 0.0, visc ? pfield.viscous.sgm0 : 1.0,
+            visc ? pfield.viscous.beta : 1.5, visc ? pfield.viscous.t_sgm : 0.0))
+    rc = ccall((:vpm_rk3_step, libvpm), Cint, (Ptr{Cvoid}, Ref{VpmStepConfig}, Float64, Int32),
+               device_field(pfield), s, dt, Int32(relax))
+    visc && (pfield.viscous.t_sgm = s[].t_sgm)
+    download(pfield, Int32(0), Int32(46))
+    check(rc)
+    return nothing
+end
+
+"""
+    uj_hip(pfield) -> (U::Matrix{Float64}(3,np), J::Matrix{Float64}(9,np))
+
+Pure function of (X, Gamma, sigma) for AD; its rrule calls the hand-written
+adjoint kernel instead of differentiating through the pair loop.
+"""
+function uj_hip(pfield::vpm.ParticleField)
+    UJ_hip(pfield)
+    return pfield.particles[10:12, 1:pfield.np], pfield.particles[16:24, 1:pfield.np]
+end
+
+function ChainRulesCore.rrule(::typeof(uj_hip), pfield::vpm.ParticleField)
+    U, J = uj_hip(pfield)
+    function uj_hip_pullback((Ubar, Jbar))
+        n = pfield.np
+        Ub = Matrix{Float64}(ChainRulesCore.unthunk(Ubar)); Jb = Matrix{Float64}(ChainRulesCore.unthunk(Jbar))
+        Xb = zeros(3, n); Gb = zeros(3, n); Sb = zeros(n)
+        cfg = Ref(config(pfield; precision=1))
+        check(ccall((:vpm_uj_adjoint, libvpm), Cint,
+                    (Ptr{Cvoid}, Ref{VpmConfig}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                    device_field(pfield), cfg, Ub, Jb, Xb, Gb, Sb))
+        dparticles = zeros(size(pfield.particles))
+        dparticles[1:3, 1:n] .= Xb; dparticles[4:6, 1:n] .= Gb; dparticles[7, 1:n] .= Sb
+        return ChainRulesCore.NoTangent(), ChainRulesCore.Tangent{typeof(pfield)}(particles=dparticles)
+    end
+    return (U, J), uj_hip_pullback
+end
+
+end # module
