@@ -13,11 +13,12 @@ __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x);
 __device__ __forceinline__ float ex2(float x) { return __builtin_amdgcn_exp2f(x); }
 constexpr float kLog2e = 1.4426950408889634f;
 
-__device__ __forceinline__ float horner8(const float (&c)[8], float x)
+template <int N>
+__device__ __forceinline__ float horner(const float (&c)[N], float x)
 {
-    float a = c[7];
+    float a = c[N - 1];
 #pragma unroll
-    for (int k = 6; k >= 0; --k) a = fmaf(a, x, c[k]);
+    for (int k = N - 2; k >= 0; --k) a = fmaf(a, x, c[k]);
     return a;
 }
 
@@ -68,8 +69,8 @@ __device__ __forceinline__ void coef_full<VPM_KERNEL_GAUSSIANERF>(float r2, floa
     const float t = q * q;
     const float s3 = sinv * sinv * sinv;
     const float s5 = s3 * sinv * sinv;
-    const float Fs = horner8(kErfPhi0, t) * s3;
-    const float Hs = horner8(kErfPhi1, t) * s5;
+    const float Fs = horner(kErfPhi0, t) * s3;
+    const float Hs = horner(kErfPhi1, t) * s5;
     const float E = ex2(-0.5f * kLog2e * t);
     const float c2qE = (float)kConst2 * q * E;
     const float g = erff(q * 0.70710678118654752f) - c2qE;
@@ -91,8 +92,8 @@ __device__ __forceinline__ void coef_full<VPM_KERNEL_GAUSSIAN>(float r2, float s
     const float u = q * q * q;
     const float s3 = sinv * sinv * sinv;
     const float s5 = s3 * sinv * sinv;
-    const float Fs = horner8(kGauPhi0, u) * s3;
-    const float Hs = q * horner8(kGauPsi, u) * s5;
+    const float Fs = horner(kGauPhi0, u) * s3;
+    const float Hs = q * horner(kGauPsi, u) * s5;
     const float E = ex2(-kLog2e * u);
     const float ri2 = ri * ri;
     const float ri3 = ri2 * ri;
