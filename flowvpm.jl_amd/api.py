@@ -171,6 +171,19 @@ class Field:
         cfg = make_config(kernel, precision, transposed)
         check(lib().vpm_estr_direct(self.handle, C.byref(cfg)))
 
+    def uj_adjoint(self, Ubar, Jbar, kernel=kernel_default, precision="f32c"):
+        """vpm_uj_adjoint on the resident particles: returns (dL/dX (n,3),
+        dL/dGamma (n,3), dL/dsigma (n,)) for L = sum Ubar.U + sum Jbar:J."""
+        n = self.np
+        ub = np.ascontiguousarray(Ubar, dtype=np.float64).reshape(n * 3)
+        jb = np.ascontiguousarray(Jbar, dtype=np.float64).reshape(n * 9)
+        Xb, Gb, Sb = np.zeros(3 * n), np.zeros(3 * n), np.zeros(n)
+        cfg = make_config(kernel, precision, True)
+        d = _lib._dptr
+        check(lib().vpm_uj_adjoint(self.handle, C.byref(cfg), ub.ctypes.data_as(d), jb.ctypes.data_as(d),
+                                   Xb.ctypes.data_as(d), Gb.ctypes.data_as(d), Sb.ctypes.data_as(d)))
+        return Xb.reshape(n, 3), Gb.reshape(n, 3), Sb
+
     def profile(self, enable=True):
         check(lib().vpm_profile(self.handle, int(bool(enable))))
 
@@ -449,22 +462,93 @@ def run_vpm(pfield, dt, nsteps, *, runtime_function=None, resident=True):
 
 
 # ---------------------------------------------------------------- adjoint
-def uj_adjoint(pfield, Ubar, Jbar, *, kernel=None, device=None):
+def uj_adjoint(pfield, Ubar, Jbar, *, kernel=None, precision="f32c", device=None):
     """Gradient of L = sum Ubar.U + sum Jbar:J through one all-pairs U/J
-    evaluation: returns (dL/dX (n,3), dL/dGamma (n,3), dL/dsigma (n,))."""
+    evaluation: returns (dL/dX (n,3), dL/dGamma (n,3), dL/dsigma (n,)).
+    precision "f32c": fused FP32 pair kernel with FP64 partial sums;
+    "f64": FP64 pair arithmetic."""
     P = _as_matrix(pfield)
     kernel = kernel or (pfield.kernel if isinstance(pfield, ParticleField) else kernel_default)
     n = P.shape[0]
     f = _scratch_field(n, 0 if device is None else device)
     f.set_particles(P)
-    ub = np.ascontiguousarray(Ubar, dtype=np.float64).reshape(n * 3)
-    jb = np.ascontiguousarray(Jbar, dtype=np.float64).reshape(n * 9)
-    Xb, Gb, Sb = np.zeros(3 * n), np.zeros(3 * n), np.zeros(n)
-    cfg = make_config(kernel, "f64", True)
-    d = _lib._dptr
-    check(lib().vpm_uj_adjoint(f.handle, C.byref(cfg), ub.ctypes.data_as(d), jb.ctypes.data_as(d),
-                               Xb.ctypes.data_as(d), Gb.ctypes.data_as(d), Sb.ctypes.data_as(d)))
-    return Xb.reshape(n, 3), Gb.reshape(n, 3), Sb
+    return f.uj_adjoint(Ubar, Jbar, kernel, precision)
+
+
+class Tape:
+    """Per-stage checkpoints of one rk3 step of a device field (vpm_tape):
+    the particle state after each stage's UJ evaluation, in device memory."""
+
+    def __init__(self, field):
+        self.field = field
+        h = C.c_void_p()
+        check(lib().vpm_tape_create(field.handle, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise ValueError("tape is closed")
+        return self._h
+
+    def close(self):
+        if self._h is not None:
+            lib().vpm_tape_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rungekutta3_taped(pfield, dt, tape):
+    """rungekutta3 on the device-resident state (ReformulatedVPM without SFS,
+    inviscid, no relaxation) that records its stage states into tape."""
+    f = pfield.device_field()
+    s = step_config(pfield)
+    check(lib().vpm_rk3_step_taped(f.handle, C.byref(s), float(dt), tape.handle))
+
+
+def rk3_adjoint(tape, bar):
+    """Pullback of the step recorded in tape, in place on bar ((n, 46)
+    cotangent records: after the step in, before the step out)."""
+    bar = np.ascontiguousarray(bar, dtype=np.float64)
+    check(lib().vpm_rk3_step_adjoint(tape.handle, bar.ctypes.data_as(_lib._dptr)))
+    return bar
+
+
+def kinetic_energy_rollout_gradient(pfield, dt, nsteps):
+    """Gradient of L = sum_i |U_i|^2 after nsteps rk3 steps with respect to
+    X, Gamma and sigma of the initial state (pfield.particles): nsteps taped
+    steps (per-stage checkpoints on the device), one U/J evaluation of the
+    final state, then the adjoint back through the final U/J evaluation and
+    every recorded stage.  Returns (L, bar) with bar the (n, 46) cotangent
+    records (rows X, Gamma, sigma).  pfield is left at the initial state."""
+    n = pfield.np
+    f = pfield.device_field()
+    f.set_particles(pfield.particles[:n])
+    tapes = []
+    try:
+        for _ in range(nsteps):
+            tapes.append(Tape(f))
+            rungekutta3_taped(pfield, dt, tapes[-1])
+        f.uj_direct(pfield.kernel, pfield.precision, transposed=pfield.transposed)
+        out = np.zeros((n, NFIELDS))
+        f.get_particles(out, rows=(9, 12))
+        U = out[:, 9:12]
+        L = float(np.sum(U * U))
+        Xb, Gb, Sb = f.uj_adjoint(2.0 * U, np.zeros((n, 9)), pfield.kernel, pfield.precision)
+        bar = np.zeros((n, NFIELDS))
+        bar[:, 0:3], bar[:, 3:6], bar[:, 6] = Xb, Gb, Sb
+        for t in reversed(tapes):
+            bar = rk3_adjoint(t, bar)
+    finally:
+        for t in tapes:
+            t.close()
+    f.set_particles(pfield.particles[:n])
+    return L, bar
 
 
 def zeta0(kernel):

@@ -16,8 +16,9 @@
 // target) and one with lanes on sources (dL/dx as source, dL/dGamma,
 // dL/dsigma), each streaming the other side through LDS.
 //
-// This version runs the pair arithmetic in FP64 with the closed-form
-// derivatives of g (the FP64 reference of the adjoint).
+// The kernels below run the pair arithmetic in FP64 with the closed-form
+// derivatives of g (VPM_PREC_F64, the FP64 reference of the adjoint); the
+// fused FP32 kernel of VPM_PREC_F32C is in adjoint_f32c.hip.
 #include "vpm_internal.h"
 
 namespace vpm {
@@ -193,32 +194,41 @@ __global__ __launch_bounds__(kBlock) void adj_source_kernel(const double* __rest
     }
 }
 
-int launch_uj_adjoint(vpm_field f, int kernel, const double* Ubar, const double* Jbar, double* Xbar, double* Gbar,
-                      double* Sbar)
+int launch_uj_adjoint_f32c(vpm_field f, int kernel, const double* dU, const double* dJ, double* dX, double* dG,
+                           double* dS, float4* tcot);
+
+// Device staging of the U/J adjoint: Ubar 3n, Jbar 9n, Xbar 3n, Gbar 3n,
+// Sbar n (doubles), then the sorted FP32 cotangents of the F32C path (4n
+// float4, 16-byte aligned).
+int adj_buffers(vpm_field f, AdjBufs& b)
+{
+    const size_t n = (size_t)f->np;
+    const size_t cot_off = (19 * n + 1) & ~(size_t)1;
+    if (int rc = ensure(f->adj, sizeof(double) * (cot_off + 8 * n + 2))) return rc;
+    b.U = (double*)f->adj.p;
+    b.J = b.U + 3 * n;
+    b.X = b.J + 9 * n;
+    b.G = b.X + 3 * n;
+    b.S = b.G + 3 * n;
+    b.cot = (float4*)(b.U + cot_off);
+    return VPM_OK;
+}
+
+// U/J adjoint of the field's current particles with device-resident
+// cotangents b.U, b.J; gradients into b.X, b.G, b.S (particle order).
+int launch_uj_adjoint_dev(vpm_field f, int kernel, int precision, const AdjBufs& b)
 {
     const int64_t np = f->np;
     if (np == 0) return VPM_OK;
-    int rc;
-    // device copies: Ubar 3n, Jbar 9n, Xbar 3n, Gbar 3n, Sbar n
-    const size_t n = (size_t)np;
-    if ((rc = ensure(f->adj, sizeof(double) * 19 * n))) return rc;
-    double* dU = (double*)f->adj.p;
-    double* dJ = dU + 3 * n;
-    double* dX = dJ + 9 * n;
-    double* dG = dX + 3 * n;
-    double* dS = dG + 3 * n;
-    hipError_t e;
-    if ((e = hipMemcpyAsync(dU, Ubar, sizeof(double) * 3 * n, hipMemcpyHostToDevice, f->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(dJ, Jbar, sizeof(double) * 9 * n, hipMemcpyHostToDevice, f->stream)) != hipSuccess)
-        return hip_fail(e, "hipMemcpyAsync(cotangents)");
+    if (precision != VPM_PREC_F64) return launch_uj_adjoint_f32c(f, kernel, b.U, b.J, b.X, b.G, b.S, b.cot);
     const dim3 grid((unsigned)((np + kBlock - 1) / kBlock));
     timing_begin(f);
     switch (kernel) {
-#define VPM_ADJ(KID)                                                                                          \
-    case KID:                                                                                                 \
-        hipLaunchKernelGGL(adj_target_kernel<KID>, grid, dim3(kBlock), 0, f->stream, f->P, np, dU, dJ, dX);   \
-        hipLaunchKernelGGL(adj_source_kernel<KID>, grid, dim3(kBlock), 0, f->stream, f->P, np, dU, dJ, dX, dG, \
-                           dS);                                                                               \
+#define VPM_ADJ(KID)                                                                                            \
+    case KID:                                                                                                   \
+        hipLaunchKernelGGL(adj_target_kernel<KID>, grid, dim3(kBlock), 0, f->stream, f->P, np, b.U, b.J, b.X);  \
+        hipLaunchKernelGGL(adj_source_kernel<KID>, grid, dim3(kBlock), 0, f->stream, f->P, np, b.U, b.J, b.X,   \
+                           b.G, b.S);                                                                           \
         break;
         VPM_ADJ(VPM_KERNEL_SINGULAR) VPM_ADJ(VPM_KERNEL_GAUSSIAN) VPM_ADJ(VPM_KERNEL_GAUSSIANERF)
         VPM_ADJ(VPM_KERNEL_WINCKELMANS)
@@ -226,12 +236,28 @@ int launch_uj_adjoint(vpm_field f, int kernel, const double* Ubar, const double*
     default:
         return fail(VPM_EINVAL, "unknown kernel id");
     }
-    e = hipGetLastError();
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "adjoint kernels");
     timing_end(f, 3);
-    if ((e = hipMemcpyAsync(Xbar, dX, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, f->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(Gbar, dG, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, f->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(Sbar, dS, sizeof(double) * n, hipMemcpyDeviceToHost, f->stream)) != hipSuccess)
+    return VPM_OK;
+}
+
+int launch_uj_adjoint(vpm_field f, int kernel, int precision, const double* Ubar, const double* Jbar, double* Xbar,
+                      double* Gbar, double* Sbar)
+{
+    const int64_t np = f->np;
+    if (np == 0) return VPM_OK;
+    const size_t n = (size_t)np;
+    AdjBufs b;
+    if (int rc = adj_buffers(f, b)) return rc;
+    hipError_t e;
+    if ((e = hipMemcpyAsync(b.U, Ubar, sizeof(double) * 3 * n, hipMemcpyHostToDevice, f->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(b.J, Jbar, sizeof(double) * 9 * n, hipMemcpyHostToDevice, f->stream)) != hipSuccess)
+        return hip_fail(e, "hipMemcpyAsync(cotangents)");
+    if (int rc = launch_uj_adjoint_dev(f, kernel, precision, b)) return rc;
+    if ((e = hipMemcpyAsync(Xbar, b.X, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, f->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(Gbar, b.G, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, f->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(Sbar, b.S, sizeof(double) * n, hipMemcpyDeviceToHost, f->stream)) != hipSuccess)
         return hip_fail(e, "hipMemcpyAsync(gradients)");
     return VPM_OK;
 }

@@ -212,6 +212,108 @@ __global__ __launch_bounds__(kBlock) void uprev_kernel(double* __restrict__ P, i
     p[RUPREV] = sqrt(p[RU] * p[RU] + p[RU + 1] * p[RU + 1] + p[RU + 2] * p[RU + 2]);
 }
 
+// Pullback of update_states_kernel for ReformulatedVPM without SFS (C S = 0;
+// the state_pullback of rrules.jl:436-555, derived from the update
+// timeintegration.jl:467-528 itself).  With m1 = Mbar1' + b Xbar',
+// m23 = Mbar23' + b sigmabar', m2 = Mbar2' + b Gammabar', MM = stretching,
+// G2 = |Gamma|^2, MM4 = A (MM . Gamma)/G2, A = (f + g)/(1 + 3 f):
+//   Xbar = Xbar', Mbar1 = a m1, Ubar = dt m1, Mbar23 = a m23,
+//   sigmabar = sigmabar' - dt MM4 m23, Mbar2 = a m2,
+//   MM4bar = -dt sigma m23 - 3 dt Gamma . m2,
+//   MMbar = dt m2 + MM4bar A Gamma/G2,
+//   Gammabar = Gammabar' - 3 dt MM4 m2 + MM4bar (A MM - 2 MM4 Gamma)/G2 + (dMM/dGamma)^T MMbar,
+//   Jbar = (dMM/dJ)^T MMbar
+// (the full sum over components in d(MM4 Gamma)/dJ; rrules.jl:521 keeps only
+// the diagonal term).  B: cotangent records (NF rows per particle) of the
+// updated state in, of the state before the update out; P: the state before
+// the update with its U and J.  Ub (3 np) and Jb (9 np) receive Ubar, Jbar.
+__global__ __launch_bounds__(kBlock) void update_states_adjoint_kernel(
+    const double* __restrict__ P, double* __restrict__ B, int64_t np, double a, double b, double dt, double f,
+    double g, int transposed, double* __restrict__ Ub, double* __restrict__ Jb)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= np) return;
+    const double* p = P + i * NF;
+    double* q = B + i * NF;
+    if (p[RSTATIC] != 0) {
+        for (int k = 0; k < 3; ++k) Ub[3 * i + k] = 0.0;
+        for (int k = 0; k < 9; ++k) Jb[9 * i + k] = 0.0;
+        return;
+    }
+    const double A = (f + g) / (1 + 3 * f);
+    const double* G = p + RG;
+    const double* J = p + RJ;
+    const double sgm = p[RSIG];
+    double MM[3];
+    stretch(J, G, transposed, MM);
+    const double G2 = G[0] * G[0] + G[1] * G[1] + G[2] * G[2];
+    const double MM4 = G2 > 0 ? A * (MM[0] * G[0] + MM[1] * G[1] + MM[2] * G[2]) / G2 : 0.0;
+    double m1[3], m2[3];
+    for (int k = 0; k < 3; ++k) {
+        m1[k] = q[RM + k] + b * q[RX + k];
+        m2[k] = q[RM + 3 + k] + b * q[RG + k];
+    }
+    const double m23 = q[RM + 7] + b * q[RSIG];
+    const double MM4b = -dt * sgm * m23 - 3 * dt * (G[0] * m2[0] + G[1] * m2[1] + G[2] * m2[2]);
+    double MMb[3], Gb[3];
+    for (int k = 0; k < 3; ++k) {
+        q[RM + k] = a * m1[k];
+        Ub[3 * i + k] = dt * m1[k];
+        q[RM + 3 + k] = a * m2[k];
+        MMb[k] = dt * m2[k];
+        Gb[k] = q[RG + k] - 3 * dt * MM4 * m2[k];
+    }
+    q[RM + 7] = a * m23;
+    q[RSIG] = q[RSIG] - dt * MM4 * m23;
+    if (G2 > 0) {
+        for (int k = 0; k < 3; ++k) {
+            MMb[k] += MM4b * A * G[k] / G2;
+            Gb[k] += MM4b * (A * MM[k] - 2 * MM4 * G[k]) / G2;
+        }
+    }
+    double jb[9];
+    for (int k = 0; k < 9; ++k) jb[k] = 0.0;
+    // MM[e] = sum_x J(x, e) G[x] (transposed) or J(e, x) G[x]; J(r, c) = J[3 c + r]
+    for (int e = 0; e < 3; ++e)
+        for (int x = 0; x < 3; ++x) {
+            const int idx = transposed ? 3 * e + x : 3 * x + e;
+            jb[idx] += MMb[e] * G[x];
+            Gb[x] += J[idx] * MMb[e];
+        }
+    for (int k = 0; k < 9; ++k) Jb[9 * i + k] = jb[k];
+    for (int k = 0; k < 3; ++k) q[RG + k] = Gb[k];
+}
+
+// B[i] += (dX, dGamma, dsigma) of the U/J adjoint
+__global__ __launch_bounds__(kBlock) void adj_add_bar_kernel(double* __restrict__ B, int64_t np,
+                                                             const double* __restrict__ dX,
+                                                             const double* __restrict__ dG,
+                                                             const double* __restrict__ dS)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= np) return;
+    double* q = B + i * NF;
+    for (int k = 0; k < 3; ++k) {
+        q[RX + k] += dX[3 * i + k];
+        q[RG + k] += dG[3 * i + k];
+    }
+    q[RSIG] += dS[i];
+}
+
+// Cotangent of the state before an rk3 step: the step starts from M = 0 for
+// non-static particles (timeintegration.jl:401-414), and U, J are outputs.
+__global__ __launch_bounds__(kBlock) void adj_step_start_kernel(const double* __restrict__ P, double* __restrict__ B,
+                                                                int64_t np)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= np) return;
+    double* q = B + i * NF;
+    for (int k = 0; k < 3; ++k) q[RU + k] = 0.0;
+    for (int k = 0; k < 9; ++k) q[RJ + k] = 0.0;
+    if (P[i * NF + RSTATIC] == 0)
+        for (int k = 0; k < 9; ++k) q[RM + k] = 0.0;
+}
+
 static inline unsigned nb(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 #define VPM_CHECK_LAUNCH(what)                                                  \
@@ -295,6 +397,32 @@ int launch_update_uprev(vpm_field f)
     if (f->np == 0) return VPM_OK;
     hipLaunchKernelGGL(uprev_kernel, dim3(nb(f->np)), dim3(kBlock), 0, f->stream, f->P, f->np);
     VPM_CHECK_LAUNCH("uprev");
+    return VPM_OK;
+}
+
+int launch_update_states_adjoint(vpm_field f, double* B, double a, double b, double dt, double fr, double gr,
+                                 int transposed, double* Ub, double* Jb)
+{
+    if (f->np == 0) return VPM_OK;
+    hipLaunchKernelGGL(update_states_adjoint_kernel, dim3(nb(f->np)), dim3(kBlock), 0, f->stream, f->P, B, f->np, a,
+                       b, dt, fr, gr, transposed, Ub, Jb);
+    VPM_CHECK_LAUNCH("update_states_adjoint");
+    return VPM_OK;
+}
+
+int launch_adj_add_bar(vpm_field f, double* B, const double* dX, const double* dG, const double* dS)
+{
+    if (f->np == 0) return VPM_OK;
+    hipLaunchKernelGGL(adj_add_bar_kernel, dim3(nb(f->np)), dim3(kBlock), 0, f->stream, B, f->np, dX, dG, dS);
+    VPM_CHECK_LAUNCH("adj_add_bar");
+    return VPM_OK;
+}
+
+int launch_adj_step_start(vpm_field f, double* B)
+{
+    if (f->np == 0) return VPM_OK;
+    hipLaunchKernelGGL(adj_step_start_kernel, dim3(nb(f->np)), dim3(kBlock), 0, f->stream, f->P, B, f->np);
+    VPM_CHECK_LAUNCH("adj_step_start");
     return VPM_OK;
 }
 

@@ -220,7 +220,7 @@ int vpm_destroy(vpm_field f)
         (void)hipSetDevice(f->device);
         (void)hipStreamSynchronize(f->stream);
         DevBuf* bufs[] = {&f->pos, &f->gs, &f->cut, &f->g64, &f->part, &f->keys, &f->keys2, &f->perm,
-                          &f->iota, &f->sorttmp, &f->bbox, &f->flag, &f->adj, &f->rng};
+                          &f->iota, &f->sorttmp, &f->bbox, &f->flag, &f->adj, &f->rng, &f->bar};
         for (DevBuf* b : bufs) release(*b);
         if (f->P) (void)hipFree(f->P);
         if (f->ev0) (void)hipEventDestroy(f->ev0);
@@ -396,10 +396,12 @@ int vpm_sfs_dynamic_after(vpm_field f, const vpm_config* cfg, double alpha, doub
     return sfs_after_locked(f, cfg, alpha, rlxf, minC, maxC, force_positive, clip);
 }
 
-int vpm_rk3_step(vpm_field f, vpm_step_config* s, double dt, int32_t relax_now)
+}  // extern "C"
+
+// One rk3 step; with a tape, the state after each stage's UJ evaluation is
+// copied into the tape (the per-stage checkpoints of the step adjoint).
+static int rk3_step_locked(vpm_field f, vpm_step_config* s, double dt, int32_t relax_now, vpm_tape tape)
 {
-    VPM_FIELD_GUARD(f);
-    if (!s) return fail(VPM_EINVAL, "null vpm_step_config");
     const vpm_config* cfg = &s->uj;
     if (int rc = check_cfg(cfg)) return rc;
     if (s->viscous == 1 && cfg->kernel != VPM_KERNEL_GAUSSIANERF)   // FLOWVPM.jl:265-267, utils.jl:58-64
@@ -408,6 +410,13 @@ int vpm_rk3_step(vpm_field f, vpm_step_config* s, double dt, int32_t relax_now)
     const bool sfs = s->sfs != 0;
     int rc;
     bool reset_needed = false;
+    const size_t rec = sizeof(double) * NF * (size_t)f->np;
+    if (tape) {
+        if (sfs || s->viscous || (relax_now && s->relax))
+            return fail(VPM_EINVAL, "taped rk3 step: SFS, viscous diffusion and relaxation are not differentiated");
+        if ((rc = ensure(tape->rec, 3 * rec + 1))) return rc;
+        tape->recorded = false;
+    }
     if ((rc = launch_reset(f, 2))) return rc;                      // :401-414
     const double ab[3][2] = {{0.0, 1.0 / 3.0}, {-5.0 / 9.0, 15.0 / 16.0}, {-153.0 / 128.0, 8.0 / 15.0}};
     for (int st = 0; st < 3; ++st) {
@@ -417,6 +426,11 @@ int vpm_rk3_step(vpm_field f, vpm_step_config* s, double dt, int32_t relax_now)
             if ((rc = sfs_before_locked(f, cfg, s->sfs_alpha, s->sfs_minC, s->sfs_maxC))) return rc;
         const uint32_t fl = VPM_UJ_RESET | (sfs ? (VPM_UJ_RESET_SFS | VPM_UJ_SFS) : 0u);
         if ((rc = uj_direct_locked(f, cfg, fl))) return rc;        // :422
+        if (tape && f->np > 0) {
+            hipError_t e = hipMemcpyAsync((char*)tape->rec.p + st * rec, f->P, rec, hipMemcpyDeviceToDevice,
+                                          f->stream);
+            if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(tape)");
+        }
         if (sfs && first)
             if ((rc = sfs_after_locked(f, cfg, s->sfs_alpha, s->sfs_rlxf, s->sfs_minC, s->sfs_maxC,
                                        s->sfs_force_positive, s->sfs_clip)))
@@ -438,9 +452,96 @@ int vpm_rk3_step(vpm_field f, vpm_step_config* s, double dt, int32_t relax_now)
     }
     if ((rc = launch_update_uprev(f))) return rc;                   // particlefield.jl:443-455
     if ((rc = sync(f))) return rc;
+    if (tape) {
+        tape->np = f->np;
+        tape->s = *s;
+        tape->dt = dt;
+        tape->recorded = true;
+    }
     if (reset_needed)
         return fail(VPM_ERESET, "core-spreading reset required (sigma growth >= beta); RBF reset not provided");
     return VPM_OK;
+}
+
+extern "C" {
+
+int vpm_rk3_step(vpm_field f, vpm_step_config* s, double dt, int32_t relax_now)
+{
+    VPM_FIELD_GUARD(f);
+    if (!s) return fail(VPM_EINVAL, "null vpm_step_config");
+    return rk3_step_locked(f, s, dt, relax_now, nullptr);
+}
+
+int vpm_tape_create(vpm_field f, vpm_tape* out)
+{
+    if (!out) return fail(VPM_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (!f) return fail(VPM_EINVAL, "null vpm_field");
+    vpm_tape t = new (std::nothrow) vpm_tape_s;
+    if (!t) return fail(VPM_ENOMEM, "vpm_tape_create");
+    t->f = f;
+    *out = t;
+    return VPM_OK;
+}
+
+int vpm_tape_destroy(vpm_tape t)
+{
+    if (!t) return VPM_OK;
+    if (t->rec.p) {
+        std::lock_guard<std::mutex> lock(t->f->mu);
+        if (int rc = set_device(t->f)) return rc;
+        release(t->rec);
+    }
+    delete t;
+    return VPM_OK;
+}
+
+int vpm_rk3_step_taped(vpm_field f, vpm_step_config* s, double dt, vpm_tape t)
+{
+    VPM_FIELD_GUARD(f);
+    if (!s || !t) return fail(VPM_EINVAL, "null vpm_step_config or vpm_tape");
+    if (t->f != f) return fail(VPM_EINVAL, "tape belongs to another field");
+    return rk3_step_locked(f, s, dt, 0, t);
+}
+
+// Reverse pass over the three recorded stages: restore the stage state,
+// pull the cotangents back through update_particle_states, then through the
+// stage's U/J evaluation (adjoint kernel), and accumulate.
+int vpm_rk3_step_adjoint(vpm_tape t, double* bar)
+{
+    if (!t) return fail(VPM_EINVAL, "null vpm_tape");
+    vpm_field f = t->f;
+    VPM_FIELD_GUARD(f);
+    if (!t->recorded) return fail(VPM_EINVAL, "tape holds no recorded step");
+    const int64_t np = t->np;
+    if (np == 0) return VPM_OK;
+    if (!bar) return fail(VPM_EINVAL, "null cotangent array");
+    if (np > f->cap) return fail(VPM_ERANGE, "recorded particle count exceeds the field capacity");
+    const size_t rec = sizeof(double) * NF * (size_t)np;
+    int rc;
+    if ((rc = ensure(f->bar, rec))) return rc;
+    double* B = (double*)f->bar.p;
+    hipError_t e = hipMemcpyAsync(B, bar, rec, hipMemcpyHostToDevice, f->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(bar)");
+    f->np = np;
+    AdjBufs ab;
+    if ((rc = adj_buffers(f, ab))) return rc;
+    const vpm_step_config& s = t->s;
+    const double coef[3][2] = {{0.0, 1.0 / 3.0}, {-5.0 / 9.0, 15.0 / 16.0}, {-153.0 / 128.0, 8.0 / 15.0}};
+    for (int st = 2; st >= 0; --st) {
+        e = hipMemcpyAsync(f->P, (const char*)t->rec.p + st * rec, rec, hipMemcpyDeviceToDevice, f->stream);
+        if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(restore stage)");
+        f->perm_valid = false;
+        if ((rc = launch_update_states_adjoint(f, B, coef[st][0], coef[st][1], t->dt, s.f, s.g, s.uj.transposed,
+                                               ab.U, ab.J)))
+            return rc;
+        if ((rc = launch_uj_adjoint_dev(f, s.uj.kernel, s.uj.precision, ab))) return rc;
+        if ((rc = launch_adj_add_bar(f, B, ab.X, ab.G, ab.S))) return rc;
+    }
+    if ((rc = launch_adj_step_start(f, B))) return rc;
+    e = hipMemcpyAsync(bar, B, rec, hipMemcpyDeviceToHost, f->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(bar out)");
+    return sync(f);
 }
 
 int vpm_uj_adjoint(vpm_field f, const vpm_config* cfg, const double* Ubar, const double* Jbar, double* Xbar,
@@ -449,7 +550,7 @@ int vpm_uj_adjoint(vpm_field f, const vpm_config* cfg, const double* Ubar, const
     VPM_FIELD_GUARD(f);
     if (int rc = check_cfg(cfg)) return rc;
     if (f->np > 0 && (!Ubar || !Jbar || !Xbar || !Gbar || !Sbar)) return fail(VPM_EINVAL, "null adjoint array");
-    if (int rc = launch_uj_adjoint(f, cfg->kernel, Ubar, Jbar, Xbar, Gbar, Sbar)) return rc;
+    if (int rc = launch_uj_adjoint(f, cfg->kernel, cfg->precision, Ubar, Jbar, Xbar, Gbar, Sbar)) return rc;
     return sync(f);
 }
 

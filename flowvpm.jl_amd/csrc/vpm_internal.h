@@ -61,9 +61,21 @@ struct vpm_field_s {
     vpm::DevBuf flag;                // int device error flag
     vpm::DevBuf adj;                 // adjoint / E_str staging scratch
     vpm::DevBuf rng;                 // leaf-pair range lists
+    vpm::DevBuf bar;                 // cotangent records of the step adjoint
     bool profiling = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     vpm::Timing timing[4];
+};
+
+// Stage records of one rk3 step (state after each stage's UJ evaluation),
+// the input of vpm_rk3_step_adjoint.
+struct vpm_tape_s {
+    vpm_field f = nullptr;
+    int64_t np = 0;
+    vpm::DevBuf rec;                 // double [3][np][46]
+    vpm_step_config s{};
+    double dt = 0.0;
+    bool recorded = false;
 };
 
 namespace vpm {
@@ -101,8 +113,19 @@ int launch_sfs_coefficient(vpm_field f, int transposed, double alpha, double rlx
 int launch_update_uprev(vpm_field f);
 int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb, const int64_t* te,
                       const int64_t* sb, const int64_t* se, int64_t npairs);
-int launch_uj_adjoint(vpm_field f, int kernel, const double* Ubar, const double* Jbar,
+int launch_uj_adjoint(vpm_field f, int kernel, int precision, const double* Ubar, const double* Jbar,
                       double* Xbar, double* Gbar, double* Sbar);
+// device-side U/J adjoint staging (in f->adj) and launch
+struct AdjBufs {
+    double *U, *J, *X, *G, *S;
+    float4* cot;
+};
+int adj_buffers(vpm_field f, AdjBufs& b);
+int launch_uj_adjoint_dev(vpm_field f, int kernel, int precision, const AdjBufs& b);
+int launch_update_states_adjoint(vpm_field f, double* B, double a, double b, double dt, double fr, double gr,
+                                 int transposed, double* Ub, double* Jb);
+int launch_adj_add_bar(vpm_field f, double* B, const double* dX, const double* dG, const double* dS);
+int launch_adj_step_start(vpm_field f, double* B);
 
 double zeta0_of(int kernel);
 

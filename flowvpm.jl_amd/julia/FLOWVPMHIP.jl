@@ -141,23 +141,24 @@ function rungekutta3_hip(pfield::vpm.ParticleField, dt::Real; relax::Bool=false,
 end
 
 """
-    uj_hip(pfield) -> (U::Matrix{Float64}(3,np), J::Matrix{Float64}(9,np))
+    uj_hip(pfield; precision=0) -> (U::Matrix{Float64}(3,np), J::Matrix{Float64}(9,np))
 
 Pure function of (X, Gamma, sigma) for AD; its rrule calls the hand-written
-adjoint kernel instead of differentiating through the pair loop.
+adjoint kernel instead of differentiating through the pair loop (precision 0:
+fused FP32 pair kernel with FP64 partial sums, 1: FP64 pairs).
 """
-function uj_hip(pfield::vpm.ParticleField)
-    UJ_hip(pfield)
+function uj_hip(pfield::vpm.ParticleField; precision=0)
+    UJ_hip(pfield; precision)
     return pfield.particles[10:12, 1:pfield.np], pfield.particles[16:24, 1:pfield.np]
 end
 
-function ChainRulesCore.rrule(::typeof(uj_hip), pfield::vpm.ParticleField)
-    U, J = uj_hip(pfield)
+function ChainRulesCore.rrule(::typeof(uj_hip), pfield::vpm.ParticleField; precision=0)
+    U, J = uj_hip(pfield; precision)
     function uj_hip_pullback((Ubar, Jbar))
         n = pfield.np
         Ub = Matrix{Float64}(ChainRulesCore.unthunk(Ubar)); Jb = Matrix{Float64}(ChainRulesCore.unthunk(Jbar))
         Xb = zeros(3, n); Gb = zeros(3, n); Sb = zeros(n)
-        cfg = Ref(config(pfield; precision=1))
+        cfg = Ref(config(pfield; precision))
         check(ccall((:vpm_uj_adjoint, libvpm), Cint,
                     (Ptr{Cvoid}, Ref{VpmConfig}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
                     device_field(pfield), cfg, Ub, Jb, Xb, Gb, Sb))

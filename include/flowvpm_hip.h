@@ -152,9 +152,32 @@ int vpm_rk3_step(vpm_field f, vpm_step_config* s, double dt, int32_t relax_now);
 /* For L = sum_i Ubar_i . U_i + sum_i sum_k Jbar_i[k] J_i[k] with U, J from one
  * all-pairs evaluation of the field (fmm.jl:102-168), writes dL/dX (3 np),
  * dL/dGamma (3 np) and dL/dsigma (np), particle-major.  Host arrays;
- * Ubar is 3 np, Jbar 9 np in the J storage order. */
+ * Ubar is 3 np, Jbar 9 np in the J storage order.  cfg->precision selects
+ * the fused FP32 pair kernel with FP64 partial sums (VPM_PREC_F32C) or FP64
+ * pair arithmetic (VPM_PREC_F64). */
 int vpm_uj_adjoint(vpm_field f, const vpm_config* cfg, const double* Ubar, const double* Jbar,
                    double* Xbar, double* Gbar, double* Sbar);
+
+/* ---- adjoint of the time step (replaces the ReverseDiff pass through
+ *      rungekutta3, timeintegration.jl:388-461, with the pullbacks of
+ *      rrules.jl:179-330 (pairs) and rrules.jl:344-555 (update)) ----------- */
+/* A tape holds the three stage records of one rk3 step of a field: the
+ * particle state after each stage's UJ evaluation (46 doubles per particle
+ * and stage, device memory), i.e. per-stage checkpoints of X, Gamma, sigma
+ * with their U and J. */
+typedef struct vpm_tape_s* vpm_tape;
+int vpm_tape_create(vpm_field f, vpm_tape* out);
+int vpm_tape_destroy(vpm_tape t);
+/* vpm_rk3_step (ReformulatedVPM, no SFS, inviscid, no relaxation) that also
+ * records its stage states into t (overwriting what t held). */
+int vpm_rk3_step_taped(vpm_field f, vpm_step_config* s, double dt, vpm_tape t);
+/* Pullback of the step recorded in t.  bar: np x 46 cotangent records
+ * (host, particle record layout).  On entry the rows X, Gamma, sigma and M of
+ * the state after the step; on exit the cotangents of the state before it
+ * (X, Gamma, sigma; M rows zero).  The U/J adjoint runs at the precision of
+ * the recorded step.  Leaves the field holding the state before the step
+ * (with the U and J of its first stage). */
+int vpm_rk3_step_adjoint(vpm_tape t, double* bar);
 
 /* ---- measurement -------------------------------------------------------- */
 /* When enabled, the library brackets each all-pairs kernel launch with HIP

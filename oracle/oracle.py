@@ -69,6 +69,11 @@ def lib():
         L.vpmo_update_particle_states.argtypes = [_dp, C.c_int64, C.c_double, C.c_double, C.c_double,
                                                   _dp, C.c_double, C.c_double, C.c_double, C.c_int]
         L.vpmo_relax.argtypes = [_dp, C.c_int64, C.c_double, C.c_int]
+        L.vpmo_update_particle_states_adjoint.argtypes = [_dp, _dp, C.c_int64, C.c_double, C.c_double,
+                                                          C.c_double, C.c_double, C.c_double, C.c_int]
+        L.vpmo_uj_adjoint_scale.argtypes = [_dp, C.c_int64, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int]
+        L.vpmo_rk3_step_adjoint.restype = C.c_int
+        L.vpmo_rk3_step_adjoint.argtypes = [_dp, C.c_int64, C.c_void_p, C.c_double, _dp]
         L.vpmo_core_spreading_rk3.restype = C.c_int
         L.vpmo_core_spreading_rk3.argtypes = [_dp, C.c_int64, C.c_double, C.c_double, C.c_double,
                                               C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double]
@@ -214,7 +219,33 @@ def uj_adjoint(P, kernel, Ubar, Jbar, nthreads=None):
     return Xb.reshape(n, 3), Gb.reshape(n, 3), Sb
 
 
+def uj_adjoint_scale(P, kernel, Ubar, Jbar, nthreads=None):
+    """Per-output sums of |pair contribution| of uj_adjoint (the rounding
+    error scale of the adjoint sums)."""
+    n = P.shape[0]
+    Xm, Gm, Sm = np.zeros(3 * n), np.zeros(3 * n), np.zeros(n)
+    lib().vpmo_uj_adjoint_scale(_check(P), n, _kid(kernel),
+                                np.ascontiguousarray(Ubar, dtype=np.float64).reshape(-1),
+                                np.ascontiguousarray(Jbar, dtype=np.float64).reshape(-1),
+                                Xm, Gm, Sm, nthreads or nthreads_default())
+    return Xm.reshape(n, 3), Gm.reshape(n, 3), Sm
+
+
 def loss_local(P, kernel, Ubar, Jbar, k):
     return lib().vpmo_loss_local(_check(P), P.shape[0], _kid(kernel),
                                  np.ascontiguousarray(Ubar, dtype=np.float64).reshape(-1),
                                  np.ascontiguousarray(Jbar, dtype=np.float64).reshape(-1), int(k))
+
+
+def update_particle_states_adjoint(P, B, a, b, dt, f, g, transposed=True):
+    """In place on B (cotangent records, shape (n, 46)); P is the state before
+    the update."""
+    lib().vpmo_update_particle_states_adjoint(_check(P), _check(B), P.shape[0], a, b, dt, f, g,
+                                              int(transposed))
+
+
+def rk3_step_adjoint(P0, settings, dt, B):
+    """In place on B: cotangents after one rk3 step -> before it."""
+    rc = lib().vpmo_rk3_step_adjoint(_check(P0), P0.shape[0], C.addressof(settings), dt, _check(B))
+    if rc != 0:
+        raise ValueError("rk3_step_adjoint: unsupported settings (SFS or viscous) rc=%d" % rc)

@@ -23,6 +23,9 @@
  *                      inputs, plus the single-ring translation speed test
  *                      of test/runtests_singlevortexring.jl:22-23,143
  *   vpmo_uj_adjoint    central finite differences of vpmo_loss_local
+ *   vpmo_update_particle_states_adjoint / vpmo_rk3_step_adjoint
+ *                      central finite differences of vpmo_update_particle_states
+ *                      and of a loss through whole vpmo_rk3_step calls
  */
 #include "vpm_oracle.h"
 #include <math.h>
@@ -660,6 +663,42 @@ void vpmo_uj_adjoint(const double* P, int64_t np, int kernel,
     }
 }
 
+/* Summation scale of vpmo_uj_adjoint: the same sums with every pair
+ * contribution replaced by its absolute value (componentwise).  An FP32 pair
+ * evaluation is accurate to a few units of FP32 rounding per term, so its
+ * error in an output is bounded by a small multiple of this scale, however
+ * much the terms cancel. */
+void vpmo_uj_adjoint_scale(const double* P, int64_t np, int kernel,
+                           const double* Ubar, const double* Jbar,
+                           double* Xmag, double* Gmag, double* Smag, int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
+    for (int64_t i = 0; i < np; i++) {
+        const double* ti = P + i * VPMO_NFIELDS;
+        double ax[3] = {0, 0, 0};
+        for (int64_t j = 0; j < np; j++) {
+            double gd[3], gG[3], gs;
+            pair_adjoint(kernel, ti, P + j * VPMO_NFIELDS, Ubar + 3 * i, Jbar + 9 * i, gd, gG, &gs);
+            for (int a = 0; a < 3; a++) ax[a] += fabs(gd[a]);
+        }
+        for (int a = 0; a < 3; a++) Xmag[3 * i + a] = ax[a];
+    }
+#pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
+    for (int64_t j = 0; j < np; j++) {
+        const double* sj = P + j * VPMO_NFIELDS;
+        double ax[3] = {0, 0, 0}, aG[3] = {0, 0, 0}, as = 0;
+        for (int64_t i = 0; i < np; i++) {
+            double gd[3], gG[3], gs;
+            pair_adjoint(kernel, P + i * VPMO_NFIELDS, sj, Ubar + 3 * i, Jbar + 9 * i, gd, gG, &gs);
+            for (int a = 0; a < 3; a++) { ax[a] += fabs(gd[a]); aG[a] += fabs(gG[a]); }
+            as += fabs(gs);
+        }
+        for (int a = 0; a < 3; a++) { Xmag[3 * j + a] += ax[a]; Gmag[3 * j + a] = aG[a]; }
+        Smag[j] = as;
+    }
+}
+
 static long double pair_loss(int kernel, const double* ti, const double* sj,
                              const double* ub, const double* Jb)
 {
@@ -681,4 +720,134 @@ double vpmo_loss_local(const double* P, int64_t np, int kernel,
     for (int64_t i = 0; i < np; i++)
         if (i != k) L += pair_loss(kernel, P + i * VPMO_NFIELDS, pk, Ubar + 3 * i, Jbar + 9 * i);
     return (double)L;
+}
+
+/* ---- adjoint of the time step ----
+ * Pullback of update_particle_states for ReformulatedVPM without SFS
+ * (timeintegration.jl:467-528 with C S = 0; the pullback the reference writes
+ * as state_pullback in rrules.jl:436-555, here derived from the update
+ * itself).  With m1 = Mbar1' + b Xbar', m23 = Mbar23' + b sigmabar',
+ * m2 = Mbar2' + b Gammabar', MM = stretching(J, Gamma), G2 = |Gamma|^2,
+ * MM4 = A (MM . Gamma)/G2, A = (f + g)/(1 + 3 f):
+ *   Xbar = Xbar',  Mbar1 = a m1,  Ubar = dt m1,
+ *   Mbar23 = a m23,  sigmabar = sigmabar' - dt MM4 m23,
+ *   Mbar2 = a m2,  MMbar = dt m2 + MM4bar A Gamma/G2,
+ *   MM4bar = -dt sigma m23 - 3 dt (Gamma . m2),
+ *   Gammabar = Gammabar' - 3 dt MM4 m2 + MM4bar (A MM - 2 MM4 Gamma)/G2
+ *              + dMM/dGamma^T MMbar,   Jbar = dMM/dJ^T MMbar.
+ * (rrules.jl:521 keeps only the eta = zeta term of d(MM4 Gamma_zeta)/dJ;
+ * the full sum over zeta is used here.)
+ * B[i] is the cotangent record of particle i (46 rows, same layout as P):
+ * on entry the rows X, Gamma, sigma, M of the updated state; on exit those
+ * rows for the state before the update plus Ubar in the U rows and Jbar in
+ * the J rows.  P holds the state before the update (with its U and J). */
+void vpmo_update_particle_states_adjoint(const double* P, double* B, int64_t np, double a, double b,
+                                         double dt, double f, double g, int transposed)
+{
+    const double A = (f + g) / (1 + 3 * f);
+    for (int64_t i = 0; i < np; i++) {
+        const double* p = P + i * VPMO_NFIELDS;
+        double* q = B + i * VPMO_NFIELDS;
+        for (int k = 0; k < 3; k++) q[RU + k] = 0.0;
+        for (int k = 0; k < 9; k++) q[RJ + k] = 0.0;
+        if (is_static(p)) continue;
+        const double* G = p + RG;
+        const double* J = p + RJ;
+        const double sgm = p[RSIG];
+        double MM[3];
+        stretching(J, G, transposed, MM);
+        const double G2 = G[0] * G[0] + G[1] * G[1] + G[2] * G[2];
+        const double MM4 = G2 > 0 ? A * (MM[0] * G[0] + MM[1] * G[1] + MM[2] * G[2]) / G2 : 0.0;
+        double m1[3], m2[3];
+        for (int k = 0; k < 3; k++) {
+            m1[k] = q[RM + k] + b * q[RX + k];
+            m2[k] = q[RM + 3 + k] + b * q[RG + k];
+        }
+        const double m23 = q[RM + 7] + b * q[RSIG];
+        double MM4b = -dt * sgm * m23 - 3 * dt * (G[0] * m2[0] + G[1] * m2[1] + G[2] * m2[2]);
+        double MMb[3], Gb[3];
+        for (int k = 0; k < 3; k++) {
+            q[RM + k] = a * m1[k];
+            q[RU + k] = dt * m1[k];
+            q[RM + 3 + k] = a * m2[k];
+            MMb[k] = dt * m2[k];
+            Gb[k] = q[RG + k] - 3 * dt * MM4 * m2[k];
+        }
+        q[RM + 7] = a * m23;
+        q[RSIG] = q[RSIG] - dt * MM4 * m23;
+        if (G2 > 0) {
+            for (int k = 0; k < 3; k++) {
+                MMb[k] += MM4b * A * G[k] / G2;
+                Gb[k] += MM4b * (A * MM[k] - 2 * MM4 * G[k]) / G2;
+            }
+        }
+        /* MM[e] = sum_x J(x, e) G[x] (transposed) or J(e, x) G[x]; J(r, c) = J[3 c + r] */
+        for (int e = 0; e < 3; e++)
+            for (int x = 0; x < 3; x++) {
+                const int idx = transposed ? 3 * e + x : 3 * x + e;
+                q[RJ + idx] += MMb[e] * G[x];
+                Gb[x] += J[idx] * MMb[e];
+            }
+        for (int k = 0; k < 3; k++) q[RG + k] = Gb[k];
+    }
+}
+
+/* Pullback of one vpmo_rk3_step (no SFS, inviscid, no relaxation): B holds
+ * the cotangent records of the state after the step and is replaced by those
+ * of the state before it (rows X, Gamma, sigma; M rows are zero because the
+ * step starts from M = 0).  The forward step is recomputed from P0 with the
+ * three stage states (after each stage's UJ) kept in memory.  Returns -1 for
+ * a setting whose adjoint is not restated. */
+int vpmo_rk3_step_adjoint(const double* P0, int64_t np, const vpmo_settings* s, double dt, double* B)
+{
+    if (s->sfs != 0 || s->viscous != 0) return -1;
+    const size_t rec = (size_t)np * VPMO_NFIELDS;
+    double* tape = (double*)malloc(sizeof(double) * rec * 4);
+    if (!tape) return -2;
+    double* P = tape + 3 * rec;
+    memcpy(P, P0, sizeof(double) * rec);
+    const double zeta0 = vpmo_zeta(s->kernel, 0.0);
+    for (int64_t i = 0; i < np; i++) {
+        double* p = P + i * VPMO_NFIELDS;
+        if (p[RSTATIC] == 0) for (int k = 0; k < 9; k++) p[RM + k] = 0.0;
+    }
+    const double ab[3][2] = { {0.0, 1.0 / 3.0}, {-5.0 / 9.0, 15.0 / 16.0}, {-153.0 / 128.0, 8.0 / 15.0} };
+    for (int st = 0; st < 3; st++) {
+        vpmo_uj_direct(P, np, s->kernel, 1, 0, 0, s->transposed, s->nthreads);
+        memcpy(tape + st * rec, P, sizeof(double) * rec);
+        vpmo_update_particle_states(P, np, ab[st][0], ab[st][1], dt, s->Uinf, s->f, s->g, zeta0, s->transposed);
+    }
+    double* Ub = (double*)malloc(sizeof(double) * (size_t)np * 19);
+    if (!Ub) { free(tape); return -2; }
+    double* Jb = Ub + 3 * np;
+    double* Xb = Jb + 9 * np;
+    double* Gb = Xb + 3 * np;
+    double* Sb = Gb + 3 * np;
+    for (int st = 2; st >= 0; st--) {
+        const double* S = tape + st * rec;
+        vpmo_update_particle_states_adjoint(S, B, np, ab[st][0], ab[st][1], dt, s->f, s->g, s->transposed);
+        for (int64_t i = 0; i < np; i++) {
+            const double* q = B + i * VPMO_NFIELDS;
+            for (int k = 0; k < 3; k++) Ub[3 * i + k] = q[RU + k];
+            for (int k = 0; k < 9; k++) Jb[9 * i + k] = q[RJ + k];
+        }
+        vpmo_uj_adjoint(S, np, s->kernel, Ub, Jb, Xb, Gb, Sb, s->nthreads);
+        for (int64_t i = 0; i < np; i++) {
+            double* q = B + i * VPMO_NFIELDS;
+            for (int k = 0; k < 3; k++) {
+                q[RX + k] += Xb[3 * i + k];
+                q[RG + k] += Gb[3 * i + k];
+            }
+            q[RSIG] += Sb[i];
+        }
+    }
+    for (int64_t i = 0; i < np; i++) {
+        double* q = B + i * VPMO_NFIELDS;
+        for (int k = 0; k < 3; k++) q[RU + k] = 0.0;
+        for (int k = 0; k < 9; k++) q[RJ + k] = 0.0;
+        if (P0[i * VPMO_NFIELDS + RSTATIC] == 0) for (int k = 0; k < 9; k++) q[RM + k] = 0.0;
+    }
+    free(Ub);
+    free(tape);
+    return 0;
 }

@@ -98,10 +98,24 @@ int vpmo_rk3_step(double* P, int64_t np, vpmo_settings* s, double dt, int relax_
 void vpmo_uj_adjoint(const double* P, int64_t np, int kernel,
                      const double* Ubar, const double* Jbar,
                      double* Xbar, double* Gbar, double* Sbar, int nthreads);
+/* The same sums with |contribution| per pair and component: the summation
+ * scale that bounds the rounding error of any evaluation of the adjoint. */
+void vpmo_uj_adjoint_scale(const double* P, int64_t np, int kernel,
+                           const double* Ubar, const double* Jbar,
+                           double* Xmag, double* Gmag, double* Smag, int nthreads);
 /* The part of L that depends on particle k (all pairs with k as target or as
  * source), summed in long double; for finite differences. */
 double vpmo_loss_local(const double* P, int64_t np, int kernel,
                        const double* Ubar, const double* Jbar, int64_t k);
+/* Pullback of update_particle_states (ReformulatedVPM, no SFS;
+ * timeintegration.jl:467-528, rrules.jl:405-555).  B: cotangent records
+ * (46 rows per particle) of the updated state in, of the state before the
+ * update out, plus Ubar/Jbar in the U/J rows.  P: the state before. */
+void vpmo_update_particle_states_adjoint(const double* P, double* B, int64_t np, double a, double b,
+                                         double dt, double f, double g, int transposed);
+/* Pullback of one vpmo_rk3_step (no SFS, inviscid, no relaxation); B as
+ * above for the whole step.  Returns 0, or -1 for an unsupported setting. */
+int vpmo_rk3_step_adjoint(const double* P0, int64_t np, const vpmo_settings* s, double dt, double* B);
 
 #ifdef __cplusplus
 }

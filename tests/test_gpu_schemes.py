@@ -174,7 +174,7 @@ def test_adjoint_vs_oracle(vpm, orc, kernel):
     P = vpm.workloads.random_cloud(1500, seed=21)
     w, W = vpm.workloads.adjoint_weights(1500)
     Xr, Gr, Sr = orc.uj_adjoint(P, kernel, w, W)
-    Xb, Gb, Sb = vpm.uj_adjoint(P, w, W, kernel=kernel)
+    Xb, Gb, Sb = vpm.uj_adjoint(P, w, W, kernel=kernel, precision="f64")
     for name, a, r in (("X", Xb, Xr), ("Gamma", Gb, Gr), ("sigma", Sb[:, None], Sr[:, None])):
         st = vpm.metrics.err_stats(a, r)
         _say("adjoint", kernel, name, st)
@@ -185,7 +185,8 @@ def test_adjoint_golden(vpm):
     g = golden("adjoint.json")
     P = vpm.fields.make_particles(g["X"], g["Gamma"], g["sigma"])
     for kernel in ("gaussianerf", "winckelmans", "gaussian", "singular"):
-        Xb, Gb, Sb = vpm.uj_adjoint(P, np.array(g["Ubar"]), np.array(g["Jbar"]), kernel=kernel)
+        Xb, Gb, Sb = vpm.uj_adjoint(P, np.array(g["Ubar"]), np.array(g["Jbar"]), kernel=kernel,
+                                    precision="f64")
         ref = g[kernel]
         np.testing.assert_allclose(Xb, ref["Xbar"], rtol=1e-9, atol=1e-12 * np.abs(ref["Xbar"]).max())
         np.testing.assert_allclose(Gb, ref["Gbar"], rtol=1e-9, atol=1e-12 * np.abs(ref["Gbar"]).max())

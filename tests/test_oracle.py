@@ -237,3 +237,108 @@ def test_adjoint_matches_fd(orc, vpm):
             fd = (orc.loss_local(Pp, "gaussianerf", w, W, k) - orc.loss_local(Pm, "gaussianerf", w, W, k)) / (2 * h)
             an = Xb[k, col] if col < 3 else (Gb[k, col - 3] if col < 6 else Sb[k])
             assert abs(fd - an) <= 2e-6 * max(abs(an), 1e-3 * np.abs(Xb).max()), (k, col, fd, an)
+
+
+def _random_state(vpm, n, seed):
+    """Particle records with random X, Gamma, sigma, U, J and M (the inputs of
+    update_particle_states)."""
+    rng = np.random.default_rng(seed)
+    P = _cloud(vpm, n=n, seed=seed)
+    P[:, 9:12] = rng.normal(0, 1, (n, 3))       # U
+    P[:, 15:24] = rng.normal(0, 1, (n, 9))      # J
+    P[:, 27:36] = rng.normal(0, 1, (n, 9))      # M
+    return P
+
+
+@pytest.mark.parametrize("transposed", [True, False])
+def test_update_adjoint_matches_fd(orc, vpm, transposed):
+    """Pullback of update_particle_states against central differences of the
+    update itself, for every input row it reads (X, Gamma, sigma, M, U, J)."""
+    n = 6
+    P = _random_state(vpm, n, seed=11)
+    P[5, 42] = 1.0                              # one static particle
+    rng = np.random.default_rng(12)
+    Wout = np.zeros_like(P)
+    for rows in ((0, 3), (3, 6), (6, 7), (27, 30), (30, 33), (34, 35)):
+        Wout[:, rows[0]:rows[1]] = rng.normal(0, 1, (n, rows[1] - rows[0]))
+    a, b, dt, f, g = -5.0 / 9.0, 15.0 / 16.0, 0.01, 0.1, 0.2
+    Uinf = np.array([0.3, -0.1, 0.2])
+    zeta0 = orc.zeta("gaussianerf", 0.0) if hasattr(orc, "zeta") else 1.0
+
+    def loss(Q):
+        Q = Q.copy()
+        orc.update_particle_states(Q, a, b, dt, Uinf, f, g, zeta0, transposed)
+        return float(np.sum(Wout * Q))
+
+    B = Wout.copy()
+    orc.update_particle_states_adjoint(P, B, a, b, dt, f, g, transposed)
+    h = 1e-6
+    rows = list(range(0, 7)) + list(range(9, 12)) + list(range(15, 24)) + [27, 28, 29, 30, 31, 32, 34]
+    for i in range(n):
+        for r in rows:
+            Pp, Pm = P.copy(), P.copy()
+            Pp[i, r] += h
+            Pm[i, r] -= h
+            fd = (loss(Pp) - loss(Pm)) / (2 * h)
+            an = B[i, r]
+            assert abs(fd - an) <= 1e-7 * max(1.0, abs(an)), (i, r, fd, an)
+
+
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans"])
+def test_rk3_step_adjoint_matches_fd(orc, vpm, kernel):
+    """Pullback of one whole rk3 step (three UJ evaluations and updates)
+    against central differences of a linear functional of the state after
+    the step, on X, Gamma and sigma of the state before it."""
+    n = 30
+    P = _cloud(vpm, n=n, seed=21, sig=(0.15, 0.3))
+    P[:, 3:6] *= 0.05
+    rng = np.random.default_rng(22)
+    Wout = np.zeros_like(P)
+    Wout[:, 0:7] = rng.normal(0, 1, (n, 7))
+    s = orc.make_settings(kernel=kernel, f=0.0, g=0.2, relax=0, Uinf=(0.1, 0.0, -0.05))
+    dt = 0.05
+
+    def loss(Q):
+        Q = Q.copy()
+        orc.rk3_step(Q, s, dt, relax_now=False)
+        return float(np.sum(Wout[:, 0:7] * Q[:, 0:7]))
+
+    B = Wout.copy()
+    orc.rk3_step_adjoint(P, s, dt, B)
+    assert np.all(B[:, 9:36] == 0.0)
+    h = 1e-6
+    scale = np.abs(B[:, 0:7]).max()
+    for i in rng.choice(n, 8, replace=False):
+        for r in range(7):
+            Pp, Pm = P.copy(), P.copy()
+            Pp[i, r] += h
+            Pm[i, r] -= h
+            fd = (loss(Pp) - loss(Pm)) / (2 * h)
+            an = B[i, r]
+            assert abs(fd - an) <= 1e-6 * max(abs(an), 1e-3 * scale), (i, r, fd, an)
+
+
+def test_rk3_step_adjoint_rejects_sfs(orc, vpm):
+    P = _cloud(vpm, n=5)
+    s = orc.make_settings(sfs=1)
+    with pytest.raises(ValueError):
+        orc.rk3_step_adjoint(P, s, 0.01, np.zeros_like(P))
+
+
+def test_adjoint_scale_bounds_adjoint(orc, vpm):
+    """The summation scale dominates the adjoint componentwise and equals it
+    when every contribution has one sign (a two-particle field)."""
+    P = _cloud(vpm, n=50, seed=8)
+    rng = np.random.default_rng(8)
+    w, W = rng.normal(size=(50, 3)), rng.normal(size=(50, 9))
+    for kernel in KERNELS:
+        got = orc.uj_adjoint(P, kernel, w, W)
+        mag = orc.uj_adjoint_scale(P, kernel, w, W)
+        for a, m in zip(got, mag):
+            assert np.all(np.abs(a) <= m * (1 + 1e-12) + 1e-300)
+    P2 = P[:2].copy()
+    w2, W2 = np.zeros((2, 3)), np.zeros((2, 9))
+    w2[0] = (1.0, 0.0, 0.0)
+    got = orc.uj_adjoint(P2, "gaussianerf", w2, W2)
+    mag = orc.uj_adjoint_scale(P2, "gaussianerf", w2, W2)
+    np.testing.assert_allclose(np.abs(got[1]), mag[1], rtol=1e-14)
