@@ -79,6 +79,60 @@ def copy_calibration(mib):
     hip.hipFree(b)
 
 
+def rollout_adjoint(vpm, P, kernel, nsteps=3, dt=0.005):
+    """Forward + adjoint through nsteps rk3 steps on the jet (rVPM, no SFS,
+    inviscid, no relaxation), per-stage checkpoints on the device,
+    L = sum |U_i|^2 at the final state."""
+    api = vpm.api
+    n = P.shape[0]
+    pf = api.ParticleField(n, kernel=kernel, formulation=api.rVPM, relaxation=api.norelaxation,
+                           precision="f32c")
+    api.add_particles(pf, P)
+    tm = {}
+    log("bench: rollout forward+adjoint, %d rk3 steps" % nsteps)
+    L, bar = api.kinetic_energy_rollout_gradient(pf, dt, nsteps, timings=tm)
+    pf.close()
+    out = {"N": n, "steps": nsteps, "dt": dt, "formulation": "rVPM (f=0, g=1/5), no SFS, inviscid",
+           "loss": "sum |U_i|^2 after the last step", "L": L,
+           "forward_ms": tm["forward_ms"], "adjoint_ms": tm["adjoint_ms"],
+           "ratio": tm["adjoint_ms"] / tm["forward_ms"],
+           "uj_evaluations": 3 * nsteps + 1, "checkpoint_bytes": tm["tape_bytes"],
+           "grad_norms": {"X": float(np.linalg.norm(bar[:, 0:3])), "Gamma": float(np.linalg.norm(bar[:, 3:6])),
+                          "sigma": float(np.linalg.norm(bar[:, 6]))}}
+    log("bench: rollout forward %.0f ms, adjoint %.0f ms" % (tm["forward_ms"], tm["adjoint_ms"]))
+    return out
+
+
+def cfg2_rollout(vpm, nsteps=10, dt=0.01):
+    """Two leapfrogging rings, N = 262,144: nsteps rk3 steps of rVPM with the
+    dynamic SFS model (SFS_Cd_twolevel_nobackscatter), core spreading
+    (nu = 1e-4) and Pedrizzetti relaxation every step; FP32c pair passes
+    against the FP64 rollout."""
+    api = vpm.api
+    P = vpm.workloads.leapfrog_cfg2()
+    n = P.shape[0]
+    sg0 = float(P[0, 6])
+    res = {"N": n, "steps": nsteps, "dt": dt}
+    finals = {}
+    for prec in ("f32c", "f64"):
+        pf = api.ParticleField(n, kernel="gaussianerf", formulation=api.rVPM, relaxation=api.pedrizzetti,
+                               SFS=api.SFS_Cd_twolevel_nobackscatter, viscous=api.CoreSpreading(1e-4, sg0),
+                               precision=prec)
+        api.add_particles(pf, P)
+        log("bench: cfg2 rollout %s" % prec)
+        t0 = time.perf_counter()
+        api.run_vpm(pf, dt, nsteps)
+        res["%s_s" % prec] = time.perf_counter() - t0
+        finals[prec] = pf.particles[:n].copy()
+        pf.close()
+    A, R = finals["f32c"], finals["f64"]
+    res["error_f32c_vs_f64"] = {name: vpm.metrics.err_stats(A[:, sl], R[:, sl])
+                                for name, sl in (("X", slice(0, 3)), ("Gamma", slice(3, 6)),
+                                                 ("sigma", slice(6, 7)))}
+    log("bench: cfg2 errors %s" % res["error_f32c_vs_f64"])
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +143,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f64", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-rollout", action="store_true",
+                    help="skip forward+adjoint through 3 RK3 steps on the jet")
+    ap.add_argument("--no-cfg2", action="store_true",
+                    help="skip the 10-step leapfrog-rings rollout, FP32c vs FP64")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -155,11 +213,13 @@ def main():
         f.get_particles(out64, rows=(9, 24))
         eu = vpm.metrics.err_stats(out32[:, 9:12], out64[:, 9:12])
         ej = vpm.metrics.err_stats(out32[:, 15:24], out64[:, 15:24])
+        es = vpm.metrics.err_stats(vpm.metrics.stretching(out32[:, 15:24], P[:, 3:6]),
+                                   vpm.metrics.stretching(out64[:, 15:24], P[:, 3:6]))
         res["f64"] = {"value": pairs / w64, "unit": "pair-interactions/s", "ms_per_step": 1e3 * w64,
                       "kernel_ms": k64, "speedup_f32c_over_f64": w64 / (wall / args.steps),
                       "frac_of_fp64_peak": pairs * FLOP_PER_PAIR[kernel] / (k64 / 1e3) / PEAK_FP64}
-        res["accuracy_f32c_vs_f64_gpu"] = {"U": eu, "J": ej}
-        log("bench: FP64 %.1f ms; err U %s J %s" % (1e3 * w64, eu, ej))
+        res["accuracy_f32c_vs_f64_gpu"] = {"U": eu, "J": ej, "dGamma_dt": es}
+        log("bench: FP64 %.1f ms; err U %s J %s dGamma/dt %s" % (1e3 * w64, eu, ej, es))
     f.profile(False)
 
     if not args.no_adjoint:
@@ -184,13 +244,21 @@ def main():
             (ka, 1e3 * ta, 1e3 * k_avg_s))
         f.profile(False)
 
+    if not args.no_rollout:
+        res["rollout_adjoint"] = rollout_adjoint(vpm, P, kernel)
+
+    if not args.no_cfg2:
+        res["cfg2_rollout"] = cfg2_rollout(vpm)
+
     if not args.no_cpu:
         log("bench: CPU baseline")
         cb, ref_rows, nt = cpu_baseline(P, kernel)
         res["cpu_baseline"] = cb
         eu = vpm.metrics.err_stats(out32[:nt, 9:12], ref_rows[:, 9:12])
         ej = vpm.metrics.err_stats(out32[:nt, 15:24], ref_rows[:, 15:24])
-        res["accuracy_f32c_vs_cpu_oracle"] = {"targets": nt, "U": eu, "J": ej}
+        es = vpm.metrics.err_stats(vpm.metrics.stretching(out32[:nt, 15:24], P[:nt, 3:6]),
+                                   vpm.metrics.stretching(ref_rows[:, 15:24], P[:nt, 3:6]))
+        res["accuracy_f32c_vs_cpu_oracle"] = {"targets": nt, "U": eu, "J": ej, "dGamma_dt": es}
     f.close()
     if args.copy_mib:
         copy_calibration(args.copy_mib)

@@ -250,6 +250,12 @@ class ParticleField:
             self._field = Field(self.maxparticles, self.device)
         return self._field
 
+    def close(self):
+        """Release the device copy (and its scratch buffers) now."""
+        if self._field is not None:
+            self._field.close()
+            self._field = None
+
 
 def isLES(pfield):
     return isinstance(pfield.SFS, DynamicSFS)
@@ -519,24 +525,29 @@ def rk3_adjoint(tape, bar):
     return bar
 
 
-def kinetic_energy_rollout_gradient(pfield, dt, nsteps):
+def kinetic_energy_rollout_gradient(pfield, dt, nsteps, timings=None):
     """Gradient of L = sum_i |U_i|^2 after nsteps rk3 steps with respect to
     X, Gamma and sigma of the initial state (pfield.particles): nsteps taped
     steps (per-stage checkpoints on the device), one U/J evaluation of the
     final state, then the adjoint back through the final U/J evaluation and
     every recorded stage.  Returns (L, bar) with bar the (n, 46) cotangent
-    records (rows X, Gamma, sigma).  pfield is left at the initial state."""
+    records (rows X, Gamma, sigma).  pfield is left at the initial state.
+    With a dict `timings`, the wall times of the forward and the adjoint
+    phases (ms, every call synchronizes) and the tape size are stored in it."""
+    import time
     n = pfield.np
     f = pfield.device_field()
     f.set_particles(pfield.particles[:n])
     tapes = []
     try:
+        t0 = time.perf_counter()
         for _ in range(nsteps):
             tapes.append(Tape(f))
             rungekutta3_taped(pfield, dt, tapes[-1])
         f.uj_direct(pfield.kernel, pfield.precision, transposed=pfield.transposed)
         out = np.zeros((n, NFIELDS))
         f.get_particles(out, rows=(9, 12))
+        t1 = time.perf_counter()
         U = out[:, 9:12]
         L = float(np.sum(U * U))
         Xb, Gb, Sb = f.uj_adjoint(2.0 * U, np.zeros((n, 9)), pfield.kernel, pfield.precision)
@@ -544,6 +555,10 @@ def kinetic_energy_rollout_gradient(pfield, dt, nsteps):
         bar[:, 0:3], bar[:, 3:6], bar[:, 6] = Xb, Gb, Sb
         for t in reversed(tapes):
             bar = rk3_adjoint(t, bar)
+        t2 = time.perf_counter()
+        if timings is not None:
+            timings.update(forward_ms=1e3 * (t1 - t0), adjoint_ms=1e3 * (t2 - t1),
+                           tape_bytes=3 * nsteps * n * NFIELDS * 8)
     finally:
         for t in tapes:
             t.close()

@@ -180,6 +180,20 @@ __device__ __forceinline__ void adj_pair(const AdjTarget& T, const float4 a, con
         Kc = -5.f * H * ri2;
     } else {
         adj_coef<K>(r2, b.w, F, H, Kc, Fs, Hs);
+        if constexpr (Far<K>::cheap) {
+            // pairs beyond the source's far radius take the far form in near
+            // blocks too, so a pair's value does not depend on its block
+            // (padding rows carry a zero radius: r2 == 0 keeps the zeros)
+            if (r2 >= a.w && r2 > 0.f) {
+                const float ri = rsq(r2);
+                const float ri2 = ri * ri;
+                F = ri2 * ri;
+                H = -3.f * F * ri2;
+                Kc = -5.f * H * ri2;
+                Fs = 0.f;
+                Hs = 0.f;
+            }
+        }
     }
     // v = d x Gamma'
     const float vx = fmaf(dy, b.z, -dz * b.y);

@@ -24,3 +24,14 @@ def err_stats(F, R, floor=1e-6):
     if e.size == 0:
         return {"max": 0.0, "rms": 0.0}
     return {"max": float(e.max()), "rms": float(np.sqrt(np.mean(e * e)))}
+
+
+def stretching(J, G, transposed=True):
+    """(Gamma . grad^T) U per particle from the J rows (flat column-major,
+    J(r, c) = J[3 c + r]): transposed J^T Gamma, classic J Gamma
+    (src/FLOWVPM_timeintegration.jl:489-499).  Host-side, for error reports."""
+    J = np.asarray(J, dtype=np.float64).reshape(-1, 3, 3)   # J[i][c][r]
+    G = np.asarray(G, dtype=np.float64).reshape(-1, 3)
+    if transposed:
+        return np.einsum("icr,ir->ic", J, G)
+    return np.einsum("icr,ic->ir", J, G)

@@ -126,3 +126,17 @@ def test_gen_kernel_fits_reproducible(vpm):
                          capture_output=True, text=True).stdout
     with open(os.path.join(csrc, "kernel_fits.h")) as fh:
         assert fh.read().strip() == out.strip()
+
+
+def test_stretching_helper_matches_oracle_update(vpm, orc):
+    """metrics.stretching against the stretching term inside the oracle's
+    update_particle_states (f = g = 0 so MM4 = 0, a = 0, dt = 1: the second
+    M column becomes exactly the stretching term)."""
+    rng = np.random.default_rng(3)
+    P = vpm.fields.make_particles(rng.normal(size=(7, 3)), rng.normal(size=(7, 3)), np.full(7, 0.1))
+    P[:, 15:24] = rng.normal(size=(7, 9))
+    for transposed in (True, False):
+        Q = P.copy()
+        orc.update_particle_states(Q, 0.0, 1.0, 1.0, np.zeros(3), 0.0, 0.0, 1.0, transposed)
+        np.testing.assert_allclose(vpm.metrics.stretching(P[:, 15:24], P[:, 3:6], transposed), Q[:, 30:33],
+                                   rtol=1e-14, atol=1e-14)

@@ -30,6 +30,11 @@ pytestmark = pytest.mark.gpu
 KERNELS = ["gaussianerf", "winckelmans", "gaussian", "singular"]
 ADJ_SCALED_TOL = 1e-5
 ADJ_NORM_TOL = 1e-5
+# dL/dsigma terms are proportional to E = e^{-q^2/2} (Gaussian-erf) or
+# e^{-q^3} (Gaussian), whose relative condition in q is q^2 <= 49 and
+# 3 q^3 <= 81 inside the far radius: FP32-rounded q (and 1/sigma) give each
+# term a relative error of up to ~1e2 roundings, hence the looser bound.
+ADJ_SCALED_TOL_SIGMA = 1e-4
 # A particle with no neighbour inside the far-field radius has dL/dsigma made
 # only of e^{-q^2/2}-small terms (q >= 7, Gaussian: e^{-q^3}, q >= 3), which
 # the FP32 path drops; its scale is floored at this fraction of the field's
@@ -53,7 +58,8 @@ def _check(vpm, tag, got, ref, scale):
         normwise = float(np.linalg.norm(a - r) / max(np.linalg.norm(r), 1e-300))
         st = vpm.metrics.err_stats(a, r)
         _say("adjoint f32c", tag, name, "rel", st, "scaled max %.2e" % scaled.max(), "normwise %.2e" % normwise)
-        assert scaled.max() < ADJ_SCALED_TOL and normwise < ADJ_NORM_TOL, (tag, name, scaled.max(), normwise)
+        tol = ADJ_SCALED_TOL_SIGMA if name == "sigma" else ADJ_SCALED_TOL
+        assert scaled.max() < tol and normwise < ADJ_NORM_TOL, (tag, name, scaled.max(), normwise)
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
@@ -135,5 +141,5 @@ def test_adjoint_cfg3(vpm, orc, kernel):
         an, mag = flat[grp][k, c], mags[grp][k, c]
         e = abs(fd - an) / mag
         worst = max(worst, e)
-        assert e < ADJ_SCALED_TOL, (k, col, fd, an, mag)
+        assert e < (ADJ_SCALED_TOL_SIGMA if col == 6 else ADJ_SCALED_TOL), (k, col, fd, an, mag)
     _say("cfg3 finite differences, worst error / summation scale", kernel, worst)
