@@ -96,6 +96,8 @@ def rollout_adjoint(vpm, P, kernel, nsteps=3, dt=0.005):
            "loss": "sum |U_i|^2 after the last step", "L": L,
            "forward_ms": tm["forward_ms"], "adjoint_ms": tm["adjoint_ms"],
            "ratio": tm["adjoint_ms"] / tm["forward_ms"],
+           "forward_step_ms": tm["forward_step_ms"], "final_uj_ms": tm["final_uj_ms"],
+           "final_adjoint_ms": tm["final_adjoint_ms"], "step_adjoint_ms": tm["step_adjoint_ms"],
            "uj_evaluations": 3 * nsteps + 1, "checkpoint_bytes": tm["tape_bytes"],
            "grad_norms": {"X": float(np.linalg.norm(bar[:, 0:3])), "Gamma": float(np.linalg.norm(bar[:, 3:6])),
                           "sigma": float(np.linalg.norm(bar[:, 6]))}}

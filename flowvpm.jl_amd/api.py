@@ -541,9 +541,11 @@ def kinetic_energy_rollout_gradient(pfield, dt, nsteps, timings=None):
     tapes = []
     try:
         t0 = time.perf_counter()
+        marks = []
         for _ in range(nsteps):
             tapes.append(Tape(f))
             rungekutta3_taped(pfield, dt, tapes[-1])
+            marks.append(time.perf_counter())
         f.uj_direct(pfield.kernel, pfield.precision, transposed=pfield.transposed)
         out = np.zeros((n, NFIELDS))
         f.get_particles(out, rows=(9, 12))
@@ -553,11 +555,15 @@ def kinetic_energy_rollout_gradient(pfield, dt, nsteps, timings=None):
         Xb, Gb, Sb = f.uj_adjoint(2.0 * U, np.zeros((n, 9)), pfield.kernel, pfield.precision)
         bar = np.zeros((n, NFIELDS))
         bar[:, 0:3], bar[:, 3:6], bar[:, 6] = Xb, Gb, Sb
+        t_adj0 = time.perf_counter()
         for t in reversed(tapes):
             bar = rk3_adjoint(t, bar)
         t2 = time.perf_counter()
         if timings is not None:
+            steps = np.diff([t0] + marks)
             timings.update(forward_ms=1e3 * (t1 - t0), adjoint_ms=1e3 * (t2 - t1),
+                           forward_step_ms=[1e3 * x for x in steps], final_uj_ms=1e3 * (t1 - marks[-1]),
+                           final_adjoint_ms=1e3 * (t_adj0 - t1), step_adjoint_ms=1e3 * (t2 - t_adj0) / nsteps,
                            tape_bytes=3 * nsteps * n * NFIELDS * 8)
     finally:
         for t in tapes:

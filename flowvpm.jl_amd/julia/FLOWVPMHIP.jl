@@ -169,4 +169,59 @@ function ChainRulesCore.rrule(::typeof(uj_hip), pfield::vpm.ParticleField; preci
     return (U, J), uj_hip_pullback
 end
 
+"""
+    rungekutta3_taped!(pfield, dt; precision=0) -> tape
+
+One rungekutta3 step (ReformulatedVPM, no SFS, inviscid, no relaxation) that
+also keeps the three stage states on the device (per-stage checkpoints of X,
+Gamma, sigma with their U and J).  The tape feeds `rungekutta3_pullback!`.
+"""
+mutable struct Tape
+    h::Ptr{Cvoid}
+    pfield::Any
+end
+
+function rungekutta3_taped!(pfield::vpm.ParticleField, dt::Real; precision=0)
+    upload(pfield)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:vpm_tape_create, libvpm), Cint, (Ptr{Cvoid}, Ref{Ptr{Cvoid}}), device_field(pfield), r))
+    t = Tape(r[], pfield)
+    finalizer(t -> ccall((:vpm_tape_destroy, libvpm), Cint, (Ptr{Cvoid},), t.h), t)
+    s = Ref(VpmStepConfig(config(pfield; precision), pfield.formulation.f, pfield.formulation.g,
+            Tuple(pfield.Uinf(pfield.t)), Int32(0), Int32(0), Int32(0), Int32(0), 0.0, 0.0, 0.0, 1.0, 0.0,
+            Int32(0), 0, 0.0, 1.0, 1.5, 0.0))
+    check(ccall((:vpm_rk3_step_taped, libvpm), Cint, (Ptr{Cvoid}, Ref{VpmStepConfig}, Float64, Ptr{Cvoid}),
+                device_field(pfield), s, dt, t.h))
+    download(pfield, Int32(0), Int32(46))
+    return t
+end
+
+"""
+    rungekutta3_pullback!(tape, bar)
+
+bar :: Matrix{Float64}(46, np), the cotangent of pfield.particles after the
+taped step (rows X, Gamma, sigma, M); replaced by the cotangent before it.
+"""
+rungekutta3_pullback!(t::Tape, bar::Matrix{Float64}) =
+    GC.@preserve bar check(ccall((:vpm_rk3_step_adjoint, libvpm), Cint, (Ptr{Cvoid}, Ptr{Float64}), t.h, bar))
+
+# rrule of one step on the particle matrix: the forward records the tape, the
+# pullback runs the reverse sweep over its three stages on the device.
+function rungekutta3_hip_step(pfield::vpm.ParticleField, dt::Real; precision=0)
+    rungekutta3_taped!(pfield, dt; precision)
+    return copy(pfield.particles)
+end
+
+function ChainRulesCore.rrule(::typeof(rungekutta3_hip_step), pfield::vpm.ParticleField, dt::Real; precision=0)
+    tape = rungekutta3_taped!(pfield, dt; precision)
+    out = copy(pfield.particles)
+    function step_pullback(outbar)
+        bar = Matrix{Float64}(ChainRulesCore.unthunk(outbar))
+        rungekutta3_pullback!(tape, bar)
+        return ChainRulesCore.NoTangent(), ChainRulesCore.Tangent{typeof(pfield)}(particles=bar),
+               ChainRulesCore.NoTangent()
+    end
+    return out, step_pullback
+end
+
 end # module
