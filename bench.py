@@ -105,6 +105,43 @@ def rollout_adjoint(vpm, P, kernel, nsteps=3, dt=0.005):
     return out
 
 
+def cfg1_cloud(vpm, n=65536, reps=5):
+    """Random cloud N = 65,536 (X ~ U[0,1]^3, Gamma ~ N(0, 1e-3^2), sigma ~
+    U[0.02, 0.04], seed 1): one U+J evaluation per kernel in FP32c and FP64 on
+    the GPU, throughput and per-output error of FP32c against FP64."""
+    api = vpm.api
+    P = vpm.workloads.random_cloud(n, seed=1)
+    f = api.Field(n, 0)
+    f.set_particles(P)
+    out = {"N": n}
+    for kernel in ("gaussianerf", "winckelmans"):
+        r = {}
+        rows = {}
+        for prec in ("f32c", "f64"):
+            f.uj_direct(kernel, prec)
+            f.synchronize()
+            t0 = time.perf_counter()
+            k = reps if prec == "f32c" else 1
+            for _ in range(k):
+                f.uj_direct(kernel, prec)
+            f.synchronize()
+            dt = (time.perf_counter() - t0) / k
+            r["%s_pairs_per_s" % prec] = float(n) * n / dt
+            Q = np.zeros_like(P)
+            f.get_particles(Q, rows=(9, 24))
+            rows[prec] = Q
+        A, R = rows["f32c"], rows["f64"]
+        r["error_f32c_vs_f64"] = {
+            "U": vpm.metrics.err_stats(A[:, 9:12], R[:, 9:12]),
+            "J": vpm.metrics.err_stats(A[:, 15:24], R[:, 15:24]),
+            "dGamma_dt": vpm.metrics.err_stats(vpm.metrics.stretching(A[:, 15:24], P[:, 3:6]),
+                                               vpm.metrics.stretching(R[:, 15:24], P[:, 3:6]))}
+        out[kernel] = r
+        log("bench: cfg1 %s %s" % (kernel, r))
+    f.close()
+    return out
+
+
 def cfg2_rollout(vpm, nsteps=10, dt=0.01):
     """Two leapfrogging rings, N = 262,144: nsteps rk3 steps of rVPM with the
     dynamic SFS model (SFS_Cd_twolevel_nobackscatter), core spreading
@@ -147,6 +184,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-rollout", action="store_true",
                     help="skip forward+adjoint through 3 RK3 steps on the jet")
+    ap.add_argument("--no-cfg1", action="store_true",
+                    help="skip the N=65,536 random-cloud accuracy/throughput report")
     ap.add_argument("--no-cfg2", action="store_true",
                     help="skip the 10-step leapfrog-rings rollout, FP32c vs FP64")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -248,6 +287,9 @@ def main():
 
     if not args.no_rollout:
         res["rollout_adjoint"] = rollout_adjoint(vpm, P, kernel)
+
+    if not args.no_cfg1:
+        res["cfg1_cloud"] = cfg1_cloud(vpm)
 
     if not args.no_cfg2:
         res["cfg2_rollout"] = cfg2_rollout(vpm)
