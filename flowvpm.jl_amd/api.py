@@ -12,6 +12,7 @@ Reference interfaces mirrored here (src/ paths of FLOWVPM.jl):
   rungekutta3(pfield, dt; relax) / nextstep   FLOWVPM_timeintegration.jl:388-461,
                                               FLOWVPM_particlefield.jl:435-460
   run_vpm!(pfield, dt, nsteps)                FLOWVPM_utils.jl:41-139 (relax cadence)
+  zeta_direct(pfield)                         FLOWVPM_viscous.jl:486-515
   kernel / formulation / relaxation / SFS / viscous aliases   FLOWVPM.jl:80-256
 
 Indices are 0-based (particle i of Julia is i-1 here); particle storage is
@@ -170,6 +171,11 @@ class Field:
     def estr_direct(self, kernel=kernel_default, precision="f32c", transposed=True):
         cfg = make_config(kernel, precision, transposed)
         check(lib().vpm_estr_direct(self.handle, C.byref(cfg)))
+
+    def zeta_direct(self, kernel=kernel_default, precision="f32c"):
+        """vpm_zeta_direct: J[1:3] of every particle <- the zeta field."""
+        cfg = make_config(kernel, precision, True)
+        check(lib().vpm_zeta_direct(self.handle, C.byref(cfg)))
 
     def uj_adjoint(self, Ubar, Jbar, kernel=kernel_default, precision="f32c"):
         """vpm_uj_adjoint on the resident particles: returns (dL/dX (n,3),
@@ -479,6 +485,18 @@ def uj_adjoint(pfield, Ubar, Jbar, *, kernel=None, precision="f32c", device=None
     f = _scratch_field(n, 0 if device is None else device)
     f.set_particles(P)
     return f.uj_adjoint(Ubar, Jbar, kernel, precision)
+
+
+def zeta_direct(pfield, *, precision=None):
+    """zeta_direct(pfield) (FLOWVPM_viscous.jl:486-493): J[1:3] of every
+    particle (static included) <- sum_j Gamma_j zeta(r/sigma_j)/sigma_j^3, on
+    the device; the host copy is updated."""
+    if pfield.np == 0:
+        return
+    f = pfield.device_field()
+    f.set_particles(pfield.particles[:pfield.np])
+    f.zeta_direct(pfield.kernel, precision or pfield.precision)
+    f.get_particles(pfield.particles, rows=(15, 18), np_=pfield.np)
 
 
 class Tape:

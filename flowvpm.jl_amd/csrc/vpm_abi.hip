@@ -472,6 +472,14 @@ int vpm_rk3_step(vpm_field f, vpm_step_config* s, double dt, int32_t relax_now)
     return rk3_step_locked(f, s, dt, relax_now, nullptr);
 }
 
+int vpm_zeta_direct(vpm_field f, const vpm_config* cfg)
+{
+    VPM_FIELD_GUARD(f);
+    if (int rc = check_cfg(cfg)) return rc;
+    if (int rc = launch_zeta(f, cfg->kernel, cfg->precision)) return rc;
+    return sync(f);
+}
+
 int vpm_tape_create(vpm_field f, vpm_tape* out)
 {
     if (!out) return fail(VPM_EINVAL, "null output pointer");

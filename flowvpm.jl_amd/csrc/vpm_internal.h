@@ -126,6 +126,7 @@ int launch_update_states_adjoint(vpm_field f, double* B, double a, double b, dou
                                  int transposed, double* Ub, double* Jb);
 int launch_adj_add_bar(vpm_field f, double* B, const double* dX, const double* dG, const double* dS);
 int launch_adj_step_start(vpm_field f, double* B);
+int launch_zeta(vpm_field f, int kernel, int precision);
 
 double zeta0_of(int kernel);
 

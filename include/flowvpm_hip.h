@@ -148,6 +148,12 @@ int vpm_sfs_dynamic_after(vpm_field f, const vpm_config* cfg, double alpha, doub
  * CoreSpreading would reset core sizes (viscous.jl:202-220, not provided). */
 int vpm_rk3_step(vpm_field f, vpm_step_config* s, double dt, int32_t relax_now);
 
+/* zeta_direct (src/FLOWVPM_viscous.jl:486-515): J[1:3] of every particle,
+ * static ones included, <- sum over every particle j (the target itself
+ * included) of Gamma_j zeta(r/sigma_j)/sigma_j^3.  The other J rows are left
+ * as they are.  The singular kernel always runs in FP64. */
+int vpm_zeta_direct(vpm_field f, const vpm_config* cfg);
+
 /* ---- adjoint (replaces src/FLOWVPM_rrules.jl:179-330, UJS_pullback) ------ */
 /* For L = sum_i Ubar_i . U_i + sum_i sum_k Jbar_i[k] J_i[k] with U, J from one
  * all-pairs evaluation of the field (fmm.jl:102-168), writes dL/dX (3 np),

@@ -72,6 +72,7 @@ def lib():
         L.vpmo_update_particle_states_adjoint.argtypes = [_dp, _dp, C.c_int64, C.c_double, C.c_double,
                                                           C.c_double, C.c_double, C.c_double, C.c_int]
         L.vpmo_uj_adjoint_scale.argtypes = [_dp, C.c_int64, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int]
+        L.vpmo_zeta_direct.argtypes = [_dp, C.c_int64, C.c_int, C.c_int]
         L.vpmo_rk3_step_adjoint.restype = C.c_int
         L.vpmo_rk3_step_adjoint.argtypes = [_dp, C.c_int64, C.c_void_p, C.c_double, _dp]
         L.vpmo_core_spreading_rk3.restype = C.c_int
@@ -156,6 +157,11 @@ def uj_direct(P, kernel, reset=True, reset_sfs=False, sfs=False, transposed=True
     lib().vpmo_uj_direct(_check(P), np_ if np_ is not None else P.shape[0], _kid(kernel),
                          int(reset), int(reset_sfs), int(sfs), int(transposed),
                          nthreads or nthreads_default())
+
+
+def zeta_direct(P, kernel, nthreads=None):
+    """zeta_direct (viscous.jl:486-515), in place on J rows 15:18."""
+    lib().vpmo_zeta_direct(_check(P), P.shape[0], _kid(kernel), nthreads or nthreads_default())
 
 
 def estr_direct(P, kernel, transposed=True, np_=None, nthreads=None):

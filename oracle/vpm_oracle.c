@@ -23,6 +23,7 @@
  *                      inputs, plus the single-ring translation speed test
  *                      of test/runtests_singlevortexring.jl:22-23,143
  *   vpmo_uj_adjoint    central finite differences of vpmo_loss_local
+ *   vpmo_zeta_direct   mpmath brute force; one particle gives Gamma zeta(0)/sigma^3
  *   vpmo_update_particle_states_adjoint / vpmo_rk3_step_adjoint
  *                      central finite differences of vpmo_update_particle_states
  *                      and of a loss through whole vpmo_rk3_step calls
@@ -850,4 +851,33 @@ int vpmo_rk3_step_adjoint(const double* P0, int64_t np, const vpmo_settings* s, 
     free(Ub);
     free(tape);
     return 0;
+}
+
+/* ---- zeta_direct (viscous.jl:486-515) ----
+ * J[1:3] of every particle (static included) is reset, then every target i
+ * gets sum over every source j (i itself included: no r == 0 skip) of
+ * Gamma_j zeta(r/sigma_j)/sigma_j^3. */
+void vpmo_zeta_direct(double* P, int64_t np, int kernel, int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+    for (int64_t i = 0; i < np; i++)                                 /* :487-489 */
+        for (int k = 0; k < 3; k++) P[i * VPMO_NFIELDS + RJ + k] = 0.0;
+#pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
+    for (int64_t i = 0; i < np; i++) {
+        double* pi = P + i * VPMO_NFIELDS;
+        double acc[3] = {0, 0, 0};
+        for (int64_t j = 0; j < np; j++) {
+            const double* pj = P + j * VPMO_NFIELDS;
+            const double dX1 = pi[RX] - pj[RX];                      /* :500-502 */
+            const double dX2 = pi[RX + 1] - pj[RX + 1];
+            const double dX3 = pi[RX + 2] - pj[RX + 2];
+            const double r = sqrt(dX1 * dX1 + dX2 * dX2 + dX3 * dX3);
+            const double s = pj[RSIG];
+            const double zeta_sgm = 1 / (s * s * s) * vpmo_zeta(kernel, r / s);   /* :505 */
+            acc[0] += pj[RG + 0] * zeta_sgm;                        /* :507-509 */
+            acc[1] += pj[RG + 1] * zeta_sgm;
+            acc[2] += pj[RG + 2] * zeta_sgm;
+        }
+        for (int k = 0; k < 3; k++) pi[RJ + k] = acc[k];
+    }
 }

@@ -107,6 +107,10 @@ void vpmo_uj_adjoint_scale(const double* P, int64_t np, int kernel,
  * source), summed in long double; for finite differences. */
 double vpmo_loss_local(const double* P, int64_t np, int kernel,
                        const double* Ubar, const double* Jbar, int64_t k);
+/* zeta_direct (viscous.jl:486-515): J[1:3] of every particle (static
+ * included) <- sum over all particles j (self included) of
+ * Gamma_j zeta(r/sigma_j)/sigma_j^3. */
+void vpmo_zeta_direct(double* P, int64_t np, int kernel, int nthreads);
 /* Pullback of update_particle_states (ReformulatedVPM, no SFS;
  * timeintegration.jl:467-528, rrules.jl:405-555).  B: cotangent records
  * (46 rows per particle) of the updated state in, of the state before the

@@ -342,3 +342,38 @@ def test_adjoint_scale_bounds_adjoint(orc, vpm):
     got = orc.uj_adjoint(P2, "gaussianerf", w2, W2)
     mag = orc.uj_adjoint_scale(P2, "gaussianerf", w2, W2)
     np.testing.assert_allclose(np.abs(got[1]), mag[1], rtol=1e-14)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_zeta_direct_matches_mpmath(orc, vpm, kernel):
+    """zeta_direct against 30-digit brute force of the closed-form zeta
+    (kernel.jl:45,51,60,69-74) on a small cloud with a duplicated particle;
+    J[1:3] of static particles is written too and the other J rows untouched."""
+    import mpmath as mp
+    mp.mp.dps = 30
+    P = _cloud(vpm, n=24, seed=31)
+    P[5, 0:3] = P[4, 0:3]                   # coincident pair: the singular delta sees both
+    P[7, 42] = 1.0
+    P[:, 18:24] = 7.0
+    Q = P.copy()
+    orc.zeta_direct(Q, kernel)
+    c1 = 1 / (2 * mp.pi) ** mp.mpf(1.5)
+    def zeta(q):
+        if kernel == "singular":
+            return mp.mpf(1) if q == 0 else mp.mpf(0)
+        if kernel == "gaussianerf":
+            return c1 * mp.exp(-q * q / 2)
+        if kernel == "gaussian":
+            return 3 / (4 * mp.pi) * mp.exp(-q ** 3)
+        return 7.5 / (4 * mp.pi) / (q * q + 1) ** mp.mpf(3.5)
+    for i in range(24):
+        acc = [mp.mpf(0)] * 3
+        for j in range(24):
+            d = [mp.mpf(P[i, k]) - mp.mpf(P[j, k]) for k in range(3)]
+            r = mp.sqrt(sum(x * x for x in d))
+            s = mp.mpf(P[j, 6])
+            z = zeta(r / s) / s ** 3
+            acc = [acc[k] + mp.mpf(P[j, 3 + k]) * z for k in range(3)]
+        ref = np.array([float(a) for a in acc])
+        np.testing.assert_allclose(Q[i, 15:18], ref, rtol=1e-12, atol=1e-14 * max(1.0, np.abs(ref).max()))
+    assert np.all(Q[:, 18:24] == 7.0)
