@@ -67,6 +67,7 @@ _SIGS = {
                                         C.c_double, C.c_int32, C.c_int32]),
     "vpm_rk3_step": (C.c_int, [C.c_void_p, C.POINTER(StepConfig), C.c_double, C.c_int32]),
     "vpm_uj_adjoint": (C.c_int, [C.c_void_p, C.POINTER(Config), _dptr, _dptr, _dptr, _dptr, _dptr]),
+    "vpm_estr_ranges": (C.c_int, [C.c_void_p, C.POINTER(Config), _i64p, _i64p, _i64p, _i64p, C.c_int64]),
     "vpm_zeta_direct": (C.c_int, [C.c_void_p, C.POINTER(Config)]),
     "vpm_tape_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "vpm_tape_destroy": (C.c_int, [C.c_void_p]),

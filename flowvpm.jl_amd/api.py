@@ -9,6 +9,7 @@ Reference interfaces mirrored here (src/ paths of FLOWVPM.jl):
   UJ_direct(source, target)                   FLOWVPM_UJ.jl:48-50  -> UJ_hip_fields
   fmm.direct!(tgt, tgt_range, _, src, _, src_range)  FLOWVPM_fmm.jl:102-168 -> p2p_ranges
   Estr_direct!(pfield)                        FLOWVPM_subfilterscale_models.jl:43-92
+  Estr_fmm!(... direct_list)                  FLOWVPM_subfilterscale_models.jl:157-188 -> estr_ranges
   rungekutta3(pfield, dt; relax) / nextstep   FLOWVPM_timeintegration.jl:388-461,
                                               FLOWVPM_particlefield.jl:435-460
   run_vpm!(pfield, dt, nsteps)                FLOWVPM_utils.jl:41-139 (relax cadence)
@@ -385,6 +386,28 @@ def p2p_ranges(pfield, tb, te, sb, se, *, kernel=None, precision=None, device=No
     tmp = np.empty_like(P)
     f.get_particles(tmp, rows=(9, 24))
     P[:, 9:24] = tmp[:, 9:24]
+
+
+def estr_ranges(pfield, tb, te, sb, se, *, kernel=None, precision=None, transposed=None, device=None):
+    """Estr_fmm! over a direct list of 0-based half-open (target, source)
+    range pairs (FLOWVPM_subfilterscale_models.jl:157-188): E_str pairs onto
+    SFS, no static skip, no reset."""
+    P = _as_matrix(pfield)
+    pf = pfield if isinstance(pfield, ParticleField) else None
+    kernel = kernel or (pf.kernel if pf else kernel_default)
+    precision = precision or (pf.precision if pf else "f32c")
+    transposed = (pf.transposed if pf else True) if transposed is None else transposed
+    f = _scratch_field(P.shape[0], 0 if device is None else device)
+    f.set_particles(P)
+    arrs = [np.ascontiguousarray(a, dtype=np.int64) for a in (tb, te, sb, se)]
+    if not (len(arrs[0]) == len(arrs[1]) == len(arrs[2]) == len(arrs[3])):
+        raise VPMError(_lib.VPM_EINVAL, "range arrays differ in length")
+    cfg = make_config(kernel, precision, transposed)
+    ptr = [a.ctypes.data_as(_lib._i64p) for a in arrs]
+    check(lib().vpm_estr_ranges(f.handle, C.byref(cfg), ptr[0], ptr[1], ptr[2], ptr[3], len(arrs[0])))
+    tmp = np.empty_like(P)
+    f.get_particles(tmp, rows=(39, 42))
+    P[:, 39:42] = tmp[:, 39:42]
 
 
 def Estr_hip(pfield, *, kernel=None, precision=None, transposed=None, device=None):

@@ -126,15 +126,22 @@ __global__ __launch_bounds__(kBlock) void ranges_f64_kernel(const RangeItem* __r
 }
 #pragma clang fp contract(on)
 
-int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb, const int64_t* te,
-                      const int64_t* sb, const int64_t* se, int64_t npairs)
+// Host plan of a pair list: elementary target intervals, each with the
+// source ranges of every pair covering it (pair order), cut into blocks of
+// at most 256 targets.
+struct RangePlan {
+    std::vector<RangeItem> items;
+    std::vector<int64_t> lsb, lse;
+};
+
+static void plan_ranges(const int64_t* tb, const int64_t* te, const int64_t* sb, const int64_t* se, int64_t npairs,
+                        RangePlan& plan)
 {
-    // elementary target intervals
     std::vector<int64_t> cuts;
     cuts.reserve(2 * (size_t)npairs);
     for (int64_t k = 0; k < npairs; ++k)
         if (tb[k] < te[k] && sb[k] < se[k]) { cuts.push_back(tb[k]); cuts.push_back(te[k]); }
-    if (cuts.empty()) return VPM_OK;
+    if (cuts.empty()) return;
     std::sort(cuts.begin(), cuts.end());
     cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
     const size_t nint = cuts.size() - 1;
@@ -146,42 +153,57 @@ int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb,
         for (size_t m = lo; m < hi; ++m) count[m + 1]++;
     }
     for (size_t m = 0; m < nint; ++m) count[m + 1] += count[m];
-    std::vector<int64_t> lsb((size_t)count[nint]), lse((size_t)count[nint]);
+    plan.lsb.assign((size_t)count[nint], 0);
+    plan.lse.assign((size_t)count[nint], 0);
     std::vector<int64_t> fill(count.begin(), count.end() - 1);
     for (int64_t k = 0; k < npairs; ++k) {
         if (!(tb[k] < te[k] && sb[k] < se[k])) continue;
         const size_t lo = std::lower_bound(cuts.begin(), cuts.end(), tb[k]) - cuts.begin();
         const size_t hi = std::lower_bound(cuts.begin(), cuts.end(), te[k]) - cuts.begin();
         for (size_t m = lo; m < hi; ++m) {
-            lsb[(size_t)fill[m]] = sb[k];
-            lse[(size_t)fill[m]] = se[k];
+            plan.lsb[(size_t)fill[m]] = sb[k];
+            plan.lse[(size_t)fill[m]] = se[k];
             fill[m]++;
         }
     }
-    std::vector<RangeItem> items;
     for (size_t m = 0; m < nint; ++m) {
         if (count[m + 1] == count[m]) continue;
         for (int64_t t = cuts[m]; t < cuts[m + 1]; t += kBlock)
-            items.push_back({t, std::min<int64_t>(t + kBlock, cuts[m + 1]), count[m], count[m + 1]});
+            plan.items.push_back({t, std::min<int64_t>(t + kBlock, cuts[m + 1]), count[m], count[m + 1]});
     }
-    if (items.empty()) return VPM_OK;
+}
 
+static int upload_plan(vpm_field f, const RangePlan& plan, RangeItem*& d_items, int64_t*& d_lsb, int64_t*& d_lse)
+{
+    const size_t bytes_items = sizeof(RangeItem) * plan.items.size();
+    const size_t bytes_list = sizeof(int64_t) * plan.lsb.size();
+    if (int rc = ensure(f->rng, bytes_items + 2 * bytes_list + 64)) return rc;
+    char* base = (char*)f->rng.p;
+    d_items = (RangeItem*)base;
+    d_lsb = (int64_t*)(base + bytes_items);
+    d_lse = d_lsb + plan.lsb.size();
+    hipError_t e;
+    if ((e = hipMemcpyAsync(d_items, plan.items.data(), bytes_items, hipMemcpyHostToDevice, f->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(d_lsb, plan.lsb.data(), bytes_list, hipMemcpyHostToDevice, f->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(d_lse, plan.lse.data(), bytes_list, hipMemcpyHostToDevice, f->stream)) != hipSuccess)
+        return hip_fail(e, "hipMemcpyAsync(range lists)");
+    return VPM_OK;
+}
+
+int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb, const int64_t* te,
+                      const int64_t* sb, const int64_t* se, int64_t npairs)
+{
+    RangePlan plan;
+    plan_ranges(tb, te, sb, se, npairs, plan);
+    if (plan.items.empty()) return VPM_OK;
     Staged st;
     int rc;
     if ((rc = stage_field(f, kernel, precision, false, st))) return rc;
-    const size_t bytes_items = sizeof(RangeItem) * items.size();
-    const size_t bytes_list = sizeof(int64_t) * lsb.size();
-    if ((rc = ensure(f->rng, bytes_items + 2 * bytes_list + 64))) return rc;
-    char* base = (char*)f->rng.p;
-    RangeItem* d_items = (RangeItem*)base;
-    int64_t* d_lsb = (int64_t*)(base + bytes_items);
-    int64_t* d_lse = d_lsb + lsb.size();
+    RangeItem* d_items;
+    int64_t *d_lsb, *d_lse;
+    if ((rc = upload_plan(f, plan, d_items, d_lsb, d_lse))) return rc;
+    const dim3 grid((unsigned)plan.items.size());
     hipError_t e;
-    if ((e = hipMemcpyAsync(d_items, items.data(), bytes_items, hipMemcpyHostToDevice, f->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(d_lsb, lsb.data(), bytes_list, hipMemcpyHostToDevice, f->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(d_lse, lse.data(), bytes_list, hipMemcpyHostToDevice, f->stream)) != hipSuccess)
-        return hip_fail(e, "hipMemcpyAsync(range lists)");
-    const dim3 grid((unsigned)items.size());
     timing_begin(f);
     if (precision == VPM_PREC_F64) {
         switch (kernel) {
@@ -202,6 +224,259 @@ int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb,
     if (e != hipSuccess) return hip_fail(e, "p2p_ranges kernel");
     timing_end(f, precision == VPM_PREC_F64 ? 1 : 0);
     // the host vectors must outlive the asynchronous copies
+    e = hipStreamSynchronize(f->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    return VPM_OK;
+}
+
+// ------------------------------------------------------------ E_str ranges
+// Estr_fmm! over a direct list (src/FLOWVPM_subfilterscale_models.jl:157-188):
+// E_str pairs (:16-41) for every (target range, source range), no static skip,
+// accumulated onto SFS.
+//
+// F32C: within a block (<= 256 targets of one interval), J' = J - J_ref with
+// J_ref the J of the block's first target (FP64), and
+//     sum_j zeta_j (J'_i - J'_j)^T Gamma_j = J'_i^T A - B,
+//     A = sum zeta Gamma, B = sum zeta J'_j^T Gamma_j,
+// positions hi/lo relative to that target, sources beyond q_far skipped per
+// wave (Gaussian-erf, Gaussian), sums folded into TwoSum every 32 sources.
+template <int K>
+__global__ __launch_bounds__(kBlock) void estr_ranges_f32c_kernel(const RangeItem* __restrict__ items,
+                                                                  const int64_t* __restrict__ lsb,
+                                                                  const int64_t* __restrict__ lse,
+                                                                  double* __restrict__ P, int transposed)
+{
+    __shared__ float4 lp[kTile];   // s'_hi, a
+    __shared__ float4 lg[kTile];   // Gamma, b
+    __shared__ float4 lq[kTile];   // J'^T Gamma (or J' Gamma), cut
+    __shared__ float4 ll[kTile];   // s'_lo
+    const RangeItem it = items[blockIdx.x];
+    const int tid = threadIdx.x;
+    const double* r0 = P + it.t_begin * NF;
+    const double ox = r0[RX], oy = r0[RX + 1], oz = r0[RX + 2];
+    double Jr[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) Jr[c] = r0[RJ + c];
+    const int64_t t = it.t_begin + tid;
+    const bool active = t < it.t_end;
+    const double* tr = P + (active ? t : it.t_begin) * NF;
+    float tx, ty, tz, tlx, tly, tlz;
+    split_rel(tr[RX], ox, tx, tlx);
+    split_rel(tr[RX + 1], oy, ty, tly);
+    split_rel(tr[RX + 2], oz, tz, tlz);
+    float sum[6], err[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) { sum[c] = 0.f; err[c] = 0.f; }
+    const double l2e = 1.4426950408889634;
+    for (int64_t l = it.l_begin; l < it.l_end; ++l) {
+        const int64_t sb = lsb[l], se = lse[l];
+        for (int64_t base = sb; base < se; base += kTile) {
+            const int n = (int)((se - base) < kTile ? (se - base) : kTile);
+            __syncthreads();
+            if (tid < n) {
+                const double* sr = P + (base + tid) * NF;
+                float4 p, lo;
+                split_rel(sr[RX], ox, p.x, lo.x);
+                split_rel(sr[RX + 1], oy, p.y, lo.y);
+                split_rel(sr[RX + 2], oz, p.z, lo.z);
+                const double sg = sr[RSIG];
+                const double si = 1.0 / sg, s2 = si * si, s3 = s2 * si;
+                double a, b, cut;
+                if (K == VPM_KERNEL_GAUSSIANERF) { a = -0.5 * l2e * s2; b = kConst1 * s3; cut = 49.0 * sg * sg; }
+                else if (K == VPM_KERNEL_GAUSSIAN) { a = -l2e * s3; b = kConst3 * s3; cut = 9.0 * sg * sg; }
+                else { a = s2; b = kConst4 * 7.5 * s3; cut = (double)FLT_MAX; }
+                p.w = (float)a;
+                lo.w = 0.f;
+                const double G0 = sr[RG], G1 = sr[RG + 1], G2 = sr[RG + 2];
+                double Jp[9];
+#pragma unroll
+                for (int c = 0; c < 9; ++c) Jp[c] = sr[RJ + c] - Jr[c];
+                float4 q;
+                if (transposed) {
+                    q = make_float4((float)(Jp[0] * G0 + Jp[1] * G1 + Jp[2] * G2),
+                                    (float)(Jp[3] * G0 + Jp[4] * G1 + Jp[5] * G2),
+                                    (float)(Jp[6] * G0 + Jp[7] * G1 + Jp[8] * G2), (float)cut);
+                } else {
+                    q = make_float4((float)(Jp[0] * G0 + Jp[3] * G1 + Jp[6] * G2),
+                                    (float)(Jp[1] * G0 + Jp[4] * G1 + Jp[7] * G2),
+                                    (float)(Jp[2] * G0 + Jp[5] * G1 + Jp[8] * G2), (float)cut);
+                }
+                lp[tid] = p;
+                ll[tid] = lo;
+                lq[tid] = q;
+                lg[tid] = make_float4((float)G0, (float)G1, (float)G2, (float)b);
+            }
+            __syncthreads();
+            for (int s0 = 0; s0 < n; s0 += 32) {
+                const int s1 = min(s0 + 32, n);
+                float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                for (int s = s0; s < s1; ++s) {
+                    const float4 a = lp[s];
+                    const float4 c = lq[s];
+                    float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
+                    float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    if (__any(r2 < c.w)) {
+                        const float4 lo = ll[s];
+                        const float4 g = lg[s];
+                        dx += tlx - lo.x;
+                        dy += tly - lo.y;
+                        dz += tlz - lo.z;
+                        r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                        float z;
+                        if constexpr (K == VPM_KERNEL_GAUSSIANERF) {
+                            z = g.w * ex2(a.w * r2);
+                        } else if constexpr (K == VPM_KERNEL_GAUSSIAN) {
+                            const float r = r2 > 0.f ? r2 * rsq(r2) : 0.f;
+                            z = g.w * ex2(a.w * r2 * r);
+                        } else {
+                            const float w = rsq(fmaf(a.w, r2, 1.f));
+                            const float w2 = w * w;
+                            z = g.w * (w2 * w2) * (w2 * w);
+                        }
+                        acc[0] = fmaf(z, g.x, acc[0]);
+                        acc[1] = fmaf(z, g.y, acc[1]);
+                        acc[2] = fmaf(z, g.z, acc[2]);
+                        acc[3] = fmaf(z, c.x, acc[3]);
+                        acc[4] = fmaf(z, c.y, acc[4]);
+                        acc[5] = fmaf(z, c.z, acc[5]);
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    const float x = acc[c];
+                    const float tt = sum[c] + x;
+                    const float bp = tt - sum[c];
+                    err[c] += (sum[c] - (tt - bp)) + (x - bp);
+                    sum[c] = tt;
+                }
+            }
+        }
+    }
+    if (active) {
+        double* rec = P + t * NF;
+        double A[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) A[c] = (double)sum[c] + (double)err[c];
+        double Jp[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) Jp[c] = rec[RJ + c] - Jr[c];
+        for (int q = 0; q < 3; ++q) {
+            const double S = transposed ? Jp[3 * q] * A[0] + Jp[3 * q + 1] * A[1] + Jp[3 * q + 2] * A[2] - A[3 + q]
+                                        : Jp[q] * A[0] + Jp[3 + q] * A[1] + Jp[6 + q] * A[2] - A[3 + q];
+            rec[RSFS + q] += S;
+        }
+    }
+}
+
+#pragma clang fp contract(off)
+template <int K>
+__device__ __forceinline__ double zeta_ref_r(double r)
+{
+    if constexpr (K == VPM_KERNEL_SINGULAR) return r == 0 ? 1.0 : 0.0;
+    else if constexpr (K == VPM_KERNEL_GAUSSIAN) return kConst3 * exp(-r * r * r);
+    else if constexpr (K == VPM_KERNEL_GAUSSIANERF) return kConst1 * exp(-r * r / 2);
+    else {
+        double temp = r * r + 1;
+        const double temp2 = temp * temp * temp;
+        temp = temp2 * temp2 * temp;
+        return kConst4 * 7.5 / sqrt(temp);
+    }
+}
+
+// FP64: subfilterscale_models.jl:16-41 per pair, list order, accumulating
+// onto SFS directly as the reference does.
+template <int K>
+__global__ __launch_bounds__(kBlock) void estr_ranges_f64_kernel(const RangeItem* __restrict__ items,
+                                                                 const int64_t* __restrict__ lsb,
+                                                                 const int64_t* __restrict__ lse,
+                                                                 double* __restrict__ P, int transposed)
+{
+    __shared__ double lx[kTile][4];
+    __shared__ double lg[kTile][3];
+    __shared__ double lj[kTile][9];
+    const RangeItem it = items[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int64_t t = it.t_begin + tid;
+    const bool active = t < it.t_end;
+    const double* tr = P + (active ? t : it.t_begin) * NF;
+    const double tx = tr[RX], ty = tr[RX + 1], tz = tr[RX + 2];
+    double JT[9];
+    for (int c = 0; c < 9; ++c) JT[c] = tr[RJ + c];
+    double s1 = tr[RSFS], s2 = tr[RSFS + 1], s3 = tr[RSFS + 2];
+    for (int64_t l = it.l_begin; l < it.l_end; ++l) {
+        const int64_t sb = lsb[l], se = lse[l];
+        for (int64_t base = sb; base < se; base += kTile) {
+            const int n = (int)((se - base) < kTile ? (se - base) : kTile);
+            __syncthreads();
+            if (tid < n) {
+                const double* sr = P + (base + tid) * NF;
+                lx[tid][0] = sr[RX]; lx[tid][1] = sr[RX + 1]; lx[tid][2] = sr[RX + 2]; lx[tid][3] = sr[RSIG];
+                lg[tid][0] = sr[RG]; lg[tid][1] = sr[RG + 1]; lg[tid][2] = sr[RG + 2];
+                for (int c = 0; c < 9; ++c) lj[tid][c] = sr[RJ + c];
+            }
+            __syncthreads();
+            for (int s = 0; s < n; ++s) {
+                const double dx = lx[s][0] - tx, dy = lx[s][1] - ty, dz = lx[s][2] - tz;
+                const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                const double* JS = lj[s];
+                const double* GS = lg[s];
+                double S1, S2, S3;
+                if (transposed) {
+                    S1 = (JT[0] - JS[0]) * GS[0] + (JT[1] - JS[1]) * GS[1] + (JT[2] - JS[2]) * GS[2];
+                    S2 = (JT[3] - JS[3]) * GS[0] + (JT[4] - JS[4]) * GS[1] + (JT[5] - JS[5]) * GS[2];
+                    S3 = (JT[6] - JS[6]) * GS[0] + (JT[7] - JS[7]) * GS[1] + (JT[8] - JS[8]) * GS[2];
+                } else {
+                    S1 = (JT[0] - JS[0]) * GS[0] + (JT[3] - JS[3]) * GS[1] + (JT[6] - JS[6]) * GS[2];
+                    S2 = (JT[1] - JS[1]) * GS[0] + (JT[4] - JS[4]) * GS[1] + (JT[7] - JS[7]) * GS[2];
+                    S3 = (JT[2] - JS[2]) * GS[0] + (JT[5] - JS[5]) * GS[1] + (JT[8] - JS[8]) * GS[2];
+                }
+                const double sigma_inv = 1.0 / lx[s][3];
+                const double zeta_sgm = zeta_ref_r<K>(r * sigma_inv) * sigma_inv * sigma_inv * sigma_inv;
+                s1 += zeta_sgm * S1;
+                s2 += zeta_sgm * S2;
+                s3 += zeta_sgm * S3;
+            }
+        }
+    }
+    if (active) {
+        double* rec = P + t * NF;
+        rec[RSFS] = s1;
+        rec[RSFS + 1] = s2;
+        rec[RSFS + 2] = s3;
+    }
+}
+#pragma clang fp contract(on)
+
+int launch_estr_ranges(vpm_field f, int kernel, int precision, int transposed, const int64_t* tb,
+                       const int64_t* te, const int64_t* sb, const int64_t* se, int64_t npairs)
+{
+    RangePlan plan;
+    plan_ranges(tb, te, sb, se, npairs, plan);
+    if (plan.items.empty()) return VPM_OK;
+    int rc;
+    RangeItem* d_items;
+    int64_t *d_lsb, *d_lse;
+    if ((rc = upload_plan(f, plan, d_items, d_lsb, d_lse))) return rc;
+    const dim3 grid((unsigned)plan.items.size());
+    timing_begin(f);
+    if (precision == VPM_PREC_F64 || kernel == VPM_KERNEL_SINGULAR) {
+        switch (kernel) {
+#define VPM_E64(KID) case KID: hipLaunchKernelGGL(estr_ranges_f64_kernel<KID>, grid, dim3(kBlock), 0, f->stream, d_items, d_lsb, d_lse, f->P, transposed); break;
+            VPM_E64(VPM_KERNEL_SINGULAR) VPM_E64(VPM_KERNEL_GAUSSIAN) VPM_E64(VPM_KERNEL_GAUSSIANERF) VPM_E64(VPM_KERNEL_WINCKELMANS)
+#undef VPM_E64
+        default: return fail(VPM_EINVAL, "unknown kernel id");
+        }
+    } else {
+        switch (kernel) {
+#define VPM_E32(KID) case KID: hipLaunchKernelGGL(estr_ranges_f32c_kernel<KID>, grid, dim3(kBlock), 0, f->stream, d_items, d_lsb, d_lse, f->P, transposed); break;
+            VPM_E32(VPM_KERNEL_GAUSSIAN) VPM_E32(VPM_KERNEL_GAUSSIANERF) VPM_E32(VPM_KERNEL_WINCKELMANS)
+#undef VPM_E32
+        default: return fail(VPM_EINVAL, "unknown kernel id");
+        }
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "estr_ranges kernel");
+    timing_end(f, 2);
     e = hipStreamSynchronize(f->stream);
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     return VPM_OK;

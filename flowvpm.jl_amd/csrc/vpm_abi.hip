@@ -332,6 +332,23 @@ int vpm_p2p_ranges(vpm_field f, const vpm_config* cfg, const int64_t* tb, const 
     return sync(f);
 }
 
+int vpm_estr_ranges(vpm_field f, const vpm_config* cfg, const int64_t* tb, const int64_t* te, const int64_t* sb,
+                    const int64_t* se, int64_t npairs)
+{
+    VPM_FIELD_GUARD(f);
+    if (int rc = check_cfg(cfg)) return rc;
+    if (npairs < 0) return fail(VPM_EINVAL, "npairs < 0");
+    if (npairs == 0) return VPM_OK;
+    if (!tb || !te || !sb || !se) return fail(VPM_EINVAL, "null range array");
+    for (int64_t k = 0; k < npairs; ++k) {
+        if (tb[k] < 0 || te[k] > f->np || tb[k] > te[k] || sb[k] < 0 || se[k] > f->np || sb[k] > se[k])
+            return fail(VPM_EINVAL, "range pair " + std::to_string(k) + " outside [0, np)");
+    }
+    if (int rc = launch_estr_ranges(f, cfg->kernel, cfg->precision, cfg->transposed, tb, te, sb, se, npairs))
+        return rc;
+    return sync(f);
+}
+
 int vpm_estr_direct(vpm_field f, const vpm_config* cfg)
 {
     VPM_FIELD_GUARD(f);

@@ -127,6 +127,8 @@ int launch_update_states_adjoint(vpm_field f, double* B, double a, double b, dou
 int launch_adj_add_bar(vpm_field f, double* B, const double* dX, const double* dG, const double* dS);
 int launch_adj_step_start(vpm_field f, double* B);
 int launch_zeta(vpm_field f, int kernel, int precision);
+int launch_estr_ranges(vpm_field f, int kernel, int precision, int transposed, const int64_t* tb,
+                       const int64_t* te, const int64_t* sb, const int64_t* se, int64_t npairs);
 
 double zeta0_of(int kernel);
 

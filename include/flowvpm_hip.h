@@ -114,6 +114,13 @@ int vpm_uj_direct_fields(vpm_field target, vpm_field source, const vpm_config* c
 int vpm_p2p_ranges(vpm_field f, const vpm_config* cfg,
                    const int64_t* tb, const int64_t* te,
                    const int64_t* sb, const int64_t* se, int64_t npairs);
+/* Estr_fmm! over a direct list (src/FLOWVPM_subfilterscale_models.jl:157-188):
+ * for k < npairs, the E_str pair (:16-41) of every source in [sb[k], se[k])
+ * on every target in [tb[k], te[k]), accumulated onto SFS.  As in the
+ * reference, static particles are not skipped and SFS is not reset. */
+int vpm_estr_ranges(vpm_field f, const vpm_config* cfg,
+                    const int64_t* tb, const int64_t* te,
+                    const int64_t* sb, const int64_t* se, int64_t npairs);
 
 /* ---- terms built from J ------------------------------------------------- */
 /* Estr_direct! (src/FLOWVPM_subfilterscale_models.jl:16-92): SFS += sum over

@@ -72,6 +72,7 @@ def lib():
         L.vpmo_update_particle_states_adjoint.argtypes = [_dp, _dp, C.c_int64, C.c_double, C.c_double,
                                                           C.c_double, C.c_double, C.c_double, C.c_int]
         L.vpmo_uj_adjoint_scale.argtypes = [_dp, C.c_int64, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int]
+        L.vpmo_estr_ranges.argtypes = [_dp, C.c_int64, C.c_int, C.c_int, _ip, _ip, _ip, _ip, C.c_int64]
         L.vpmo_zeta_direct.argtypes = [_dp, C.c_int64, C.c_int, C.c_int]
         L.vpmo_rk3_step_adjoint.restype = C.c_int
         L.vpmo_rk3_step_adjoint.argtypes = [_dp, C.c_int64, C.c_void_p, C.c_double, _dp]
@@ -157,6 +158,12 @@ def uj_direct(P, kernel, reset=True, reset_sfs=False, sfs=False, transposed=True
     lib().vpmo_uj_direct(_check(P), np_ if np_ is not None else P.shape[0], _kid(kernel),
                          int(reset), int(reset_sfs), int(sfs), int(transposed),
                          nthreads or nthreads_default())
+
+
+def estr_ranges(P, kernel, tb, te, sb, se, transposed=True):
+    """Estr_fmm! over a direct list, in place on the SFS rows."""
+    a = [np.ascontiguousarray(x, dtype=np.int64) for x in (tb, te, sb, se)]
+    lib().vpmo_estr_ranges(_check(P), NFIELDS, _kid(kernel), int(transposed), *a, len(a[0]))
 
 
 def zeta_direct(P, kernel, nthreads=None):

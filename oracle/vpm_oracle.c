@@ -23,6 +23,9 @@
  *                      inputs, plus the single-ring translation speed test
  *                      of test/runtests_singlevortexring.jl:22-23,143
  *   vpmo_uj_adjoint    central finite differences of vpmo_loss_local
+ *   vpmo_estr_ranges   equals vpmo_estr_direct over the pair list of the whole
+ *                      field minus its static rows, and the sum of its
+ *                      sub-range lists
  *   vpmo_zeta_direct   mpmath brute force; one particle gives Gamma zeta(0)/sigma^3
  *   vpmo_update_particle_states_adjoint / vpmo_rk3_step_adjoint
  *                      central finite differences of vpmo_update_particle_states
@@ -880,4 +883,24 @@ void vpmo_zeta_direct(double* P, int64_t np, int kernel, int nthreads)
         }
         for (int k = 0; k < 3; k++) pi[RJ + k] = acc[k];
     }
+}
+
+/* ---- Estr_fmm! over a direct list (subfilterscale_models.jl:157-188) ----
+ * For every (target range, source range) pair in list order, every source j
+ * and every target i of the ranges: Estr_direct(i, j) (:16-41) onto SFS_i.
+ * No static particle is skipped and SFS is not reset (unlike Estr_direct!). */
+void vpmo_estr_ranges(double* P, int64_t stride, int kernel, int transposed,
+                      const int64_t* tb, const int64_t* te,
+                      const int64_t* sb, const int64_t* se, int64_t npairs)
+{
+    for (int64_t k = 0; k < npairs; k++)
+        for (int64_t j = sb[k]; j < se[k]; j++) {
+            const double* sp = P + j * stride;
+            for (int64_t i = tb[k]; i < te[k]; i++) {
+                double* tp = P + i * stride;
+                const double dx = sp[RX] - tp[RX], dy = sp[RX + 1] - tp[RX + 1], dz = sp[RX + 2] - tp[RX + 2];
+                const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                estr_pair(kernel, transposed, tp, sp, r);
+            }
+        }
 }

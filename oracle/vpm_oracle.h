@@ -107,6 +107,12 @@ void vpmo_uj_adjoint_scale(const double* P, int64_t np, int kernel,
  * source), summed in long double; for finite differences. */
 double vpmo_loss_local(const double* P, int64_t np, int kernel,
                        const double* Ubar, const double* Jbar, int64_t k);
+/* Estr_fmm! over a direct list (subfilterscale_models.jl:157-188): the E_str
+ * pair (:16-41) for every (target range, source range), list order, onto
+ * SFS; no static skip, no reset.  0-based half-open ranges, rows of stride. */
+void vpmo_estr_ranges(double* P, int64_t stride, int kernel, int transposed,
+                      const int64_t* tb, const int64_t* te,
+                      const int64_t* sb, const int64_t* se, int64_t npairs);
 /* zeta_direct (viscous.jl:486-515): J[1:3] of every particle (static
  * included) <- sum over all particles j (self included) of
  * Gamma_j zeta(r/sigma_j)/sigma_j^3. */

@@ -1,0 +1,64 @@
+"""GPU parity of E_str over a direct list (vpm_estr_ranges, Estr_fmm! of
+src/FLOWVPM_subfilterscale_models.jl:157-188) against the oracle's
+restatement: leaf pairs with overlapping and repeated target ranges, static
+particles (not skipped by Estr_fmm!) and SFS accumulated onto a nonzero
+start.  Tolerances are those of the all-pairs E_str (test_gpu_uj.py): F64
+to FP64 rounding, F32C within ESTR_MAX_TOL / ESTR_RMS_TOL per particle
+(E_str sums cancel by factors of 10-100 on random J)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ESTR_MAX_TOL = 2e-4
+ESTR_RMS_TOL = 1e-5
+
+
+def _leaf_lists(n, nleaves, seed, density=0.4):
+    rng = np.random.default_rng(seed)
+    cuts = np.sort(np.concatenate([[0, n], rng.choice(np.arange(1, n), nleaves - 1, replace=False)]))
+    leaves = list(zip(cuts[:-1], cuts[1:]))
+    tb, te, sb, se = [], [], [], []
+    for (a, b) in leaves:
+        for (c, d) in leaves:
+            if rng.random() < density:
+                tb.append(a); te.append(b); sb.append(c); se.append(d)
+    return tb, te, sb, se
+
+
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans", "gaussian", "singular"])
+@pytest.mark.parametrize("precision", ["f32c", "f64"])
+def test_estr_ranges_vs_oracle(vpm, orc, kernel, precision):
+    n = 900
+    P = vpm.workloads.random_cloud(n, seed=16)
+    P[:, 6] *= 3.0
+    rng = np.random.default_rng(17)
+    P[:, 15:24] = rng.normal(0, 1, (n, 9))
+    P[:, 39:42] = rng.normal(0, 1e-3, (n, 3))
+    P[[4, 400], 42] = 1.0
+    tb, te, sb, se = _leaf_lists(n, 21, seed=18)
+    tb += [5, 100]; te += [300, 120]; sb += [0, 40]; se += [50, 41]
+    ref = P.copy()
+    orc.estr_ranges(ref, kernel, tb, te, sb, se)
+    A = P.copy()
+    vpm.api.estr_ranges(A, tb, te, sb, se, kernel=kernel, precision=precision)
+    assert np.array_equal(A[:, 0:39], P[:, 0:39])
+    st = vpm.metrics.err_stats(A[:, 39:42], ref[:, 39:42])
+    print("estr_ranges", kernel, precision, st, flush=True)
+    if precision == "f64" or kernel == "singular":
+        assert st["max"] < 1e-11, st
+    else:
+        assert st["max"] < ESTR_MAX_TOL and st["rms"] < ESTR_RMS_TOL, st
+
+
+def test_estr_ranges_whole_field_equals_estr_direct(vpm, orc):
+    """With no static particle, the single pair (all, all) is Estr_direct!."""
+    n = 700
+    P = vpm.workloads.random_cloud(n, seed=19)
+    P[:, 6] *= 3.0
+    P[:, 15:24] = np.random.default_rng(20).normal(0, 1, (n, 9))
+    ref = P.copy()
+    orc.estr_direct(ref, "gaussianerf")
+    A = P.copy()
+    vpm.api.estr_ranges(A, [0], [n], [0], [n], kernel="gaussianerf", precision="f64")
+    np.testing.assert_allclose(A[:, 39:42], ref[:, 39:42], rtol=1e-11, atol=1e-12 * np.abs(ref[:, 39:42]).max())

@@ -377,3 +377,24 @@ def test_zeta_direct_matches_mpmath(orc, vpm, kernel):
         ref = np.array([float(a) for a in acc])
         np.testing.assert_allclose(Q[i, 15:18], ref, rtol=1e-12, atol=1e-14 * max(1.0, np.abs(ref).max()))
     assert np.all(Q[:, 18:24] == 7.0)
+
+
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans"])
+def test_estr_ranges_matches_estr_direct(orc, vpm, kernel):
+    """Estr_fmm over the single pair (all, all) equals Estr_direct! when no
+    particle is static; split into sub-ranges it sums to the same."""
+    P = _cloud(vpm, n=40, seed=12)
+    P[:, 15:24] = np.random.default_rng(13).normal(size=(40, 9))
+    A = P.copy()
+    orc.estr_direct(A, kernel)
+    B = P.copy()
+    orc.estr_ranges(B, kernel, [0], [40], [0], [40])
+    np.testing.assert_allclose(B[:, 39:42], A[:, 39:42], rtol=1e-13, atol=1e-13 * np.abs(A[:, 39:42]).max())
+    C = P.copy()
+    orc.estr_ranges(C, kernel, [0, 0, 25, 25], [25, 25, 40, 40], [0, 17, 0, 17], [17, 40, 17, 40])
+    np.testing.assert_allclose(C[:, 39:42], A[:, 39:42], rtol=1e-12, atol=1e-12 * np.abs(A[:, 39:42]).max())
+    # static particles are not skipped: a static target still receives
+    D = P.copy()
+    D[3, 42] = 1.0
+    orc.estr_ranges(D, kernel, [0], [40], [0], [40])
+    np.testing.assert_allclose(D[:, 39:42], B[:, 39:42], rtol=0, atol=0)
