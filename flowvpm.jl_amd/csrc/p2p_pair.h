@@ -236,10 +236,18 @@ __device__ __forceinline__ void tile_f32c(int n, const float4* lpos, const float
 #pragma unroll
     for (int g = 0; g < kTile / 64; ++g) {
         uint64_t fb = far[g];
-        while (fb) {
-            const int s = g * 64 + __builtin_ctzll(fb);
-            fb &= fb - 1;
-            far_pair<K>(tx, ty, tz, lpos[s], lgs[s], acc);
+        if (fb == ~0ull) {
+            // the whole group is far (the common case in Morton order): a
+            // straight-line loop with immediate LDS offsets, no scalar
+            // bookkeeping per source
+#pragma unroll 16
+            for (int s = 0; s < 64; ++s) far_pair<K>(tx, ty, tz, lpos[g * 64 + s], lgs[g * 64 + s], acc);
+        } else {
+            while (fb) {
+                const int s = g * 64 + __builtin_ctzll(fb);
+                fb &= fb - 1;
+                far_pair<K>(tx, ty, tz, lpos[s], lgs[s], acc);
+            }
         }
     }
 #pragma unroll

@@ -284,6 +284,17 @@ __global__ __launch_bounds__(kBlock) void update_states_adjoint_kernel(
     for (int k = 0; k < 3; ++k) q[RG + k] = Gb[k];
 }
 
+// out[i][r] = P[i][row0 + r]: rows of the records, packed
+__global__ __launch_bounds__(kBlock) void gather_rows_kernel(const double* __restrict__ P, int64_t np, int row0,
+                                                             int nr, double* __restrict__ out)
+{
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= np * nr) return;
+    const int64_t i = k / nr;
+    const int r = (int)(k - i * nr);
+    out[k] = P[i * NF + row0 + r];
+}
+
 // B[i] += (dX, dGamma, dsigma) of the U/J adjoint
 __global__ __launch_bounds__(kBlock) void adj_add_bar_kernel(double* __restrict__ B, int64_t np,
                                                              const double* __restrict__ dX,
@@ -407,6 +418,15 @@ int launch_update_states_adjoint(vpm_field f, double* B, double a, double b, dou
     hipLaunchKernelGGL(update_states_adjoint_kernel, dim3(nb(f->np)), dim3(kBlock), 0, f->stream, f->P, B, f->np, a,
                        b, dt, fr, gr, transposed, Ub, Jb);
     VPM_CHECK_LAUNCH("update_states_adjoint");
+    return VPM_OK;
+}
+
+int launch_gather_rows(vpm_field f, int row0, int nrows, double* out)
+{
+    const int64_t n = f->np * nrows;
+    if (n == 0) return VPM_OK;
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(nb(n)), dim3(kBlock), 0, f->stream, f->P, f->np, row0, nrows, out);
+    VPM_CHECK_LAUNCH("gather_rows");
     return VPM_OK;
 }
 

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "vpm_internal.h"
 
@@ -220,7 +221,7 @@ int vpm_destroy(vpm_field f)
         (void)hipSetDevice(f->device);
         (void)hipStreamSynchronize(f->stream);
         DevBuf* bufs[] = {&f->pos, &f->gs, &f->cut, &f->g64, &f->part, &f->keys, &f->keys2, &f->perm,
-                          &f->iota, &f->sorttmp, &f->bbox, &f->flag, &f->adj, &f->rng, &f->bar};
+                          &f->iota, &f->sorttmp, &f->bbox, &f->flag, &f->adj, &f->rng, &f->bar, &f->xfer};
         for (DevBuf* b : bufs) release(*b);
         if (f->P) (void)hipFree(f->P);
         if (f->ev0) (void)hipEventDestroy(f->ev0);
@@ -261,11 +262,25 @@ int vpm_get_particles(vpm_field f, double* particles, int64_t np, int32_t row0, 
     if (row0 < 0 || row1 > NF || row0 >= row1) return fail(VPM_EINVAL, "bad row range");
     if (np == 0) return VPM_OK;
     if (!particles) return fail(VPM_EINVAL, "null output array");
-    const size_t pitch = sizeof(double) * NF;
-    hipError_t e = hipMemcpy2DAsync(particles + row0, pitch, f->P + row0, pitch, sizeof(double) * (row1 - row0),
-                                    (size_t)np, hipMemcpyDeviceToHost, f->stream);
-    if (e != hipSuccess) return hip_fail(e, "hipMemcpy2DAsync(D2H)");
-    return sync(f);
+    hipError_t e;
+    if (row0 == 0 && row1 == NF) {   // whole records: one contiguous copy
+        e = hipMemcpyAsync(particles, f->P, sizeof(double) * NF * (size_t)np, hipMemcpyDeviceToHost, f->stream);
+        if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(D2H)");
+        return sync(f);
+    }
+    // a row range: gather on the device, one contiguous copy, scatter on the
+    // host (a strided 2D copy of short rows is orders of magnitude slower)
+    const int nr = row1 - row0;
+    const size_t n = (size_t)np * (size_t)nr;
+    if (int rc = ensure(f->xfer, sizeof(double) * n)) return rc;
+    if (int rc = launch_gather_rows(f, row0, nr, (double*)f->xfer.p)) return rc;
+    std::vector<double> host(n);
+    e = hipMemcpyAsync(host.data(), f->xfer.p, sizeof(double) * n, hipMemcpyDeviceToHost, f->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(D2H)");
+    if (int rc = sync(f)) return rc;
+    for (int64_t i = 0; i < np; ++i)
+        std::memcpy(particles + i * NF + row0, host.data() + (size_t)i * nr, sizeof(double) * nr);
+    return VPM_OK;
 }
 
 int vpm_get_np(vpm_field f, int64_t* np)
@@ -439,7 +454,13 @@ static int rk3_step_locked(vpm_field f, vpm_step_config* s, double dt, int32_t r
     for (int st = 0; st < 3; ++st) {
         const double a = ab[st][0], b = ab[st][1];
         const bool first = (a == 1 || a == 0);                      // subfilterscale.jl:207,219
-        if (sfs && first)
+        // F32C with a two-level test filter: the test-minus-domain
+        // differences come from the difference kernels (sfs_delta.hip)
+        // instead of a second U/J + E_str pass at alpha sigma
+        const bool sfs_diff = sfs && first && cfg->precision == VPM_PREC_F32C &&
+                              cfg->kernel != VPM_KERNEL_SINGULAR && 1.0 - s->sfs_alpha <= 0.01 &&
+                              s->sfs_alpha < 1.0;
+        if (sfs && first && !sfs_diff)
             if ((rc = sfs_before_locked(f, cfg, s->sfs_alpha, s->sfs_minC, s->sfs_maxC))) return rc;
         const uint32_t fl = VPM_UJ_RESET | (sfs ? (VPM_UJ_RESET_SFS | VPM_UJ_SFS) : 0u);
         if ((rc = uj_direct_locked(f, cfg, fl))) return rc;        // :422
@@ -447,6 +468,11 @@ static int rk3_step_locked(vpm_field f, vpm_step_config* s, double dt, int32_t r
             hipError_t e = hipMemcpyAsync((char*)tape->rec.p + st * rec, f->P, rec, hipMemcpyDeviceToDevice,
                                           f->stream);
             if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(tape)");
+        }
+        if (sfs_diff) {
+            if (s->sfs_minC < 0 || s->sfs_maxC < 0 || s->sfs_minC > s->sfs_maxC)   // subfilterscale.jl:455-461
+                return fail(VPM_EINVAL, "Invalid C bounds");
+            if ((rc = launch_sfs_delta(f, cfg->kernel, cfg->transposed, s->sfs_alpha))) return rc;
         }
         if (sfs && first)
             if ((rc = sfs_after_locked(f, cfg, s->sfs_alpha, s->sfs_rlxf, s->sfs_minC, s->sfs_maxC,

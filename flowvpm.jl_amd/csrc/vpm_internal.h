@@ -62,6 +62,7 @@ struct vpm_field_s {
     vpm::DevBuf adj;                 // adjoint / E_str staging scratch
     vpm::DevBuf rng;                 // leaf-pair range lists
     vpm::DevBuf bar;                 // cotangent records of the step adjoint
+    vpm::DevBuf xfer;                // gathered rows for partial downloads
     bool profiling = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     vpm::Timing timing[4];
@@ -127,6 +128,8 @@ int launch_update_states_adjoint(vpm_field f, double* B, double a, double b, dou
 int launch_adj_add_bar(vpm_field f, double* B, const double* dX, const double* dG, const double* dS);
 int launch_adj_step_start(vpm_field f, double* B);
 int launch_zeta(vpm_field f, int kernel, int precision);
+int launch_sfs_delta(vpm_field f, int kernel, int transposed, double alpha);
+int launch_gather_rows(vpm_field f, int row0, int nrows, double* out);
 int launch_estr_ranges(vpm_field f, int kernel, int precision, int transposed, const int64_t* tb,
                        const int64_t* te, const int64_t* sb, const int64_t* se, int64_t npairs);
 

@@ -87,12 +87,12 @@ def test_rk3_dynamic_sfs_core_spreading_vs_oracle(vpm, orc, precision):
         assert C < 1e-8
     else:
         # The pseudo-3-level procedure takes M = S(alpha sigma) - S(sigma)
-        # with alpha = 0.999 (subfilterscale.jl:571-585): a difference of two
-        # U/J passes 1e-3 apart, which amplifies the FP32c U/J error by
-        # ~1/(1 - alpha) in C and, through C SFS sigma^3/zeta0, in dGamma/dt.
-        # Positions and core sizes stay at the FP32c level.
-        assert e["X"] < 1e-6 and e["sigma"] < 1e-6, e
-        assert e["Gamma"] < 2e-4, e
+        # with alpha = 0.999 (subfilterscale.jl:571-585).  The F32C step
+        # evaluates these differences directly (sfs_delta.hip) instead of
+        # subtracting two U/J passes 1e-3 apart, which would amplify the
+        # FP32 U/J error by ~1/(1 - alpha) (measured 2e-4 in Gamma that way).
+        assert e["X"] < 1e-7 and e["sigma"] < 1e-7, e
+        assert e["Gamma"] < 2e-6, e
 
 
 def test_per_particle_kernels_vs_oracle(vpm, orc):
