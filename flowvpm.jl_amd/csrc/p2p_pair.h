@@ -204,22 +204,22 @@ __device__ __forceinline__ WaveBox wave_box(float tx, float ty, float tz)
     return b;
 }
 
-// One LDS tile of n <= 256 sources against the lane's target (F32C).
+// One LDS tile of n <= 64 NG sources against the lane's target (F32C).
 // a = (s'_hi, far threshold), b = (Gamma', 1/sigma), l = s'_lo.
 // Each wave first splits the tile with one ballot per 64 sources: a source
 // is near when its distance to the wave's target box is below q_far sigma
 // (then some lane may be near; the test is conservative).  The far and the
 // near sources are then visited by iterating over the set bits of the two
 // masks (scalar loop control), so the far loop body is straight-line.
-template <int K>
+template <int K, int NG = kTile / 64>
 __device__ __forceinline__ void tile_f32c(int n, const float4* lpos, const float4* lgs, const float4* llo,
                                           float tx, float ty, float tz, float tlx, float tly, float tlz,
                                           const WaveBox& box, float (&acc)[kAccF32])
 {
     const int lane = threadIdx.x & 63;
-    uint64_t near[kTile / 64], far[kTile / 64];
+    uint64_t near[NG], far[NG];
 #pragma unroll
-    for (int g = 0; g < kTile / 64; ++g) {
+    for (int g = 0; g < NG; ++g) {
         const int s = g * 64 + lane;
         bool nr = false, ok = s < n;
         if (ok) {
@@ -234,7 +234,7 @@ __device__ __forceinline__ void tile_f32c(int n, const float4* lpos, const float
         far[g] = __ballot(ok && !nr);
     }
 #pragma unroll
-    for (int g = 0; g < kTile / 64; ++g) {
+    for (int g = 0; g < NG; ++g) {
         uint64_t fb = far[g];
         if (fb == ~0ull) {
             // the whole group is far (the common case in Morton order): a
@@ -251,7 +251,7 @@ __device__ __forceinline__ void tile_f32c(int n, const float4* lpos, const float
         }
     }
 #pragma unroll
-    for (int g = 0; g < kTile / 64; ++g) {
+    for (int g = 0; g < NG; ++g) {
         uint64_t nb = near[g];
         while (nb) {
             const int s = g * 64 + __builtin_ctzll(nb);

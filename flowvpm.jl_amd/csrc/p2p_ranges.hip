@@ -81,6 +81,81 @@ __global__ __launch_bounds__(kBlock) void ranges_f32c_kernel(const RangeItem* __
     }
 }
 
+// Wave-granular variant for small target intervals (FMM leaves): each wave
+// owns one item of <= 64 targets and streams that item's source ranges
+// through its own 64-row LDS slot, with no block-wide barrier, so a leaf of
+// 32-64 particles keeps half to all of the wave's lanes busy.  Within a wave
+// LDS accesses complete in issue order; the scheduling barriers keep the
+// compiler from moving the next chunk's stores above this chunk's loads.
+template <int K>
+__global__ __launch_bounds__(kBlock) void ranges_wave_f32c_kernel(const RangeItem* __restrict__ items,
+                                                                  int64_t nitems,
+                                                                  const int64_t* __restrict__ lsb,
+                                                                  const int64_t* __restrict__ lse,
+                                                                  const double* __restrict__ pos, int64_t ld,
+                                                                  const float4* __restrict__ gs,
+                                                                  const float* __restrict__ cut,
+                                                                  double* __restrict__ P)
+{
+    __shared__ float4 lpos_all[kBlock / 64][64];
+    __shared__ float4 lgs_all[kBlock / 64][64];
+    __shared__ float4 llo_all[kBlock / 64][64];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t w = (int64_t)blockIdx.x * (kBlock / 64) + wave;
+    if (w >= nitems) return;   // no block barrier below
+    float4* lpos = lpos_all[wave];
+    float4* lgs = lgs_all[wave];
+    float4* llo = llo_all[wave];
+    const RangeItem it = items[w];
+    const int64_t ib = it.t_begin;
+    const double ox = pos[ib], oy = pos[ld + ib], oz = pos[2 * ld + ib];
+    const int64_t t = it.t_begin + lane;
+    const bool active = t < it.t_end;
+    const int64_t tt = active ? t : ib;
+    float tx, ty, tz, tlx, tly, tlz;
+    split_rel(pos[tt], ox, tx, tlx);
+    split_rel(pos[ld + tt], oy, ty, tly);
+    split_rel(pos[2 * ld + tt], oz, tz, tlz);
+    const WaveBox box = wave_box(tx, ty, tz);
+    float sum[kAccF32], err[kAccF32];
+#pragma unroll
+    for (int c = 0; c < kAccF32; ++c) { sum[c] = 0.f; err[c] = 0.f; }
+    for (int64_t l = it.l_begin; l < it.l_end; ++l) {
+        const int64_t sb = lsb[l], se = lse[l];
+        for (int64_t base = sb; base < se; base += 64) {
+            const int n = (int)((se - base) < 64 ? (se - base) : 64);
+            __builtin_amdgcn_wave_barrier();
+            if (lane < n) {
+                const int64_t j = base + lane;
+                float4 p, q;
+                split_rel(pos[j], ox, p.x, q.x);
+                split_rel(pos[ld + j], oy, p.y, q.y);
+                split_rel(pos[2 * ld + j], oz, p.z, q.z);
+                p.w = cut[j];
+                q.w = 0.f;
+                lpos[lane] = p;
+                llo[lane] = q;
+                lgs[lane] = gs[j];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            float acc[kAccF32];
+#pragma unroll
+            for (int c = 0; c < kAccF32; ++c) acc[c] = 0.f;
+            tile_f32c<K, 1>(n, lpos, lgs, llo, tx, ty, tz, tlx, tly, tlz, box, acc);
+            fold_twosum(acc, sum, err);
+        }
+    }
+    if (active) {
+        double a[kAccF32];
+#pragma unroll
+        for (int c = 0; c < kAccF32; ++c) a[c] = (double)sum[c] + (double)err[c];
+        add_f32c_to_record(P + t * NF, a);
+    }
+}
+
 #pragma clang fp contract(off)
 template <int K>
 __global__ __launch_bounds__(kBlock) void ranges_f64_kernel(const RangeItem* __restrict__ items,
@@ -134,9 +209,46 @@ struct RangePlan {
     std::vector<int64_t> lsb, lse;
 };
 
-static void plan_ranges(const int64_t* tb, const int64_t* te, const int64_t* sb, const int64_t* se, int64_t npairs,
-                        RangePlan& plan)
+// Fast plan for the usual FMM direct list: every target range is a leaf, so
+// ranges are identical or disjoint, and the pairs of one target range are
+// listed together.  Then the list itself is the per-interval source list.
+// Returns false (plan untouched) when the list is not of that form.
+static bool plan_ranges_leaves(const int64_t* tb, const int64_t* te, const int64_t* sb, const int64_t* se,
+                               int64_t npairs, int64_t chunk, RangePlan& plan)
 {
+    struct Run { int64_t tb, te, l0, l1; };
+    std::vector<Run> runs;
+    for (int64_t k = 0; k < npairs;) {
+        int64_t m = k + 1;
+        while (m < npairs && tb[m] == tb[k] && te[m] == te[k]) ++m;
+        runs.push_back({tb[k], te[k], k, m});
+        k = m;
+    }
+    std::vector<size_t> order(runs.size());
+    for (size_t r = 0; r < runs.size(); ++r) order[r] = r;
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return runs[a].tb < runs[b].tb; });
+    for (size_t r = 1; r < order.size(); ++r) {
+        const Run& p = runs[order[r - 1]];
+        const Run& q = runs[order[r]];
+        if (q.tb < p.te || (q.tb == p.tb && q.te > q.tb && p.te > p.tb)) return false;   // overlap or split run
+    }
+    plan.lsb.assign(sb, sb + npairs);
+    plan.lse.assign(se, se + npairs);
+    for (const Run& r : runs) {
+        if (r.te <= r.tb) continue;
+        for (int64_t t = r.tb; t < r.te; t += chunk)
+            plan.items.push_back({t, std::min<int64_t>(t + chunk, r.te), r.l0, r.l1});
+    }
+    return true;
+}
+
+static void plan_ranges(const int64_t* tb, const int64_t* te, const int64_t* sb, const int64_t* se, int64_t npairs,
+                        RangePlan& plan, int64_t chunk = kBlock)
+{
+    if (plan_ranges_leaves(tb, te, sb, se, npairs, chunk, plan)) return;
+    plan.items.clear();
+    plan.lsb.clear();
+    plan.lse.clear();
     std::vector<int64_t> cuts;
     cuts.reserve(2 * (size_t)npairs);
     for (int64_t k = 0; k < npairs; ++k)
@@ -168,8 +280,8 @@ static void plan_ranges(const int64_t* tb, const int64_t* te, const int64_t* sb,
     }
     for (size_t m = 0; m < nint; ++m) {
         if (count[m + 1] == count[m]) continue;
-        for (int64_t t = cuts[m]; t < cuts[m + 1]; t += kBlock)
-            plan.items.push_back({t, std::min<int64_t>(t + kBlock, cuts[m + 1]), count[m], count[m + 1]});
+        for (int64_t t = cuts[m]; t < cuts[m + 1]; t += chunk)
+            plan.items.push_back({t, std::min<int64_t>(t + chunk, cuts[m + 1]), count[m], count[m + 1]});
     }
 }
 
@@ -193,8 +305,17 @@ static int upload_plan(vpm_field f, const RangePlan& plan, RangeItem*& d_items, 
 int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb, const int64_t* te,
                       const int64_t* sb, const int64_t* se, int64_t npairs)
 {
+    // F32C over short target intervals (FMM leaves, mean <= 128 targets):
+    // one wave per <= 64 targets; otherwise one block per <= 256 targets
+    bool wave = false;
+    if (precision != VPM_PREC_F64 && npairs > 0) {
+        int64_t tot = 0, cnt = 0;
+        for (int64_t k = 0; k < npairs; ++k)
+            if (te[k] > tb[k] && (k == 0 || tb[k] != tb[k - 1] || te[k] != te[k - 1])) { tot += te[k] - tb[k]; ++cnt; }
+        wave = cnt > 0 && tot <= 128 * cnt;
+    }
     RangePlan plan;
-    plan_ranges(tb, te, sb, se, npairs, plan);
+    plan_ranges(tb, te, sb, se, npairs, plan, wave ? 64 : kBlock);
     if (plan.items.empty()) return VPM_OK;
     Staged st;
     int rc;
@@ -210,6 +331,15 @@ int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb,
 #define VPM_R64(KID) case KID: hipLaunchKernelGGL(ranges_f64_kernel<KID>, grid, dim3(kBlock), 0, f->stream, d_items, d_lsb, d_lse, st.pos, st.ld, st.g64, f->P); break;
             VPM_R64(VPM_KERNEL_SINGULAR) VPM_R64(VPM_KERNEL_GAUSSIAN) VPM_R64(VPM_KERNEL_GAUSSIANERF) VPM_R64(VPM_KERNEL_WINCKELMANS)
 #undef VPM_R64
+        default: return fail(VPM_EINVAL, "unknown kernel id");
+        }
+    } else if (wave) {
+        const int64_t ni = (int64_t)plan.items.size();
+        const dim3 wgrid((unsigned)((ni + kBlock / 64 - 1) / (kBlock / 64)));
+        switch (kernel) {
+#define VPM_RW(KID) case KID: hipLaunchKernelGGL(ranges_wave_f32c_kernel<KID>, wgrid, dim3(kBlock), 0, f->stream, d_items, ni, d_lsb, d_lse, st.pos, st.ld, st.gs, st.cut, f->P); break;
+            VPM_RW(VPM_KERNEL_SINGULAR) VPM_RW(VPM_KERNEL_GAUSSIAN) VPM_RW(VPM_KERNEL_GAUSSIANERF) VPM_RW(VPM_KERNEL_WINCKELMANS)
+#undef VPM_RW
         default: return fail(VPM_EINVAL, "unknown kernel id");
         }
     } else {

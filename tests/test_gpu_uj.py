@@ -265,3 +265,32 @@ def test_cfg1_full_size_sample(vpm, orc):
             tol = F64_TOL if prec == "f64" else F32_TOL
             assert eu["max"] < tol and ej["max"] < tol
             assert np.all(np.isfinite(A[:, 9:24]))
+
+
+@pytest.mark.parametrize("leaf", [16, 64, 200])
+@pytest.mark.parametrize("precision", ["f32c", "f64"])
+def test_p2p_ranges_leaf_lists(vpm, orc, leaf, precision):
+    """FMM-style direct lists: target ranges are leaves (identical or
+    disjoint), each target leaf's pairs listed together (the linear-time
+    plan), then the same pairs with one target leaf split into two runs (the
+    general interval plan).  Leaves of 16 and 64 take the wave-per-leaf
+    kernel, leaves of 200 the block kernel."""
+    n = 1500
+    P = vpm.workloads.random_cloud(n, seed=26)
+    P[:, 6] *= 3.0
+    starts = np.arange(0, n, leaf)
+    ends = np.minimum(starts + leaf, n)
+    C = np.array([P[a:b, 0:3].mean(0) for a, b in zip(starts, ends)])
+    tb, te, sb, se = [], [], [], []
+    for i in range(len(starts)):
+        for j in range(len(starts)):
+            if np.linalg.norm(C[i] - C[j]) < 0.35:
+                tb.append(starts[i]); te.append(ends[i]); sb.append(starts[j]); se.append(ends[j])
+    for lists in ((tb, te, sb, se), (tb + tb[:1], te + te[:1], sb + sb[:1], se + se[:1])):
+        ref = P.copy()
+        orc.direct_ranges(ref, "gaussianerf", *lists)
+        A = P.copy()
+        vpm.p2p_ranges(A, *lists, kernel="gaussianerf", precision=precision)
+        eu, ej = _uj_err(vpm, A, ref)
+        tol = F64_TOL if precision == "f64" else F32_TOL
+        assert eu["max"] < tol and ej["max"] < tol, (leaf, eu, ej)
