@@ -105,6 +105,38 @@ def rollout_adjoint(vpm, P, kernel, nsteps=3, dt=0.005):
     return out
 
 
+def leaf_list(vpm, P, kernel, allpairs_rate, leaf=64, reps=3):
+    """The near-field service of an FMM (FastMultipole's direct list through
+    fmm.direct!, src/FLOWVPM_fmm.jl:102-168) on the jet: leaves of 64
+    Morton-consecutive particles, direct pairs within 3 median leaf radii,
+    one vpm_p2p_ranges call per evaluation."""
+    import ctypes as C
+    api = vpm.api
+    Q = vpm.workloads.morton_order(P)
+    n = Q.shape[0]
+    tb, te, sb, se = vpm.workloads.leaf_direct_list(Q, leaf)
+    pairs = float(np.sum((te - tb) * (se - sb)))
+    f = api.Field(n, 0)
+    f.set_particles(Q)
+    cfg = api.make_config(kernel, "f32c", True)
+    arrs = [np.ascontiguousarray(a, dtype=np.int64) for a in (tb, te, sb, se)]
+    ptr = [a.ctypes.data_as(vpm._lib._i64p) for a in arrs]
+    L = vpm._lib.lib()
+    vpm._lib.check(L.vpm_p2p_ranges(f.handle, C.byref(cfg), *ptr, len(tb)))
+    f.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        vpm._lib.check(L.vpm_p2p_ranges(f.handle, C.byref(cfg), *ptr, len(tb)))
+    wall = (time.perf_counter() - t0) / reps
+    ms, cnt = f.timing(0)
+    f.close()
+    out = {"N": n, "leaf": leaf, "list_pairs": len(tb), "particle_pairs": pairs, "sources_per_target": pairs / n,
+           "call_ms": 1e3 * wall, "kernel_ms": ms / cnt, "pairs_per_s_kernel": pairs / (ms / cnt / 1e3),
+           "pairs_per_s_call": pairs / wall, "fraction_of_allpairs_rate": pairs / (ms / cnt / 1e3) / allpairs_rate}
+    log("bench: leaf list %s" % out)
+    return out
+
+
 def cfg1_cloud(vpm, n=65536, reps=5):
     """Random cloud N = 65,536 (X ~ U[0,1]^3, Gamma ~ N(0, 1e-3^2), sigma ~
     U[0.02, 0.04], seed 1): one U+J evaluation per kernel in FP32c and FP64 on
@@ -184,6 +216,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-rollout", action="store_true",
                     help="skip forward+adjoint through 3 RK3 steps on the jet")
+    ap.add_argument("--no-leaf", action="store_true",
+                    help="skip the FMM leaf-pair list (vpm_p2p_ranges) on the jet")
     ap.add_argument("--no-cfg1", action="store_true",
                     help="skip the N=65,536 random-cloud accuracy/throughput report")
     ap.add_argument("--no-cfg2", action="store_true",
@@ -287,6 +321,9 @@ def main():
 
     if not args.no_rollout:
         res["rollout_adjoint"] = rollout_adjoint(vpm, P, kernel)
+
+    if not args.no_leaf:
+        res["fmm_leaf_list"] = leaf_list(vpm, P, kernel, value)
 
     if not args.no_cfg1:
         res["cfg1_cloud"] = cfg1_cloud(vpm)

@@ -153,3 +153,45 @@ def ring_centroid(P):
     """|Gamma|-weighted centroid (vortexrings_functions.jl:275-316)."""
     w = np.linalg.norm(P[:, 3:6], axis=1)
     return (P[:, 0:3] * w[:, None]).sum(axis=0) / w.sum()
+
+
+def morton_order(P):
+    """Particles reordered along a 21-bit-per-axis Z-order curve (the order
+    an octree code stores its leaves in)."""
+    X = P[:, 0:3]
+    lo, hi = X.min(0), X.max(0)
+    ext = max(float((hi - lo).max()), 1e-300)
+    u = ((X - lo) / ext * (2 ** 21 - 1)).astype(np.uint64)
+
+    def spread(v):
+        v = v & np.uint64(0x1fffff)
+        for s, m in ((32, 0x1f00000000ffff), (16, 0x1f0000ff0000ff), (8, 0x100f00f00f00f00f),
+                     (4, 0x10c30c30c30c30c3), (2, 0x1249249249249249)):
+            v = (v | (v << np.uint64(s))) & np.uint64(m)
+        return v
+
+    key = spread(u[:, 0]) | (spread(u[:, 1]) << np.uint64(1)) | (spread(u[:, 2]) << np.uint64(2))
+    return P[np.argsort(key, kind="stable")].copy()
+
+
+def leaf_direct_list(P, leaf=64, radius=3.0):
+    """An FMM-style direct list over P (already in Morton order): leaves are
+    runs of `leaf` consecutive particles; leaf j is a direct source of leaf i
+    when their centres are within `radius` times the median leaf radius (i = j
+    included).  Returns 0-based half-open (tb, te, sb, se), grouped by target
+    leaf."""
+    from scipy.spatial import cKDTree
+    n = P.shape[0]
+    starts = np.arange(0, n, leaf)
+    ends = np.minimum(starts + leaf, n)
+    C = np.add.reduceat(P[:, 0:3], starts, axis=0) / (ends - starts)[:, None]
+    owner = np.repeat(np.arange(len(starts)), ends - starts)
+    R = np.zeros(len(starts))
+    np.maximum.at(R, owner, np.linalg.norm(P[:, 0:3] - C[owner], axis=1))
+    pairs = cKDTree(C).query_pairs(radius * float(np.median(R)), output_type="ndarray")
+    nl = len(starts)
+    ii = np.concatenate([pairs[:, 0], pairs[:, 1], np.arange(nl)])
+    jj = np.concatenate([pairs[:, 1], pairs[:, 0], np.arange(nl)])
+    o = np.lexsort((jj, ii))
+    ii, jj = ii[o], jj[o]
+    return starts[ii], ends[ii], starts[jj], ends[jj]

@@ -140,3 +140,22 @@ def test_stretching_helper_matches_oracle_update(vpm, orc):
         orc.update_particle_states(Q, 0.0, 1.0, 1.0, np.zeros(3), 0.0, 0.0, 1.0, transposed)
         np.testing.assert_allclose(vpm.metrics.stretching(P[:, 15:24], P[:, 3:6], transposed), Q[:, 30:33],
                                    rtol=1e-14, atol=1e-14)
+
+
+def test_leaf_direct_list_shape(vpm):
+    """morton_order is a permutation; the leaf list covers every leaf with
+    itself, is symmetric and is grouped by target leaf (the linear-time plan
+    of vpm_p2p_ranges relies on that grouping)."""
+    P = vpm.workloads.random_cloud(3000, seed=4)
+    Q = vpm.workloads.morton_order(P)
+    assert np.array_equal(np.sort(Q[:, 0]), np.sort(P[:, 0]))
+    tb, te, sb, se = vpm.workloads.leaf_direct_list(Q, leaf=64)
+    pairs = set(zip(tb.tolist(), sb.tolist()))
+    for a in range(0, 3000, 64):
+        assert (a, a) in pairs
+    assert all((s, t) in pairs for (t, s) in pairs)
+    first = {}
+    for k, t in enumerate(tb.tolist()):
+        first.setdefault(t, k)
+    assert all(tb[first[t]:first[t] + int(np.sum(tb == t))].tolist() == [t] * int(np.sum(tb == t)) for t in first)
+    assert np.all(te - tb <= 64) and np.all(te > tb)
