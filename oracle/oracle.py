@@ -73,6 +73,9 @@ def lib():
                                                           C.c_double, C.c_double, C.c_double, C.c_int]
         L.vpmo_uj_adjoint_scale.argtypes = [_dp, C.c_int64, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int]
         L.vpmo_estr_ranges.argtypes = [_dp, C.c_int64, C.c_int, C.c_int, _ip, _ip, _ip, _ip, C.c_int64]
+        L.vpmo_core_spreading_reset.restype = C.c_int
+        L.vpmo_core_spreading_reset.argtypes = [_dp, C.c_int64, C.c_int, C.c_double, C.c_int, C.c_double,
+                                                C.c_int, C.c_int, C.POINTER(C.c_int), _dp]
         L.vpmo_zeta_direct.argtypes = [_dp, C.c_int64, C.c_int, C.c_int]
         L.vpmo_rk3_step_adjoint.restype = C.c_int
         L.vpmo_rk3_step_adjoint.argtypes = [_dp, C.c_int64, C.c_void_p, C.c_double, _dp]
@@ -164,6 +167,16 @@ def estr_ranges(P, kernel, tb, te, sb, se, transposed=True):
     """Estr_fmm! over a direct list, in place on the SFS rows."""
     a = [np.ascontiguousarray(x, dtype=np.int64) for x in (tb, te, sb, se)]
     lib().vpmo_estr_ranges(_check(P), NFIELDS, _kid(kernel), int(transposed), *a, len(a[0]))
+
+
+def core_spreading_reset(P, kernel, sgm0, itmax=15, tol=1e-3, iterror=True, nthreads=None):
+    """Core-spreading reset + RBF conjugate gradient, in place; returns
+    (status, iterations, residuals)."""
+    it = C.c_int(0)
+    res = np.zeros(3)
+    rc = lib().vpmo_core_spreading_reset(_check(P), P.shape[0], _kid(kernel), sgm0, int(itmax), tol,
+                                         int(iterror), nthreads or nthreads_default(), C.byref(it), res)
+    return rc, it.value, res
 
 
 def zeta_direct(P, kernel, nthreads=None):

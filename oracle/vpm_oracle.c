@@ -26,6 +26,9 @@
  *   vpmo_estr_ranges   equals vpmo_estr_direct over the pair list of the whole
  *                      field minus its static rows, and the sum of its
  *                      sub-range lists
+ *   vpmo_core_spreading_reset  converged Gamma reproduces the target
+ *                      vorticity; iterated to tight tolerance it equals a dense
+ *                      solve of the zeta system
  *   vpmo_zeta_direct   mpmath brute force; one particle gives Gamma zeta(0)/sigma^3
  *   vpmo_update_particle_states_adjoint / vpmo_rk3_step_adjoint
  *                      central finite differences of vpmo_update_particle_states
@@ -903,4 +906,95 @@ void vpmo_estr_ranges(double* P, int64_t stride, int kernel, int transposed,
                 estr_pair(kernel, transposed, tp, sp, r);
             }
         }
+}
+
+/* ---- core-spreading reset with RBF interpolation ----
+ * viscous.jl:202-220 (target vorticity from the zeta field, sigma <- sgm0)
+ * and rbf_conjugategradient (:314-478), with zeta = zeta_direct.  Vectors
+ * and sums run over the non-static particles (iterator(pfield),
+ * particlefield.jl:368-374); zeta_direct over all of them.  alpha is 0 when
+ * a component has converged (Julia's false is a strong zero).  Returns 0, or
+ * 1 when itmax is reached unconverged with iterror set; *iters and res[3]
+ * (sqrt(rr/rr0)) report the iteration count and residuals. */
+int vpmo_core_spreading_reset(double* P, int64_t np, int kernel, double sgm0, int itmax, double tol,
+                              int iterror, int nthreads, int* iters, double res[3])
+{
+    const double eps2 = 2 * 2.220446049250313e-16;
+    vpmo_zeta_direct(P, np, kernel, nthreads);                      /* :204 */
+    for (int64_t n = 0; n < np; n++) {                              /* :206-214 */
+        double* p = P + n * VPMO_NFIELDS;
+        if (is_static(p)) continue;
+        for (int i = 0; i < 3; i++) p[RM + 6 + i] = p[RJ + i];
+        p[RSIG] = sgm0;
+    }
+    double rr0[3] = {0, 0, 0}, rrs[3], prev[3], pAp[3], alpha[3], beta[3];
+    int flags[3];
+    for (int64_t n = 0; n < np; n++) {                              /* :345-352 */
+        double* p = P + n * VPMO_NFIELDS;
+        if (is_static(p)) continue;
+        for (int i = 0; i < 3; i++) {
+            p[RM + i] = p[RM + 6 + i] * p[RVOL];
+            p[RG + i] = p[RM + i];
+        }
+    }
+    vpmo_zeta_direct(P, np, kernel, nthreads);                      /* :355 */
+    for (int64_t n = 0; n < np; n++) {                              /* :357-368 */
+        double* p = P + n * VPMO_NFIELDS;
+        if (is_static(p)) continue;
+        for (int i = 0; i < 3; i++) {
+            p[RM + 3 + i] = p[RM + 6 + i] - p[RJ + i];
+            p[RG + i] = p[RM + 3 + i];
+            rr0[i] += p[RM + 3 + i] * p[RM + 3 + i];
+        }
+    }
+    for (int i = 0; i < 3; i++) {
+        rrs[i] = rr0[i];
+        flags[i] = sqrt(rr0[i]) > tol || sqrt(rrs[i] / rr0[i]) > tol;
+    }
+    int it = 0, status = 0;
+    for (it = 1; it <= itmax; it++) {                               /* :376-447 */
+        if (!(flags[0] || flags[1] || flags[2])) { it--; break; }
+        vpmo_zeta_direct(P, np, kernel, nthreads);
+        for (int i = 0; i < 3; i++) pAp[i] = 0;
+        for (int64_t n = 0; n < np; n++) {
+            const double* p = P + n * VPMO_NFIELDS;
+            if (is_static(p)) continue;
+            for (int i = 0; i < 3; i++) pAp[i] += p[RG + i] * p[RJ + i];
+        }
+        for (int i = 0; i < 3; i++) {
+            alpha[i] = flags[i] ? rrs[i] / pAp[i] : 0.0;
+            prev[i] = rrs[i];
+            rrs[i] = 0;
+        }
+        for (int64_t n = 0; n < np; n++) {
+            double* p = P + n * VPMO_NFIELDS;
+            if (is_static(p)) continue;
+            for (int i = 0; i < 3; i++) {
+                p[RM + i] += alpha[i] * p[RG + i];
+                p[RM + i + 3] -= alpha[i] * p[RJ + i];
+                rrs[i] += p[RM + i + 3] * p[RM + i + 3];
+            }
+        }
+        for (int i = 0; i < 3; i++) {
+            beta[i] = rrs[i] / prev[i];
+            if (fabs(prev[i]) <= eps2) beta[i] = 1;
+        }
+        for (int64_t n = 0; n < np; n++) {
+            double* p = P + n * VPMO_NFIELDS;
+            if (is_static(p)) continue;
+            for (int i = 0; i < 3; i++) p[RG + i] = p[RM + i + 3] + beta[i] * p[RG + i];
+        }
+        for (int i = 0; i < 3; i++)
+            flags[i] = flags[i] && (fabs(rr0[i]) <= eps2 ? 0 : sqrt(rrs[i] / rr0[i]) > tol);
+        if (it == itmax && (flags[0] || flags[1] || flags[2]) && iterror) status = 1;
+    }
+    if (it > itmax) it = itmax;
+    for (int64_t n = 0; n < np; n++) {                              /* :450-455 */
+        double* p = P + n * VPMO_NFIELDS;
+        if (is_static(p)) continue;
+        for (int i = 0; i < 3; i++) p[RG + i] = p[RM + i];
+    }
+    if (iters) *iters = it;
+    if (res) for (int i = 0; i < 3; i++) res[i] = rr0[i] > 0 ? sqrt(rrs[i] / rr0[i]) : 0.0;
+    return status;
 }

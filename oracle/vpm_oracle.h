@@ -113,6 +113,10 @@ double vpmo_loss_local(const double* P, int64_t np, int kernel,
 void vpmo_estr_ranges(double* P, int64_t stride, int kernel, int transposed,
                       const int64_t* tb, const int64_t* te,
                       const int64_t* sb, const int64_t* se, int64_t npairs);
+/* Core-spreading reset (viscous.jl:202-220) with rbf_conjugategradient
+ * (:314-478) and zeta_direct; returns 1 when unconverged with iterror. */
+int vpmo_core_spreading_reset(double* P, int64_t np, int kernel, double sgm0, int itmax, double tol,
+                              int iterror, int nthreads, int* iters, double res[3]);
 /* zeta_direct (viscous.jl:486-515): J[1:3] of every particle (static
  * included) <- sum over all particles j (self included) of
  * Gamma_j zeta(r/sigma_j)/sigma_j^3. */

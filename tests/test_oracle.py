@@ -398,3 +398,73 @@ def test_estr_ranges_matches_estr_direct(orc, vpm, kernel):
     D[3, 42] = 1.0
     orc.estr_ranges(D, kernel, [0], [40], [0], [40])
     np.testing.assert_allclose(D[:, 39:42], B[:, 39:42], rtol=0, atol=0)
+
+
+def _lattice(vpm, m=4, h=0.1, sig=1.3, seed=3):
+    rng = np.random.default_rng(seed)
+    g = np.arange(m) * h
+    X = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3) + rng.uniform(-0.1, 0.1, (m ** 3, 3)) * h
+    G = rng.normal(0, 1e-3, (m ** 3, 3))
+    P = vpm.fields.make_particles(X, G, np.full(m ** 3, sig * h))
+    P[:, 7] = h ** 3                           # vol
+    return P
+
+
+@pytest.mark.parametrize("kernel", ["gaussianerf", "gaussian", "winckelmans"])
+def test_core_spreading_reset_equals_dense_solve(orc, vpm, kernel):
+    """Iterated to a tight tolerance, the RBF conjugate gradient gives the
+    Gamma that solves the zeta system A Gamma = omega_target (dense solve),
+    with A_ij = zeta(r_ij/sgm0)/sgm0^3 (self included).  The tolerance stays
+    at 1e-7: below an absolute r.r of 2 eps the reference's CG sets beta = 1
+    (viscous.jl:411-416) and stalls, which the oracle restates."""
+    P = _lattice(vpm)
+    n = P.shape[0]
+    Q = P.copy()
+    orc.zeta_direct(Q, kernel)
+    omega = Q[:, 15:18].copy()
+    sg0 = 0.07          # 0.7 h: a well-conditioned zeta matrix, so CG reaches 1e-12
+    R = P.copy()
+    rc, its, res = orc.core_spreading_reset(R, kernel, sg0, itmax=500, tol=1e-7, iterror=True)
+    assert rc == 0 and its < 500, (rc, its, res)
+    assert np.all(R[:, 6] == sg0)
+    X = P[:, 0:3]
+    r = np.linalg.norm(X[:, None, :] - X[None, :, :], axis=2) / sg0
+    if kernel == "gaussianerf":
+        z = np.exp(-r * r / 2) / (2 * np.pi) ** 1.5
+    elif kernel == "gaussian":
+        z = 3 / (4 * np.pi) * np.exp(-r ** 3)
+    else:
+        z = 7.5 / (4 * np.pi) / (r * r + 1) ** 3.5
+    A = z / sg0 ** 3
+    x = np.linalg.solve(A, omega)
+    np.testing.assert_allclose(R[:, 3:6], x, rtol=1e-5, atol=1e-5 * np.abs(x).max())
+
+
+def test_core_spreading_reset_residual_and_static(orc, vpm):
+    """At the default tolerance (1e-3) the new strengths reproduce the target
+    vorticity to that relative residual per component.  A static particle
+    keeps Gamma and sigma (its zeta contribution enters every A p of the
+    reference's CG as a constant, so with statics only the recursive residual
+    is controlled)."""
+    P = _lattice(vpm)
+    Q = P.copy()
+    orc.zeta_direct(Q, "gaussianerf")
+    omega = Q[:, 15:18].copy()
+    R = P.copy()
+    rc, its, res = orc.core_spreading_reset(R, "gaussianerf", 0.07, itmax=50)
+    assert rc == 0 and np.all(res <= 1e-3), (rc, its, res)
+    S = R.copy()
+    orc.zeta_direct(S, "gaussianerf")
+    for c in range(3):
+        e = np.linalg.norm(S[:, 15 + c] - omega[:, c]) / np.linalg.norm(omega[:, c])
+        assert e <= 1.5e-3, (c, e)
+    P2 = P.copy()
+    P2[10, 42] = 1.0
+    R2 = P2.copy()
+    rc, its, res = orc.core_spreading_reset(R2, "gaussianerf", 0.07, itmax=50)
+    assert rc == 0 and np.all(res <= 1e-3)
+    assert np.array_equal(R2[10, 3:7], P2[10, 3:7])
+    # with iterror, an unconverged solve reports status 1
+    R3 = P.copy()
+    rc, its, res = orc.core_spreading_reset(R3, "gaussianerf", 0.07, itmax=2)
+    assert rc == 1 and its == 2
