@@ -204,6 +204,41 @@ def cfg2_rollout(vpm, nsteps=10, dt=0.01):
     return res
 
 
+def core_spreading_reset(vpm, P, kernel, itmax=15, tol=1e-3):
+    """The CoreSpreading reset on the jet (viscous.jl:202-220, 314-478):
+    zeta field -> target, sigma <- sgm0 (the jet's own core size), then the
+    RBF conjugate gradient with CoreSpreading's defaults (itmax 15, tol
+    1e-3; the iteration count and residuals are reported, not required).
+    Also one zeta_direct evaluation alone (F32C, kernel time)."""
+    api = vpm.api
+    n = P.shape[0]
+    g = api.Field(n, 0)
+    g.set_particles(P)
+    g.zeta_direct(kernel, "f32c")                      # warmup (sort, staging)
+    g.profile(True)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.zeta_direct(kernel, "f32c")
+    tz = time.perf_counter() - t0
+    kz, cz = g.timing(4)
+    g.set_particles(P)
+    sgm0 = float(np.median(P[:, 6]))
+    g.profile(True)
+    t0 = time.perf_counter()
+    its, res = g.core_spreading_reset(sgm0, kernel, "f32c", itmax=itmax, tol=tol, iterror=False)
+    tr = time.perf_counter() - t0
+    kr, cr = g.timing(4)
+    g.profile(False)
+    g.close()
+    out = {"N": n, "zeta_kernel_ms": kz / max(cz, 1), "zeta_call_ms": 1e3 * tz,
+           "zeta_pairs_per_s": float(n) * n / (kz / 1e3 / max(cz, 1)),
+           "reset_ms": 1e3 * tr, "reset_iterations": its, "reset_residuals": [float(x) for x in res],
+           "reset_zeta_evaluations": cr, "reset_zeta_kernel_ms": kr, "sgm0": sgm0,
+           "itmax": itmax, "tol": tol}
+    log("bench: core-spreading reset %s" % out)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,6 +253,8 @@ def main():
                     help="skip forward+adjoint through 3 RK3 steps on the jet")
     ap.add_argument("--no-leaf", action="store_true",
                     help="skip the FMM leaf-pair list (vpm_p2p_ranges) on the jet")
+    ap.add_argument("--no-reset", action="store_true",
+                    help="skip zeta_direct and the CoreSpreading RBF reset on the jet")
     ap.add_argument("--no-cfg1", action="store_true",
                     help="skip the N=65,536 random-cloud accuracy/throughput report")
     ap.add_argument("--no-cfg2", action="store_true",
@@ -324,6 +361,9 @@ def main():
 
     if not args.no_leaf:
         res["fmm_leaf_list"] = leaf_list(vpm, P, kernel, value)
+
+    if not args.no_reset:
+        res["core_spreading_reset"] = core_spreading_reset(vpm, P, kernel)
 
     if not args.no_cfg1:
         res["cfg1_cloud"] = cfg1_cloud(vpm)

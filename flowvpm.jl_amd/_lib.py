@@ -11,9 +11,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libflowvpm_hip.so")
 HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "flowvpm_hip.h")
 
-VPM_OK, VPM_EINVAL, VPM_ERANGE, VPM_ENOMEM, VPM_EDEVICE, VPM_ENAN, VPM_ERESET = 0, -1, -2, -3, -4, -5, -6
+VPM_OK, VPM_EINVAL, VPM_ERANGE, VPM_ENOMEM, VPM_EDEVICE, VPM_ENAN, VPM_ERESET, VPM_ECONVERGE = \
+    0, -1, -2, -3, -4, -5, -6, -7
 STATUS_NAMES = {0: "VPM_OK", -1: "VPM_EINVAL", -2: "VPM_ERANGE", -3: "VPM_ENOMEM",
-                -4: "VPM_EDEVICE", -5: "VPM_ENAN", -6: "VPM_ERESET"}
+                -4: "VPM_EDEVICE", -5: "VPM_ENAN", -6: "VPM_ERESET", -7: "VPM_ECONVERGE"}
 KERNELS = {"singular": 0, "gaussian": 1, "gaussianerf": 2, "winckelmans": 3}
 PRECISIONS = {"f32c": 0, "f64": 1}
 UJ_RESET, UJ_RESET_SFS, UJ_SFS, UJ_NO_SORT = 1, 2, 4, 8
@@ -37,7 +38,8 @@ class StepConfig(C.Structure):
                 ("sfs_alpha", C.c_double), ("sfs_rlxf", C.c_double), ("sfs_minC", C.c_double),
                 ("sfs_maxC", C.c_double), ("rlxf", C.c_double),
                 ("viscous", C.c_int32), ("reserved", C.c_int32),
-                ("nu", C.c_double), ("sgm0", C.c_double), ("beta", C.c_double), ("t_sgm", C.c_double)]
+                ("nu", C.c_double), ("sgm0", C.c_double), ("beta", C.c_double), ("t_sgm", C.c_double),
+                ("rbf_itmax", C.c_int32), ("rbf_iterror", C.c_int32), ("rbf_tol", C.c_double)]
 
 
 _dptr = C.POINTER(C.c_double)
@@ -69,6 +71,11 @@ _SIGS = {
     "vpm_uj_adjoint": (C.c_int, [C.c_void_p, C.POINTER(Config), _dptr, _dptr, _dptr, _dptr, _dptr]),
     "vpm_estr_ranges": (C.c_int, [C.c_void_p, C.POINTER(Config), _i64p, _i64p, _i64p, _i64p, C.c_int64]),
     "vpm_zeta_direct": (C.c_int, [C.c_void_p, C.POINTER(Config)]),
+    "vpm_zeta_ranges": (C.c_int, [C.c_void_p, C.POINTER(Config), _i64p, _i64p, _i64p, _i64p, C.c_int64]),
+    "vpm_core_spreading_reset": (C.c_int, [C.c_void_p, C.POINTER(Config), C.c_double, C.c_int32, C.c_double,
+                                           C.c_int32, C.POINTER(C.c_int32), _dptr]),
+    "vpm_rbf_conjugategradient": (C.c_int, [C.c_void_p, C.POINTER(Config), C.c_int32, C.c_double, C.c_int32,
+                                            C.POINTER(C.c_int32), _dptr]),
     "vpm_tape_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "vpm_tape_destroy": (C.c_int, [C.c_void_p]),
     "vpm_rk3_step_taped": (C.c_int, [C.c_void_p, C.POINTER(StepConfig), C.c_double, C.c_void_p]),

@@ -101,12 +101,18 @@ class Inviscid:
 
 @dataclass
 class CoreSpreading:
-    """CoreSpreading(nu, sgm0; beta=1.5) sigma update (FLOWVPM_viscous.jl:65-223).
-    The RBF core-size reset is not provided: a step that would trigger it
-    raises VPMError(VPM_ERESET)."""
+    """CoreSpreading(nu, sgm0; beta=1.5, itmax=15, tol=1e-3, iterror=True)
+    (FLOWVPM_viscous.jl:63-223): core spreading, and the core-size reset by
+    RBF interpolation with the conjugate gradient (:202-220, :314-478) when
+    the growth reaches beta, with zeta = zeta_direct.  itmax=0 leaves the
+    reset to the caller: the step then raises VPMError(VPM_ERESET).  An
+    unconverged CG with iterror raises VPMError(VPM_ECONVERGE)."""
     nu: float
     sgm0: float
     beta: float = 1.5
+    itmax: int = 15
+    tol: float = 1e-3
+    iterror: bool = True
     t_sgm: float = 0.0
 
 
@@ -177,6 +183,27 @@ class Field:
         """vpm_zeta_direct: J[1:3] of every particle <- the zeta field."""
         cfg = make_config(kernel, precision, True)
         check(lib().vpm_zeta_direct(self.handle, C.byref(cfg)))
+
+    def core_spreading_reset(self, sgm0, kernel=kernel_default, precision="f32c", itmax=15, tol=1e-3,
+                             iterror=True):
+        """vpm_core_spreading_reset on the resident particles; returns
+        (iterations, residuals sqrt(rr/rr0) per component)."""
+        cfg = make_config(kernel, precision, True)
+        it = C.c_int32(0)
+        res = np.zeros(3)
+        check(lib().vpm_core_spreading_reset(self.handle, C.byref(cfg), float(sgm0), int(itmax), float(tol),
+                                             int(bool(iterror)), C.byref(it), res.ctypes.data_as(_lib._dptr)))
+        return it.value, res
+
+    def rbf_conjugategradient(self, kernel=kernel_default, precision="f32c", itmax=15, tol=1e-3, iterror=True):
+        """vpm_rbf_conjugategradient on the resident particles (target
+        vorticity in M[7:9]); returns (iterations, residuals)."""
+        cfg = make_config(kernel, precision, True)
+        it = C.c_int32(0)
+        res = np.zeros(3)
+        check(lib().vpm_rbf_conjugategradient(self.handle, C.byref(cfg), int(itmax), float(tol),
+                                              int(bool(iterror)), C.byref(it), res.ctypes.data_as(_lib._dptr)))
+        return it.value, res
 
     def uj_adjoint(self, Ubar, Jbar, kernel=kernel_default, precision="f32c"):
         """vpm_uj_adjoint on the resident particles: returns (dL/dX (n,3),
@@ -410,6 +437,27 @@ def estr_ranges(pfield, tb, te, sb, se, *, kernel=None, precision=None, transpos
     P[:, 39:42] = tmp[:, 39:42]
 
 
+def zeta_ranges(pfield, tb, te, sb, se, *, kernel=None, precision=None, device=None):
+    """zeta_fmm's pair loop (FLOWVPM_viscous.jl:523-558) over a direct list
+    of 0-based half-open (target, source) range pairs: J[1:3] += the zeta
+    field of the sources, no reset, no static skip."""
+    P = _as_matrix(pfield)
+    pf = pfield if isinstance(pfield, ParticleField) else None
+    kernel = kernel or (pf.kernel if pf else kernel_default)
+    precision = precision or (pf.precision if pf else "f32c")
+    f = _scratch_field(P.shape[0], 0 if device is None else device)
+    f.set_particles(P)
+    arrs = [np.ascontiguousarray(a, dtype=np.int64) for a in (tb, te, sb, se)]
+    if not (len(arrs[0]) == len(arrs[1]) == len(arrs[2]) == len(arrs[3])):
+        raise VPMError(_lib.VPM_EINVAL, "range arrays differ in length")
+    cfg = make_config(kernel, precision, True)
+    ptr = [a.ctypes.data_as(_lib._i64p) for a in arrs]
+    check(lib().vpm_zeta_ranges(f.handle, C.byref(cfg), ptr[0], ptr[1], ptr[2], ptr[3], len(arrs[0])))
+    tmp = np.empty_like(P)
+    f.get_particles(tmp, rows=(15, 18))
+    P[:, 15:18] = tmp[:, 15:18]
+
+
 def Estr_hip(pfield, *, kernel=None, precision=None, transposed=None, device=None):
     """Estr_direct! (FLOWVPM_subfilterscale_models.jl:43-92) on the GPU."""
     P = _as_matrix(pfield)
@@ -439,6 +487,8 @@ def step_config(pfield):
     if isinstance(pfield.viscous, CoreSpreading):
         s.viscous, s.nu, s.sgm0 = 1, pfield.viscous.nu, pfield.viscous.sgm0
         s.beta, s.t_sgm = pfield.viscous.beta, pfield.viscous.t_sgm
+        s.rbf_itmax, s.rbf_tol = int(pfield.viscous.itmax), float(pfield.viscous.tol)
+        s.rbf_iterror = int(bool(pfield.viscous.iterror))
     return s
 
 
@@ -520,6 +570,44 @@ def zeta_direct(pfield, *, precision=None):
     f.set_particles(pfield.particles[:pfield.np])
     f.zeta_direct(pfield.kernel, precision or pfield.precision)
     f.get_particles(pfield.particles, rows=(15, 18), np_=pfield.np)
+
+
+def core_spreading_reset(pfield, scheme=None, *, precision=None):
+    """The CoreSpreading reset (FLOWVPM_viscous.jl:202-220): target vorticity
+    M[7:9] <- zeta_direct field, sigma <- sgm0, and Gamma by
+    rbf_conjugategradient (:314-478), on the device; the host copy is
+    updated and scheme.t_sgm set to 0.  Returns (iterations, residuals)."""
+    scheme = scheme or pfield.viscous
+    if not isinstance(scheme, CoreSpreading):
+        raise TypeError("core_spreading_reset needs a CoreSpreading scheme")
+    out = (0, np.zeros(3))
+    if pfield.np > 0:
+        f = pfield.device_field()
+        f.set_particles(pfield.particles[:pfield.np])
+        try:
+            out = f.core_spreading_reset(scheme.sgm0, pfield.kernel, precision or pfield.precision,
+                                         scheme.itmax, scheme.tol, scheme.iterror)
+        finally:
+            f.get_particles(pfield.particles, np_=pfield.np)
+    scheme.t_sgm = 0.0
+    return out
+
+
+def rbf_conjugategradient(pfield, cs=None, *, precision=None):
+    """rbf_conjugategradient(pfield, cs) (FLOWVPM_viscous.jl:314-478) on the
+    device: Gamma of the non-static particles by CG on the target vorticity
+    in M[7:9]; the host copy is updated.  Returns (iterations, residuals)."""
+    cs = cs or pfield.viscous
+    out = (0, np.zeros(3))
+    if pfield.np > 0:
+        f = pfield.device_field()
+        f.set_particles(pfield.particles[:pfield.np])
+        try:
+            out = f.rbf_conjugategradient(pfield.kernel, precision or pfield.precision, cs.itmax, cs.tol,
+                                          cs.iterror)
+        finally:
+            f.get_particles(pfield.particles, np_=pfield.np)
+    return out
 
 
 class Tape:

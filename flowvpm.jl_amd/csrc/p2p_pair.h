@@ -395,4 +395,43 @@ __device__ __forceinline__ void pair_f64(double tx, double ty, double tz, const 
 }
 #pragma clang fp contract(on)
 
+// zeta(r/sigma)/sigma^3 of a source in FP32 as b exp2(a r^2) (Gaussian-erf),
+// b exp2(a r^3) (Gaussian), b (1 + a r^2)^-3.5 (Winckelmans), with (a, b)
+// from the source's sigma in FP64 (kernel.jl:51,60,69-74).
+template <int K>
+__host__ __device__ __forceinline__ void zeta_params(double sigma, double& a, double& b)
+{
+    const double si = 1.0 / sigma;
+    const double s2 = si * si, s3 = s2 * si;
+    const double l2e = 1.4426950408889634;
+    if (K == VPM_KERNEL_GAUSSIANERF) { a = -0.5 * l2e * s2; b = kConst1 * s3; }
+    else if (K == VPM_KERNEL_GAUSSIAN) { a = -l2e * s3; b = kConst3 * s3; }
+    else { a = s2; b = kConst4 * 7.5 * s3; }
+}
+
+template <int K>
+__device__ __forceinline__ float zeta_ab(float r2, float a, float b)
+{
+    if constexpr (K == VPM_KERNEL_GAUSSIANERF) {
+        return b * ex2(a * r2);
+    } else if constexpr (K == VPM_KERNEL_GAUSSIAN) {
+        const float r = r2 > 0.f ? r2 * rsq(r2) : 0.f;
+        return b * ex2(a * r2 * r);
+    } else {
+        const float s = rsq(fmaf(a, r2, 1.f));
+        const float s2 = s * s;
+        return b * (s2 * s2) * (s2 * s);
+    }
+}
+
+// r^2 beyond which zeta is below FP32 resolution of the self term
+// (q = 7 Gaussian-erf, q = 3 Gaussian); Winckelmans has no cut.
+template <int K>
+__host__ __device__ __forceinline__ float zeta_cut(double sigma)
+{
+    if (K == VPM_KERNEL_GAUSSIANERF) return (float)(49.0 * sigma * sigma);
+    if (K == VPM_KERNEL_GAUSSIAN) return (float)(9.0 * sigma * sigma);
+    return FLT_MAX;
+}
+
 }  // namespace vpm

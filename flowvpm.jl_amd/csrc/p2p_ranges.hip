@@ -612,4 +612,189 @@ int launch_estr_ranges(vpm_field f, int kernel, int precision, int transposed, c
     return VPM_OK;
 }
 
+// ------------------------------------------------------------- zeta ranges
+// zeta_fmm's pair loop over a direct list (src/FLOWVPM_viscous.jl:523-558):
+// for every (target range, source range), J[1:3] of each target += Gamma_j
+// zeta(r/sigma_j)/sigma_j^3 over the sources.  No reset, no static skip.
+//
+// F32C: hi/lo positions relative to the item's first target, (a, b) per
+// source as zeta_params, sources beyond zeta_cut skipped per wave, sums
+// folded into TwoSum every 32 sources and added onto the FP64 record.  BS
+// is the block size: 64 (one wave per FMM leaf) for short items, else 256.
+template <int K, int BS>
+__global__ __launch_bounds__(BS) void zeta_ranges_f32c_kernel(const RangeItem* __restrict__ items,
+                                                              const int64_t* __restrict__ lsb,
+                                                              const int64_t* __restrict__ lse,
+                                                              double* __restrict__ P)
+{
+    __shared__ float4 lp[BS];   // s'_hi, cut
+    __shared__ float4 lg[BS];   // Gamma, b
+    __shared__ float4 ll[BS];   // s'_lo, a
+    const RangeItem it = items[blockIdx.x];
+    const int tid = threadIdx.x;
+    const double* r0 = P + it.t_begin * NF;
+    const double ox = r0[RX], oy = r0[RX + 1], oz = r0[RX + 2];
+    const int64_t t = it.t_begin + tid;
+    const bool active = t < it.t_end;
+    const double* tr = P + (active ? t : it.t_begin) * NF;
+    float tx, ty, tz, tlx, tly, tlz;
+    split_rel(tr[RX], ox, tx, tlx);
+    split_rel(tr[RX + 1], oy, ty, tly);
+    split_rel(tr[RX + 2], oz, tz, tlz);
+    float sum[3] = {0.f, 0.f, 0.f}, err[3] = {0.f, 0.f, 0.f};
+    for (int64_t l = it.l_begin; l < it.l_end; ++l) {
+        const int64_t sb = lsb[l], se = lse[l];
+        for (int64_t base = sb; base < se; base += BS) {
+            const int n = (int)((se - base) < BS ? (se - base) : BS);
+            __syncthreads();
+            if (tid < n) {
+                const double* sr = P + (base + tid) * NF;
+                float4 p, lo;
+                split_rel(sr[RX], ox, p.x, lo.x);
+                split_rel(sr[RX + 1], oy, p.y, lo.y);
+                split_rel(sr[RX + 2], oz, p.z, lo.z);
+                double a, b;
+                zeta_params<K>(sr[RSIG], a, b);
+                p.w = zeta_cut<K>(sr[RSIG]);
+                lo.w = (float)a;
+                lp[tid] = p;
+                ll[tid] = lo;
+                lg[tid] = make_float4((float)sr[RG], (float)sr[RG + 1], (float)sr[RG + 2], (float)b);
+            }
+            __syncthreads();
+            for (int s0 = 0; s0 < n; s0 += 32) {
+                const int s1 = min(s0 + 32, n);
+                float acc[3] = {0.f, 0.f, 0.f};
+                for (int s = s0; s < s1; ++s) {
+                    const float4 a = lp[s];
+                    float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
+                    float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    if (__any(r2 < a.w)) {
+                        const float4 lo = ll[s];
+                        const float4 g = lg[s];
+                        dx += tlx - lo.x;
+                        dy += tly - lo.y;
+                        dz += tlz - lo.z;
+                        r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                        const float z = zeta_ab<K>(r2, lo.w, g.w);
+                        acc[0] = fmaf(g.x, z, acc[0]);
+                        acc[1] = fmaf(g.y, z, acc[1]);
+                        acc[2] = fmaf(g.z, z, acc[2]);
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const float x = acc[c];
+                    const float tt = sum[c] + x;
+                    const float bp = tt - sum[c];
+                    err[c] += (sum[c] - (tt - bp)) + (x - bp);
+                    sum[c] = tt;
+                }
+            }
+        }
+    }
+    if (active) {
+        double* rec = P + t * NF;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) rec[RJ + c] += (double)sum[c] + (double)err[c];
+    }
+}
+
+#pragma clang fp contract(off)
+// FP64: viscous.jl:545-554 per pair, list order, accumulating onto J[1:3]
+// as the reference does.
+template <int K>
+__global__ __launch_bounds__(kBlock) void zeta_ranges_f64_kernel(const RangeItem* __restrict__ items,
+                                                                 const int64_t* __restrict__ lsb,
+                                                                 const int64_t* __restrict__ lse,
+                                                                 double* __restrict__ P)
+{
+    __shared__ double4 ls[kTile];   // x, y, z, sigma
+    __shared__ double4 lg[kTile];   // Gamma
+    const RangeItem it = items[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int64_t t = it.t_begin + tid;
+    const bool active = t < it.t_end;
+    const double* tr = P + (active ? t : it.t_begin) * NF;
+    const double tx = tr[RX], ty = tr[RX + 1], tz = tr[RX + 2];
+    double acc[3] = {tr[RJ], tr[RJ + 1], tr[RJ + 2]};
+    for (int64_t l = it.l_begin; l < it.l_end; ++l) {
+        const int64_t sb = lsb[l], se = lse[l];
+        for (int64_t base = sb; base < se; base += kTile) {
+            const int n = (int)((se - base) < kTile ? (se - base) : kTile);
+            __syncthreads();
+            if (tid < n) {
+                const double* sr = P + (base + tid) * NF;
+                ls[tid] = make_double4(sr[RX], sr[RX + 1], sr[RX + 2], sr[RSIG]);
+                lg[tid] = make_double4(sr[RG], sr[RG + 1], sr[RG + 2], 0.0);
+            }
+            __syncthreads();
+            for (int s = 0; s < n; ++s) {
+                const double4 a = ls[s];
+                const double dX1 = tx - a.x, dX2 = ty - a.y, dX3 = tz - a.z;
+                const double r = sqrt(dX1 * dX1 + dX2 * dX2 + dX3 * dX3);
+                const double zeta_sgm = 1 / (a.w * a.w * a.w) * zeta_ref_r<K>(r / a.w);
+                acc[0] += lg[s].x * zeta_sgm;
+                acc[1] += lg[s].y * zeta_sgm;
+                acc[2] += lg[s].z * zeta_sgm;
+            }
+        }
+    }
+    if (active) {
+        double* rec = P + t * NF;
+        for (int c = 0; c < 3; ++c) rec[RJ + c] = acc[c];
+    }
+}
+#pragma clang fp contract(on)
+
+int launch_zeta_ranges(vpm_field f, int kernel, int precision, const int64_t* tb, const int64_t* te,
+                       const int64_t* sb, const int64_t* se, int64_t npairs)
+{
+    const bool f64 = precision == VPM_PREC_F64 || kernel == VPM_KERNEL_SINGULAR;
+    bool wave = false;
+    if (!f64 && npairs > 0) {
+        int64_t tot = 0, cnt = 0;
+        for (int64_t k = 0; k < npairs; ++k)
+            if (te[k] > tb[k] && (k == 0 || tb[k] != tb[k - 1] || te[k] != te[k - 1])) { tot += te[k] - tb[k]; ++cnt; }
+        wave = cnt > 0 && tot <= 128 * cnt;
+    }
+    RangePlan plan;
+    plan_ranges(tb, te, sb, se, npairs, plan, wave ? 64 : kBlock);
+    if (plan.items.empty()) return VPM_OK;
+    int rc;
+    RangeItem* d_items;
+    int64_t *d_lsb, *d_lse;
+    if ((rc = upload_plan(f, plan, d_items, d_lsb, d_lse))) return rc;
+    const dim3 grid((unsigned)plan.items.size());
+    timing_begin(f);
+    if (f64) {
+        switch (kernel) {
+#define VPM_Z64(KID) case KID: hipLaunchKernelGGL(zeta_ranges_f64_kernel<KID>, grid, dim3(kBlock), 0, f->stream, d_items, d_lsb, d_lse, f->P); break;
+            VPM_Z64(VPM_KERNEL_SINGULAR) VPM_Z64(VPM_KERNEL_GAUSSIAN) VPM_Z64(VPM_KERNEL_GAUSSIANERF) VPM_Z64(VPM_KERNEL_WINCKELMANS)
+#undef VPM_Z64
+        default: return fail(VPM_EINVAL, "unknown kernel id");
+        }
+    } else if (wave) {
+        switch (kernel) {
+#define VPM_ZW(KID) case KID: hipLaunchKernelGGL((zeta_ranges_f32c_kernel<KID, 64>), grid, dim3(64), 0, f->stream, d_items, d_lsb, d_lse, f->P); break;
+            VPM_ZW(VPM_KERNEL_GAUSSIAN) VPM_ZW(VPM_KERNEL_GAUSSIANERF) VPM_ZW(VPM_KERNEL_WINCKELMANS)
+#undef VPM_ZW
+        default: return fail(VPM_EINVAL, "unknown kernel id");
+        }
+    } else {
+        switch (kernel) {
+#define VPM_Z32(KID) case KID: hipLaunchKernelGGL((zeta_ranges_f32c_kernel<KID, kBlock>), grid, dim3(kBlock), 0, f->stream, d_items, d_lsb, d_lse, f->P); break;
+            VPM_Z32(VPM_KERNEL_GAUSSIAN) VPM_Z32(VPM_KERNEL_GAUSSIANERF) VPM_Z32(VPM_KERNEL_WINCKELMANS)
+#undef VPM_Z32
+        default: return fail(VPM_EINVAL, "unknown kernel id");
+        }
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "zeta_ranges kernel");
+    timing_end(f, 4);
+    e = hipStreamSynchronize(f->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    return VPM_OK;
+}
+
 }  // namespace vpm

@@ -364,6 +364,22 @@ int vpm_estr_ranges(vpm_field f, const vpm_config* cfg, const int64_t* tb, const
     return sync(f);
 }
 
+int vpm_zeta_ranges(vpm_field f, const vpm_config* cfg, const int64_t* tb, const int64_t* te, const int64_t* sb,
+                    const int64_t* se, int64_t npairs)
+{
+    VPM_FIELD_GUARD(f);
+    if (int rc = check_cfg(cfg)) return rc;
+    if (npairs < 0) return fail(VPM_EINVAL, "npairs < 0");
+    if (npairs == 0) return VPM_OK;
+    if (!tb || !te || !sb || !se) return fail(VPM_EINVAL, "null range array");
+    for (int64_t k = 0; k < npairs; ++k) {
+        if (tb[k] < 0 || te[k] > f->np || tb[k] > te[k] || sb[k] < 0 || se[k] > f->np || sb[k] > se[k])
+            return fail(VPM_EINVAL, "range pair " + std::to_string(k) + " outside [0, np)");
+    }
+    if (int rc = launch_zeta_ranges(f, cfg->kernel, cfg->precision, tb, te, sb, se, npairs)) return rc;
+    return sync(f);
+}
+
 int vpm_estr_direct(vpm_field f, const vpm_config* cfg)
 {
     VPM_FIELD_GUARD(f);
@@ -441,7 +457,7 @@ static int rk3_step_locked(vpm_field f, vpm_step_config* s, double dt, int32_t r
     const double zeta0 = zeta0_of(cfg->kernel);                    // timeintegration.jl:398
     const bool sfs = s->sfs != 0;
     int rc;
-    bool reset_needed = false;
+    bool reset_needed = false, unconverged = false;
     const size_t rec = sizeof(double) * NF * (size_t)f->np;
     if (tape) {
         if (sfs || s->viscous || (relax_now && s->relax))
@@ -484,7 +500,19 @@ static int rk3_step_locked(vpm_field f, vpm_step_config* s, double dt, int32_t r
             if (std::fabs(b - 8.0 / 15.0) <= 1e-7) {
                 s->t_sgm += dt;
                 const double beta_cur = std::sqrt(2 * s->nu * s->t_sgm / (s->sgm0 * s->sgm0) + 1);
-                if (beta_cur >= s->beta) reset_needed = true;
+                if (beta_cur >= s->beta) {                          // viscous.jl:202-220
+                    if (s->rbf_itmax > 0) {
+                        int iters = 0;
+                        double res[3];
+                        rc = launch_core_spreading_reset(f, cfg->kernel, cfg->precision, s->sgm0, s->rbf_itmax,
+                                                         s->rbf_tol, s->rbf_iterror, &iters, res);
+                        if (rc == VPM_ECONVERGE) unconverged = true;
+                        else if (rc) return rc;
+                        s->t_sgm = 0;
+                    } else {
+                        reset_needed = true;
+                    }
+                }
             }
         }
     }
@@ -502,7 +530,9 @@ static int rk3_step_locked(vpm_field f, vpm_step_config* s, double dt, int32_t r
         tape->recorded = true;
     }
     if (reset_needed)
-        return fail(VPM_ERESET, "core-spreading reset required (sigma growth >= beta); RBF reset not provided");
+        return fail(VPM_ERESET, "core-spreading reset required (sigma growth >= beta) and rbf_itmax is 0");
+    if (unconverged)
+        return fail(VPM_ECONVERGE, "RBF interpolation: maximum number of iterations reached before convergence");
     return VPM_OK;
 }
 
@@ -521,6 +551,35 @@ int vpm_zeta_direct(vpm_field f, const vpm_config* cfg)
     if (int rc = check_cfg(cfg)) return rc;
     if (int rc = launch_zeta(f, cfg->kernel, cfg->precision)) return rc;
     return sync(f);
+}
+
+int vpm_core_spreading_reset(vpm_field f, const vpm_config* cfg, double sgm0, int32_t itmax, double tol,
+                             int32_t iterror, int32_t* iters, double* res)
+{
+    VPM_FIELD_GUARD(f);
+    if (int rc = check_cfg(cfg)) return rc;
+    if (!(sgm0 > 0) || !std::isfinite(sgm0)) return fail(VPM_EINVAL, "sgm0 must be positive and finite");
+    if (itmax < 0) return fail(VPM_EINVAL, "itmax must be >= 0");
+    int it = 0;
+    double r[3] = {0, 0, 0};
+    const int rc = launch_core_spreading_reset(f, cfg->kernel, cfg->precision, sgm0, itmax, tol, iterror, &it, r);
+    if (iters) *iters = it;
+    if (res) for (int c = 0; c < 3; ++c) res[c] = r[c];
+    return rc;
+}
+
+int vpm_rbf_conjugategradient(vpm_field f, const vpm_config* cfg, int32_t itmax, double tol, int32_t iterror,
+                              int32_t* iters, double* res)
+{
+    VPM_FIELD_GUARD(f);
+    if (int rc = check_cfg(cfg)) return rc;
+    if (itmax < 0) return fail(VPM_EINVAL, "itmax must be >= 0");
+    int it = 0;
+    double r[3] = {0, 0, 0};
+    const int rc = launch_rbf_cg(f, cfg->kernel, cfg->precision, itmax, tol, iterror, &it, r);
+    if (iters) *iters = it;
+    if (res) for (int c = 0; c < 3; ++c) res[c] = r[c];
+    return rc;
 }
 
 int vpm_tape_create(vpm_field f, vpm_tape* out)
@@ -616,7 +675,7 @@ int vpm_profile(vpm_field f, int32_t enable)
 int vpm_timing(vpm_field f, int32_t which, double* total_ms, int64_t* count)
 {
     if (!f || !total_ms || !count) return fail(VPM_EINVAL, "null argument");
-    if (which < 0 || which > 3) return fail(VPM_EINVAL, "timing class must be 0..3");
+    if (which < 0 || which > 4) return fail(VPM_EINVAL, "timing class must be 0..4");
     std::lock_guard<std::mutex> lock(f->mu);
     *total_ms = f->timing[which].ms;
     *count = f->timing[which].count;

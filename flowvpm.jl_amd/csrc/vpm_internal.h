@@ -65,7 +65,7 @@ struct vpm_field_s {
     vpm::DevBuf xfer;                // gathered rows for partial downloads
     bool profiling = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    vpm::Timing timing[4];
+    vpm::Timing timing[5];      // U/J F32C, U/J F64, E_str, adjoint, zeta
 };
 
 // Stage records of one rk3 step (state after each stage's UJ evaluation),
@@ -128,8 +128,14 @@ int launch_update_states_adjoint(vpm_field f, double* B, double a, double b, dou
 int launch_adj_add_bar(vpm_field f, double* B, const double* dX, const double* dG, const double* dS);
 int launch_adj_step_start(vpm_field f, double* B);
 int launch_zeta(vpm_field f, int kernel, int precision);
+int launch_core_spreading_reset(vpm_field f, int kernel, int precision, double sgm0, int itmax, double tol,
+                                int iterror, int* iters, double res[3]);
+int launch_rbf_cg(vpm_field f, int kernel, int precision, int itmax, double tol, int iterror, int* iters,
+                  double res[3]);
 int launch_sfs_delta(vpm_field f, int kernel, int transposed, double alpha);
 int launch_gather_rows(vpm_field f, int row0, int nrows, double* out);
+int launch_zeta_ranges(vpm_field f, int kernel, int precision, const int64_t* tb, const int64_t* te,
+                       const int64_t* sb, const int64_t* se, int64_t npairs);
 int launch_estr_ranges(vpm_field f, int kernel, int precision, int transposed, const int64_t* tb,
                        const int64_t* te, const int64_t* sb, const int64_t* se, int64_t npairs);
 

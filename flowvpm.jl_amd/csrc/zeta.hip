@@ -25,28 +25,6 @@
 
 namespace vpm {
 
-template <int K>
-__device__ __forceinline__ float zeta_ab(float r2, float a, float b)
-{
-    if constexpr (K == VPM_KERNEL_GAUSSIANERF) {
-        return b * ex2(a * r2);
-    } else if constexpr (K == VPM_KERNEL_GAUSSIAN) {
-        const float r = r2 > 0.f ? r2 * rsq(r2) : 0.f;
-        return b * ex2(a * r2 * r);
-    } else {
-        const float s = rsq(fmaf(a, r2, 1.f));
-        const float s2 = s * s;
-        return b * (s2 * s2) * (s2 * s);
-    }
-}
-
-template <int K>
-__host__ __device__ __forceinline__ float zeta_cut(double sigma)
-{
-    if (K == VPM_KERNEL_GAUSSIANERF) return (float)(49.0 * sigma * sigma);
-    if (K == VPM_KERNEL_GAUSSIAN) return (float)(9.0 * sigma * sigma);
-    return FLT_MAX;
-}
 
 // Sorted source records: positions FP64 SoA, (Gamma, a) and (b, cut) FP32.
 template <int K>
@@ -63,13 +41,8 @@ __global__ __launch_bounds__(kBlock) void pack_zeta_kernel(const double* __restr
     pos[ld + k] = rec[RX + 1];
     pos[2 * ld + k] = rec[RX + 2];
     const double sg = rec[RSIG];
-    const double si = 1.0 / sg;
-    const double s2 = si * si, s3 = s2 * si;
-    const double l2e = 1.4426950408889634;
     double a, b;
-    if (K == VPM_KERNEL_GAUSSIANERF) { a = -0.5 * l2e * s2; b = kConst1 * s3; }
-    else if (K == VPM_KERNEL_GAUSSIAN) { a = -l2e * s3; b = kConst3 * s3; }
-    else { a = s2; b = kConst4 * 7.5 * s3; }
+    zeta_params<K>(sg, a, b);
     gz[k] = make_float4((float)rec[RG], (float)rec[RG + 1], (float)rec[RG + 2], (float)a);
     bc[k] = make_float2((float)b, zeta_cut<K>(sg));
 }
@@ -255,6 +228,7 @@ int launch_zeta(vpm_field f, int kernel, int precision)
     const uint32_t* perm = nullptr;
     hipError_t e;
     if (precision == VPM_PREC_F64 || kernel == VPM_KERNEL_SINGULAR) {
+        timing_begin(f);
         switch (kernel) {
 #define VPM_Z64(KID)                                                                                        \
     case KID:                                                                                               \
@@ -267,6 +241,7 @@ int launch_zeta(vpm_field f, int kernel, int precision)
             return fail(VPM_EINVAL, "unknown kernel id");
         }
         if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "zeta_f64");
+        timing_end(f, 4);
     } else {
         if (np > 1 && (rc = sort_field(f, perm))) return rc;
         if ((rc = ensure(f->pos, sizeof(double) * 3 * (size_t)ld))) return rc;
@@ -279,6 +254,7 @@ int launch_zeta(vpm_field f, int kernel, int precision)
     case KID:                                                                                                  \
         hipLaunchKernelGGL(pack_zeta_kernel<KID>, dim3(nbt), dim3(kBlock), 0, f->stream, f->P, np, perm, pos, ld, \
                            gz, bc);                                                                            \
+        timing_begin(f);                                                                                       \
         hipLaunchKernelGGL(zeta_f32c_kernel<KID>, grid, dim3(kBlock), 0, f->stream, (const double*)pos, ld,     \
                            (const float4*)gz, (const float2*)bc, np, chunk, part, np);                        \
         break;
@@ -288,6 +264,7 @@ int launch_zeta(vpm_field f, int kernel, int precision)
             return fail(VPM_EINVAL, "unknown kernel id");
         }
         if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "zeta_f32c");
+        timing_end(f, 4);
     }
     hipLaunchKernelGGL(zeta_finalize_kernel, dim3(nbt), dim3(kBlock), 0, f->stream, part, np, (int)K, np, perm,
                        f->P);

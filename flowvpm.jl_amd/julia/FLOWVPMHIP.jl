@@ -37,6 +37,7 @@ mutable struct VpmStepConfig
     rlxf::Float64
     viscous::Int32; reserved::Int32
     nu::Float64; sgm0::Float64; beta::Float64; t_sgm::Float64
+    rbf_itmax::Int32; rbf_iterror::Int32; rbf_tol::Float64   # CoreSpreading itmax, iterror, tol
 end
 
 check(rc) = rc == VPM_OK ? nothing :
@@ -114,6 +115,106 @@ function direct_hip!(pfield::vpm.ParticleField, target_ranges, source_ranges; pr
 end
 
 """
+    estr_hip!(pfield, target_ranges, source_ranges; precision=0)
+
+Estr_fmm! over a FastMultipole direct list (src/FLOWVPM_subfilterscale_models.jl:157-188):
+E_str of every (target range, source range) pair onto SFS, with no reset and
+no static skip.
+"""
+function estr_hip!(pfield::vpm.ParticleField, target_ranges, source_ranges; precision=0)
+    n = length(target_ranges)
+    tb = Int64[first(r) - 1 for r in target_ranges]; te = Int64[last(r) for r in target_ranges]
+    sb = Int64[first(r) - 1 for r in source_ranges]; se = Int64[last(r) for r in source_ranges]
+    upload(pfield)
+    cfg = Ref(config(pfield; precision))
+    check(ccall((:vpm_estr_ranges, libvpm), Cint,
+                (Ptr{Cvoid}, Ref{VpmConfig}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Int64),
+                device_field(pfield), cfg, tb, te, sb, se, n))
+    download(pfield, Int32(39), Int32(42))
+    return nothing
+end
+
+"""
+    zeta_hip(pfield; precision=0)
+
+zeta_direct(pfield) (src/FLOWVPM_viscous.jl:486-515) on the device: J[1:3]
+of every particle <- the basis-function field.  Pass it as CoreSpreading's
+`zeta` argument.
+"""
+function zeta_hip(pfield::vpm.ParticleField; precision=0)
+    upload(pfield)
+    cfg = Ref(config(pfield; precision))
+    check(ccall((:vpm_zeta_direct, libvpm), Cint, (Ptr{Cvoid}, Ref{VpmConfig}), device_field(pfield), cfg))
+    download(pfield, Int32(15), Int32(18))
+    return nothing
+end
+
+"""
+    zeta_fmm_hip(pfield; precision=0)
+
+zeta_fmm(pfield) (src/FLOWVPM_viscous.jl:523-558) with its pair loop on the
+device: the same FastMultipole tree and direct list; the particles are taken
+in tree order (sort_index) into a scratch device field, and for each
+(target branch, source branch) of the list the target branch's strengths are
+added onto J[1:3] of the source branch's particles, as the reference's loop
+does.  J is not reset.  Pass it as CoreSpreading's `zeta` argument.
+"""
+function zeta_fmm_hip(pfield::vpm.ParticleField; precision=0)
+    n = pfield.np
+    n == 0 && return nothing
+    fmm = pfield.fmm
+    FMM = vpm.FastMultipole
+    tree = FMM.Tree((pfield,); leaf_size=fmm.ncrit, shrink_recenter=fmm.shrink_recenter)
+    _, direct_list = FMM.build_interaction_lists(tree.branches, tree.branches, fmm.ncrit, fmm.theta,
+                                                 false, true, true)
+    perm = tree.sort_index_list[1]
+    Psorted = pfield.particles[:, perm[1:n]]
+    tb = Int64[]; te = Int64[]; sb = Int64[]; se = Int64[]
+    for (i_target, i_source) in direct_list
+        writes = tree.branches[i_source].bodies_index[1]     # particles written (the reference's Pi)
+        reads = tree.branches[i_target].bodies_index[1]      # particles read (the reference's Pj)
+        push!(tb, first(writes) - 1); push!(te, last(writes))
+        push!(sb, first(reads) - 1); push!(se, last(reads))
+    end
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:vpm_create, libvpm), Cint, (Cint, Int64, Ref{Ptr{Cvoid}}),
+                pfield.useGPU > 0 ? pfield.useGPU - 1 : 0, n, r))
+    h = r[]
+    try
+        cfg = Ref(config(pfield; precision))
+        check(ccall((:vpm_set_particles, libvpm), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int64), h, Psorted, n))
+        check(ccall((:vpm_zeta_ranges, libvpm), Cint,
+                    (Ptr{Cvoid}, Ref{VpmConfig}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Int64),
+                    h, cfg, tb, te, sb, se, length(tb)))
+        check(ccall((:vpm_get_particles, libvpm), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int64, Int32, Int32),
+                    h, Psorted, n, Int32(15), Int32(18)))
+    finally
+        ccall((:vpm_destroy, libvpm), Cint, (Ptr{Cvoid},), h)
+    end
+    pfield.particles[16:18, perm[1:n]] .= Psorted[16:18, :]
+    return nothing
+end
+
+"""
+    rbf_hip(pfield, cs::CoreSpreading; precision=0)
+
+rbf_conjugategradient(pfield, cs) (src/FLOWVPM_viscous.jl:314-478) on the
+device, with zeta = zeta_direct; pass it as CoreSpreading's `rbf` keyword.
+Errors like the reference when cs.iterror and itmax is reached unconverged.
+"""
+function rbf_hip(pfield::vpm.ParticleField, cs; precision=0)
+    upload(pfield)
+    cfg = Ref(config(pfield; precision))
+    it = Ref{Int32}(0); res = zeros(3)
+    rc = ccall((:vpm_rbf_conjugategradient, libvpm), Cint,
+               (Ptr{Cvoid}, Ref{VpmConfig}, Int32, Float64, Int32, Ref{Int32}, Ptr{Float64}),
+               device_field(pfield), cfg, Int32(cs.itmax), Float64(cs.tol), Int32(cs.iterror), it, res)
+    download(pfield, Int32(0), Int32(46))
+    check(rc)
+    return nothing
+end
+
+"""
     rungekutta3_hip(pfield, dt; relax=false, precision=0)
 
 One rungekutta3 step of ReformulatedVPM with UJ = UJ_direct, entirely on the
@@ -131,7 +232,9 @@ function rungekutta3_hip(pfield::vpm.ParticleField, dt::Real; relax::Bool=false,
             sfs ? pfield.SFS.alpha : 0.0, sfs ? pfield.SFS.rlxf : 0.0,
             sfs ? pfield.SFS.minC : 0.0, sfs ? pfield.SFS.maxC : 1.0, rl.rlxf,
             Int32(visc), 0, visc ? pfield.viscous.nu : 0.0, visc ? pfield.viscous.sgm0 : 1.0,
-            visc ? pfield.viscous.beta : 1.5, visc ? pfield.viscous.t_sgm : 0.0))
+            visc ? pfield.viscous.beta : 1.5, visc ? pfield.viscous.t_sgm : 0.0,
+            Int32(visc ? pfield.viscous.itmax : 0), Int32(visc && pfield.viscous.iterror),
+            visc ? Float64(pfield.viscous.tol) : 0.0))
     rc = ccall((:vpm_rk3_step, libvpm), Cint, (Ptr{Cvoid}, Ref{VpmStepConfig}, Float64, Int32),
                device_field(pfield), s, dt, Int32(relax))
     visc && (pfield.viscous.t_sgm = s[].t_sgm)
@@ -189,7 +292,7 @@ function rungekutta3_taped!(pfield::vpm.ParticleField, dt::Real; precision=0)
     finalizer(t -> ccall((:vpm_tape_destroy, libvpm), Cint, (Ptr{Cvoid},), t.h), t)
     s = Ref(VpmStepConfig(config(pfield; precision), pfield.formulation.f, pfield.formulation.g,
             Tuple(pfield.Uinf(pfield.t)), Int32(0), Int32(0), Int32(0), Int32(0), 0.0, 0.0, 0.0, 1.0, 0.0,
-            Int32(0), 0, 0.0, 1.0, 1.5, 0.0))
+            Int32(0), 0, 0.0, 1.0, 1.5, 0.0, Int32(0), Int32(0), 0.0))
     check(ccall((:vpm_rk3_step_taped, libvpm), Cint, (Ptr{Cvoid}, Ref{VpmStepConfig}, Float64, Ptr{Cvoid}),
                 device_field(pfield), s, dt, t.h))
     download(pfield, Int32(0), Int32(46))

@@ -33,7 +33,8 @@ typedef enum {
     VPM_ENOMEM = -3,    /* device allocation failed                            */
     VPM_EDEVICE = -4,   /* no such device or a HIP runtime error               */
     VPM_ENAN = -5,      /* NaN in the dynamic SFS coefficient (subfilterscale.jl:645-652) */
-    VPM_ERESET = -6     /* core-spreading reset required (viscous.jl:202); not provided */
+    VPM_ERESET = -6,    /* core-spreading reset required (viscous.jl:202) and rbf_itmax == 0 */
+    VPM_ECONVERGE = -7  /* RBF conjugate gradient unconverged at itmax (viscous.jl:421-431) */
 } vpm_status;
 
 /* Regularization kernels (src/FLOWVPM.jl:129-133, src/FLOWVPM_kernel.jl:45-84) */
@@ -71,6 +72,12 @@ typedef struct {
     int32_t reserved;
     double nu, sgm0, beta;
     double t_sgm;              /* core-spreading timer; updated in place */
+    /* CoreSpreading's RBF reset (viscous.jl:202-220, 314-478): itmax, tol,
+     * iterror of the CoreSpreading scheme (viscous.jl:63-101).  rbf_itmax == 0
+     * leaves the reset to the caller (the step returns VPM_ERESET). */
+    int32_t rbf_itmax;
+    int32_t rbf_iterror;
+    double rbf_tol;
 } vpm_step_config;
 
 typedef struct vpm_field_s* vpm_field;
@@ -121,6 +128,15 @@ int vpm_p2p_ranges(vpm_field f, const vpm_config* cfg,
 int vpm_estr_ranges(vpm_field f, const vpm_config* cfg,
                     const int64_t* tb, const int64_t* te,
                     const int64_t* sb, const int64_t* se, int64_t npairs);
+/* zeta_fmm's pair loop over a direct list (src/FLOWVPM_viscous.jl:523-558):
+ * for k < npairs, J[1:3] of every target in [tb[k], te[k]) +=
+ * Gamma_j zeta(r/sigma_j)/sigma_j^3 over the sources j in [sb[k], se[k]).
+ * As in the reference, J is not reset and static particles are not skipped.
+ * The reference's loop writes the particles of its source branch with the
+ * strengths of its target branch: pass those as tb/te and sb/se here. */
+int vpm_zeta_ranges(vpm_field f, const vpm_config* cfg,
+                    const int64_t* tb, const int64_t* te,
+                    const int64_t* sb, const int64_t* se, int64_t npairs);
 
 /* ---- terms built from J ------------------------------------------------- */
 /* Estr_direct! (src/FLOWVPM_subfilterscale_models.jl:16-92): SFS += sum over
@@ -151,8 +167,12 @@ int vpm_sfs_dynamic_after(vpm_field f, const vpm_config* cfg, double alpha, doub
 /* One rungekutta3 step for ReformulatedVPM with UJ = UJ_direct
  * (timeintegration.jl:388-461) and nextstep's U_prev update
  * (particlefield.jl:443-455).  relax_now mirrors run_vpm!'s relax flag
- * (utils.jl:94-96).  Returns VPM_ERESET (after completing the step) when
- * CoreSpreading would reset core sizes (viscous.jl:202-220, not provided). */
+ * (utils.jl:94-96).  When CoreSpreading's core growth reaches beta in the
+ * last stage (viscous.jl:189-220), the reset runs there as
+ * vpm_core_spreading_reset with zeta = zeta_direct and the rbf_* settings,
+ * and t_sgm returns to 0; with rbf_itmax == 0 the step completes without it
+ * and returns VPM_ERESET.  VPM_ECONVERGE: the reset's CG did not converge
+ * (the step is complete, Gamma holds the last iterate). */
 int vpm_rk3_step(vpm_field f, vpm_step_config* s, double dt, int32_t relax_now);
 
 /* zeta_direct (src/FLOWVPM_viscous.jl:486-515): J[1:3] of every particle,
@@ -160,6 +180,25 @@ int vpm_rk3_step(vpm_field f, vpm_step_config* s, double dt, int32_t relax_now);
  * included) of Gamma_j zeta(r/sigma_j)/sigma_j^3.  The other J rows are left
  * as they are.  The singular kernel always runs in FP64. */
 int vpm_zeta_direct(vpm_field f, const vpm_config* cfg);
+
+/* CoreSpreading reset (viscous.jl:202-220): target vorticity M[7:9] <- zeta
+ * field (zeta_direct), sigma <- sgm0, then rbf_conjugategradient
+ * (viscous.jl:314-478): Gamma of the non-static particles solves
+ * sum_j Gamma_j zeta(r_ij/sgm0)/sgm0^3 = target, per component, by CG from
+ * the guess target*vol, stopping when sqrt(rr/rr0) <= tol on all three
+ * components or after itmax iterations.  M[1:6] hold the solution and the
+ * residual, J[1:3] the last zeta field, as in the reference.  Dot products
+ * are FP64 with a fixed summation order (deterministic).  *iters and res
+ * (sqrt(rr/rr0) per component; either may be null) report the iteration.
+ * Returns VPM_ECONVERGE when itmax is reached unconverged and iterror != 0
+ * (Gamma then holds the last iterate; the reference raises). */
+int vpm_core_spreading_reset(vpm_field f, const vpm_config* cfg, double sgm0, int32_t itmax, double tol,
+                             int32_t iterror, int32_t* iters, double* res);
+/* rbf_conjugategradient(pfield, cs) alone (viscous.jl:314-478): the CG of
+ * vpm_core_spreading_reset on the target vorticity already in M[7:9], with
+ * the core sizes as they are. */
+int vpm_rbf_conjugategradient(vpm_field f, const vpm_config* cfg, int32_t itmax, double tol, int32_t iterror,
+                              int32_t* iters, double* res);
 
 /* ---- adjoint (replaces src/FLOWVPM_rrules.jl:179-330, UJS_pullback) ------ */
 /* For L = sum_i Ubar_i . U_i + sum_i sum_k Jbar_i[k] J_i[k] with U, J from one
@@ -196,7 +235,7 @@ int vpm_rk3_step_adjoint(vpm_tape t, double* bar);
 /* When enabled, the library brackets each all-pairs kernel launch with HIP
  * events on the field's stream.  vpm_timing returns the summed duration (ms)
  * and the launch count of one kernel class since the last reset:
- * 0 U/J FP32c, 1 U/J FP64, 2 E_str, 3 adjoint. */
+ * 0 U/J FP32c, 1 U/J FP64, 2 E_str, 3 adjoint, 4 zeta. */
 int vpm_profile(vpm_field f, int32_t enable);
 int vpm_timing(vpm_field f, int32_t which, double* total_ms, int64_t* count);
 

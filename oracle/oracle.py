@@ -36,7 +36,8 @@ class Settings(C.Structure):
                 ("rlxf", C.c_double),
                 ("viscous", C.c_int32), ("nthreads", C.c_int32),
                 ("nu", C.c_double), ("sgm0", C.c_double), ("beta", C.c_double),
-                ("t_sgm", C.c_double)]
+                ("t_sgm", C.c_double),
+                ("rbf_itmax", C.c_int32), ("rbf_iterror", C.c_int32), ("rbf_tol", C.c_double)]
 
 
 def build():
@@ -77,6 +78,7 @@ def lib():
         L.vpmo_core_spreading_reset.argtypes = [_dp, C.c_int64, C.c_int, C.c_double, C.c_int, C.c_double,
                                                 C.c_int, C.c_int, C.POINTER(C.c_int), _dp]
         L.vpmo_zeta_direct.argtypes = [_dp, C.c_int64, C.c_int, C.c_int]
+        L.vpmo_zeta_ranges.argtypes = [_dp, C.c_int64, C.c_int, _ip, _ip, _ip, _ip, C.c_int64]
         L.vpmo_rk3_step_adjoint.restype = C.c_int
         L.vpmo_rk3_step_adjoint.argtypes = [_dp, C.c_int64, C.c_void_p, C.c_double, _dp]
         L.vpmo_core_spreading_rk3.restype = C.c_int
@@ -184,6 +186,12 @@ def zeta_direct(P, kernel, nthreads=None):
     lib().vpmo_zeta_direct(_check(P), P.shape[0], _kid(kernel), nthreads or nthreads_default())
 
 
+def zeta_ranges(P, kernel, tb, te, sb, se):
+    """zeta_fmm's pair loop over a direct list, accumulated onto J rows 15:18."""
+    a = [np.ascontiguousarray(x, dtype=np.int64) for x in (tb, te, sb, se)]
+    lib().vpmo_zeta_ranges(_check(P), NFIELDS, _kid(kernel), *a, len(a[0]))
+
+
 def estr_direct(P, kernel, transposed=True, np_=None, nthreads=None):
     lib().vpmo_estr_direct(_check(P), np_ if np_ is not None else P.shape[0], _kid(kernel),
                            int(transposed), nthreads or nthreads_default())
@@ -219,7 +227,8 @@ def sfs_after(P, kernel, transposed, alpha, rlxf, minC, maxC, force_positive=Tru
 def make_settings(kernel="gaussianerf", transposed=True, f=0.0, g=0.2, Uinf=(0, 0, 0),
                   sfs=0, sfs_alpha=0.999, sfs_rlxf=0.005, sfs_minC=0.0, sfs_maxC=1.0,
                   sfs_force_positive=True, sfs_clip=True, relax=1, rlxf=0.3,
-                  viscous=0, nu=0.0, sgm0=1.0, beta=1.5, t_sgm=0.0, nthreads=None):
+                  viscous=0, nu=0.0, sgm0=1.0, beta=1.5, t_sgm=0.0, nthreads=None,
+                  rbf_itmax=15, rbf_tol=1e-3, rbf_iterror=True):
     s = Settings()
     s.kernel, s.transposed, s.f, s.g = _kid(kernel), int(transposed), f, g
     s.Uinf[:] = list(Uinf)
@@ -227,6 +236,7 @@ def make_settings(kernel="gaussianerf", transposed=True, f=0.0, g=0.2, Uinf=(0, 
     s.relax, s.sfs_alpha, s.sfs_rlxf, s.sfs_minC, s.sfs_maxC = int(relax), sfs_alpha, sfs_rlxf, sfs_minC, sfs_maxC
     s.rlxf, s.viscous, s.nthreads = rlxf, int(viscous), int(nthreads or nthreads_default())
     s.nu, s.sgm0, s.beta, s.t_sgm = nu, sgm0, beta, t_sgm
+    s.rbf_itmax, s.rbf_tol, s.rbf_iterror = int(rbf_itmax), float(rbf_tol), int(bool(rbf_iterror))
     return s
 
 

@@ -30,6 +30,8 @@
  *                      vorticity; iterated to tight tolerance it equals a dense
  *                      solve of the zeta system
  *   vpmo_zeta_direct   mpmath brute force; one particle gives Gamma zeta(0)/sigma^3
+ *   vpmo_zeta_ranges   the single pair (all, all) onto zeroed J equals
+ *                      vpmo_zeta_direct; sub-ranges sum to the same
  *   vpmo_update_particle_states_adjoint / vpmo_rk3_step_adjoint
  *                      central finite differences of vpmo_update_particle_states
  *                      and of a loss through whole vpmo_rk3_step calls
@@ -557,8 +559,18 @@ int vpmo_rk3_step(double* P, int64_t np, vpmo_settings* s, double dt, int relax_
         }
         vpmo_update_particle_states(P, np, a, b, dt, s->Uinf, s->f, s->g, zeta0, s->transposed);
         if (s->viscous == 1) {
-            if (vpmo_core_spreading_rk3(P, np, s->nu, s->sgm0, s->beta, &s->t_sgm, a, b, dt))
-                status = 1;
+            if (vpmo_core_spreading_rk3(P, np, s->nu, s->sgm0, s->beta, &s->t_sgm, a, b, dt)) {
+                if (s->rbf_itmax > 0) {                             /* viscous.jl:202-220 */
+                    int its;
+                    double res[3];
+                    if (vpmo_core_spreading_reset(P, np, s->kernel, s->sgm0, s->rbf_itmax, s->rbf_tol,
+                                                  s->rbf_iterror, s->nthreads, &its, res))
+                        status = 2;
+                    s->t_sgm = 0;                                   /* :218 */
+                } else {
+                    status = 1;
+                }
+            }
         }
     }
     if (relax_now && s->relax) {                                    /* :437-458 */
@@ -886,6 +898,36 @@ void vpmo_zeta_direct(double* P, int64_t np, int kernel, int nthreads)
         }
         for (int k = 0; k < 3; k++) pi[RJ + k] = acc[k];
     }
+}
+
+/* ---- zeta_fmm's pair loop over a direct list (viscous.jl:523-558) ----
+ * For every (target range, source range) pair in list order, every target i
+ * of the range gets, over the sources j in order,
+ * Gamma_j zeta(r_ij/sigma_j)/sigma_j^3 added onto J[1:3] (:541-554).  No
+ * reset (zeta_fmm zeroes nothing) and no static skip.  The reference's loop
+ * writes the particles of its *source* branch with the strengths of its
+ * *target* branch; a caller restating it passes those as targets and
+ * sources here. */
+void vpmo_zeta_ranges(double* P, int64_t stride, int kernel,
+                      const int64_t* tb, const int64_t* te,
+                      const int64_t* sb, const int64_t* se, int64_t npairs)
+{
+    for (int64_t k = 0; k < npairs; k++)
+        for (int64_t i = tb[k]; i < te[k]; i++) {
+            double* pi = P + i * stride;
+            for (int64_t j = sb[k]; j < se[k]; j++) {
+                const double* pj = P + j * stride;
+                const double dX1 = pi[RX] - pj[RX];                  /* :545-547 */
+                const double dX2 = pi[RX + 1] - pj[RX + 1];
+                const double dX3 = pi[RX + 2] - pj[RX + 2];
+                const double r = sqrt(dX1 * dX1 + dX2 * dX2 + dX3 * dX3);
+                const double s = pj[RSIG];
+                const double zeta_sgm = 1 / (s * s * s) * vpmo_zeta(kernel, r / s);   /* :550 */
+                pi[RJ + 0] += pj[RG + 0] * zeta_sgm;                 /* :552-554 */
+                pi[RJ + 1] += pj[RG + 1] * zeta_sgm;
+                pi[RJ + 2] += pj[RG + 2] * zeta_sgm;
+            }
+        }
 }
 
 /* ---- Estr_fmm! over a direct list (subfilterscale_models.jl:157-188) ----

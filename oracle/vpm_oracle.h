@@ -83,12 +83,18 @@ typedef struct {
     int32_t nthreads;
     double  nu, sgm0, beta;
     double  t_sgm;           /* core-spreading timer, updated in place */
+    int32_t rbf_itmax;       /* CoreSpreading itmax; 0: no reset (status 1) */
+    int32_t rbf_iterror;     /* CoreSpreading iterror */
+    double  rbf_tol;         /* CoreSpreading tol */
 } vpmo_settings;
 
 /* One rungekutta3 step for ReformulatedVPM (timeintegration.jl:388-461) with
  * UJ = UJ_direct.  relax_now mirrors run_vpm!'s relax flag (utils.jl:94-96).
- * Returns 0, a negative SFS code, or 1 when a core-spreading reset would be
- * required (not restated: RBF reset is out of the path). */
+ * When the core growth reaches beta in the last stage, the reset
+ * (vpmo_core_spreading_reset with the rbf_* settings) runs there and t_sgm
+ * returns to 0 (viscous.jl:202-220).  Returns 0, a negative SFS code, 1 when
+ * a reset was due with rbf_itmax == 0 (not performed), or 2 when the reset's
+ * CG did not converge with rbf_iterror set (the step is completed). */
 int vpmo_rk3_step(double* P, int64_t np, vpmo_settings* s, double dt, int relax_now);
 
 /* adjoint ------------------------------------------------------------------ */
@@ -121,6 +127,12 @@ int vpmo_core_spreading_reset(double* P, int64_t np, int kernel, double sgm0, in
  * included) <- sum over all particles j (self included) of
  * Gamma_j zeta(r/sigma_j)/sigma_j^3. */
 void vpmo_zeta_direct(double* P, int64_t np, int kernel, int nthreads);
+/* zeta_fmm's pair loop (viscous.jl:523-558) over a direct list: per
+ * (target range, source range), J[1:3] of each target += Gamma_j
+ * zeta(r/sigma_j)/sigma_j^3 over the sources; no reset, no static skip.
+ * 0-based half-open ranges, rows of stride. */
+void vpmo_zeta_ranges(double* P, int64_t stride, int kernel, const int64_t* tb, const int64_t* te,
+                      const int64_t* sb, const int64_t* se, int64_t npairs);
 /* Pullback of update_particle_states (ReformulatedVPM, no SFS;
  * timeintegration.jl:467-528, rrules.jl:405-555).  B: cotangent records
  * (46 rows per particle) of the updated state in, of the state before the

@@ -65,3 +65,60 @@ def test_zeta_direct_pfield_mirror(vpm, orc):
     R = P.copy()
     orc.zeta_direct(R, "gaussianerf")
     np.testing.assert_allclose(pf.particles[:300, 15:18], R[:, 15:18], rtol=1e-12, atol=1e-12)
+
+
+def _leaf_lists(n, nleaves, seed, density=0.4):
+    rng = np.random.default_rng(seed)
+    cuts = np.sort(np.concatenate([[0, n], rng.choice(np.arange(1, n), nleaves - 1, replace=False)]))
+    leaves = list(zip(cuts[:-1], cuts[1:]))
+    tb, te, sb, se = [], [], [], []
+    for (a, b) in leaves:
+        for (c, d) in leaves:
+            if rng.random() < density:
+                tb.append(a); te.append(b); sb.append(c); se.append(d)
+    return tb, te, sb, se
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("nleaves,extra", [(21, False), (21, True), (4, False)])
+def test_zeta_ranges_vs_oracle(vpm, orc, kernel, nleaves, extra):
+    """zeta_fmm's pair loop over a direct list (vpm_zeta_ranges): leaf lists
+    of ~43 targets (one wave per leaf) or ~225 (256-lane blocks), with
+    overlapping and repeated target ranges when `extra`; J[1:3] accumulates
+    onto a nonzero start and static particles are not skipped.  F64 within
+    FP64 rounding of the oracle; F32C within ZETA_SCALED_TOL of the
+    summation scale (the same sums with |Gamma| and |J start|)."""
+    n = 900
+    P = _cloud(vpm, n, seed=70 + nleaves)
+    P[:, 15:18] = np.random.default_rng(71).normal(0, 1e-2, (n, 3))
+    tb, te, sb, se = _leaf_lists(n, nleaves, seed=72 + nleaves)
+    if extra:
+        tb += [5, 100, 100]; te += [300, 120, 120]; sb += [0, 40, 40]; se += [50, 41, 41]
+    R = P.copy()
+    orc.zeta_ranges(R, kernel, tb, te, sb, se)
+    S = P.copy()
+    S[:, 3:6] = np.abs(S[:, 3:6])
+    S[:, 15:18] = np.abs(S[:, 15:18])
+    orc.zeta_ranges(S, kernel, tb, te, sb, se)
+    for prec in ("f64", "f32c"):
+        A = P.copy()
+        vpm.api.zeta_ranges(A, tb, te, sb, se, kernel=kernel, precision=prec)
+        assert np.array_equal(A[:, 0:15], P[:, 0:15]) and np.array_equal(A[:, 18:], P[:, 18:])
+        scaled = (np.abs(A[:, 15:18] - R[:, 15:18]) / np.maximum(S[:, 15:18], 1e-300)).max()
+        _say("zeta_ranges", kernel, nleaves, extra, prec, "scaled max %.2e" % scaled)
+        if prec == "f64" or kernel == "singular":
+            assert scaled < 1e-12, scaled
+        else:
+            assert scaled < ZETA_SCALED_TOL, scaled
+
+
+def test_zeta_ranges_whole_field_equals_zeta_direct(vpm, orc):
+    """The single pair (all, all) onto zeroed J rows is zeta_direct."""
+    n = 1000
+    P = _cloud(vpm, n, seed=74)
+    P[:, 15:18] = 0.0
+    R = P.copy()
+    orc.zeta_direct(R, "gaussianerf")
+    A = P.copy()
+    vpm.api.zeta_ranges(A, [0], [n], [0], [n], kernel="gaussianerf", precision="f64")
+    np.testing.assert_allclose(A[:, 15:18], R[:, 15:18], rtol=1e-11, atol=1e-12 * np.abs(R[:, 15:18]).max())

@@ -400,6 +400,33 @@ def test_estr_ranges_matches_estr_direct(orc, vpm, kernel):
     np.testing.assert_allclose(D[:, 39:42], B[:, 39:42], rtol=0, atol=0)
 
 
+@pytest.mark.parametrize("kernel", ["gaussianerf", "gaussian", "winckelmans", "singular"])
+def test_zeta_ranges_matches_zeta_direct(orc, vpm, kernel):
+    """zeta_fmm's pair loop over the single pair (all, all) onto zeroed J
+    rows is zeta_direct (pinned to mpmath), bit for bit (same order); split
+    into sub-ranges it sums to the same; it accumulates (no reset) and skips
+    no static particle."""
+    P = _cloud(vpm, n=40, seed=14)
+    P[7] = P[8]
+    P[8, 3:6] *= -0.5                          # a coincident pair with its own strength
+    A = P.copy()
+    orc.zeta_direct(A, kernel)
+    B = P.copy()
+    B[:, 15:18] = 0.0
+    orc.zeta_ranges(B, kernel, [0], [40], [0], [40])
+    assert np.array_equal(B[:, 15:18], A[:, 15:18])
+    assert np.array_equal(B[:, 18:24], P[:, 18:24])
+    C = P.copy()
+    C[:, 15:18] = 0.0
+    orc.zeta_ranges(C, kernel, [0, 0, 25, 25], [25, 25, 40, 40], [0, 17, 0, 17], [17, 40, 17, 40])
+    np.testing.assert_allclose(C[:, 15:18], A[:, 15:18], rtol=1e-12, atol=1e-12 * np.abs(A[:, 15:18]).max())
+    D = P.copy()
+    D[3, 42] = 1.0
+    orc.zeta_ranges(D, kernel, [0], [40], [0], [40])
+    np.testing.assert_allclose(D[:, 15:18], P[:, 15:18] + A[:, 15:18], rtol=1e-14,
+                               atol=1e-14 * np.abs(A[:, 15:18]).max())
+
+
 def _lattice(vpm, m=4, h=0.1, sig=1.3, seed=3):
     rng = np.random.default_rng(seed)
     g = np.arange(m) * h
@@ -468,3 +495,27 @@ def test_core_spreading_reset_residual_and_static(orc, vpm):
     R3 = P.copy()
     rc, its, res = orc.core_spreading_reset(R3, "gaussianerf", 0.07, itmax=2)
     assert rc == 1 and its == 2
+
+
+def test_rk3_step_runs_core_spreading_reset(orc, vpm):
+    """When the core growth reaches beta in the last stage, the step resets
+    sigma to sgm0 and t_sgm to 0 (viscous.jl:202-220), and the new strengths
+    reproduce the zeta field of the grown cores to the CG tolerance; with
+    rbf_itmax = 0 the reset is reported (status 1) and not performed."""
+    P = _lattice(vpm, sig=0.7)
+    sg0 = float(P[0, 6])
+    s = orc.make_settings(kernel="gaussianerf", relax=0, viscous=1, nu=1e-3, sgm0=sg0, beta=1.001,
+                          rbf_itmax=50)
+    R = P.copy()
+    assert orc.rk3_step(R, s, 0.01, relax_now=False) == 0
+    assert s.t_sgm == 0.0 and np.all(R[:, 6] == sg0)
+    omega = R[:, 33:36].copy()                 # target: zeta field of the grown cores (M[7:9])
+    S = R.copy()
+    orc.zeta_direct(S, "gaussianerf")
+    for c in range(3):
+        assert np.linalg.norm(S[:, 15 + c] - omega[:, c]) <= 1.5e-3 * np.linalg.norm(omega[:, c])
+    s0 = orc.make_settings(kernel="gaussianerf", relax=0, viscous=1, nu=1e-3, sgm0=sg0, beta=1.001,
+                           rbf_itmax=0)
+    R0 = P.copy()
+    assert orc.rk3_step(R0, s0, 0.01, relax_now=False) == 1
+    assert s0.t_sgm == pytest.approx(0.01) and np.all(R0[:, 6] > sg0)
