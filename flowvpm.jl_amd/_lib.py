@@ -52,6 +52,8 @@ _SIGS = {
     "vpm_destroy": (C.c_int, [C.c_void_p]),
     "vpm_set_particles": (C.c_int, [C.c_void_p, _dptr, C.c_int64]),
     "vpm_get_particles": (C.c_int, [C.c_void_p, _dptr, C.c_int64, C.c_int32, C.c_int32]),
+    "vpm_add_particles": (C.c_int, [C.c_void_p, _dptr, C.c_int64]),
+    "vpm_remove_particle": (C.c_int, [C.c_void_p, C.c_int64]),
     "vpm_get_np": (C.c_int, [C.c_void_p, _i64p]),
     "vpm_synchronize": (C.c_int, [C.c_void_p]),
     "vpm_uj_direct": (C.c_int, [C.c_void_p, C.POINTER(Config), C.c_uint32]),

@@ -153,6 +153,13 @@ class Field:
         assert P.ndim == 2 and P.shape[1] == NFIELDS
         check(lib().vpm_set_particles(self.handle, P.ctypes.data_as(_lib._dptr), P.shape[0]))
 
+    def add_particles(self, P):
+        P = np.ascontiguousarray(P, dtype=np.float64).reshape(-1, NFIELDS)
+        check(lib().vpm_add_particles(self.handle, P.ctypes.data_as(_lib._dptr), P.shape[0]))
+
+    def remove_particle(self, i):
+        check(lib().vpm_remove_particle(self.handle, int(i)))
+
     def get_particles(self, out, rows=(0, NFIELDS), np_=None):
         assert out.dtype == np.float64 and out.flags.c_contiguous and out.shape[1] == NFIELDS
         n = out.shape[0] if np_ is None else np_
@@ -526,17 +533,37 @@ def nextstep(pfield, dt, *, relax=False, resident=False):
     pfield.nt += 1
 
 
-def run_vpm(pfield, dt, nsteps, *, runtime_function=None, resident=True):
+def run_vpm(pfield, dt, nsteps, *, runtime_function=None, static_particles_function=None, resident=True):
     """The stepping loop of run_vpm! (FLOWVPM_utils.jl:86-119): relaxation
-    every nsteps_relax steps, starting at step 1.  State stays on the device
-    between steps when resident=True."""
-    if resident and pfield.np:
+    every nsteps_relax steps, starting at step 1; before each step
+    static_particles_function(pfield, t, dt) may append particles, which are
+    removed after the step unless it returns False (:100-112); then
+    runtime_function(pfield, t, dt) ends the run by returning True.  The
+    viscous scheme must be compatible with the kernel (:56-64,
+    FLOWVPM.jl:259-271).  State stays
+    on the device between steps when resident=True: appended particles are
+    uploaded with vpm_add_particles and removed with vpm_remove_particle."""
+    if isinstance(pfield.viscous, CoreSpreading) and pfield.kernel != "gaussianerf":
+        raise VPMError(_lib.VPM_EINVAL, "Kernel %s is not compatible with viscous scheme CoreSpreading; "
+                       "compatible kernels are ['gaussianerf']" % pfield.kernel)
+    if resident:
         pfield_upload(pfield)
     for i in range(0, nsteps + 1):
         rl = pfield.relaxation
         relax = rl.method != 0 and rl.nsteps_relax >= 1 and i > 0 and i % rl.nsteps_relax == 0
+        org_np = pfield.np
         if i != 0:
+            remove = None
+            if static_particles_function is not None:
+                remove = static_particles_function(pfield, pfield.t, dt)
+                if resident and pfield.np > org_np:
+                    pfield.device_field().add_particles(pfield.particles[org_np:pfield.np])
             nextstep(pfield, dt, relax=relax, resident=resident)
+            if remove is None or remove:
+                for pi in range(pfield.np - 1, org_np - 1, -1):
+                    remove_particle(pfield, pi)
+                    if resident:
+                        pfield.device_field().remove_particle(pi)
         if runtime_function is not None:
             if resident and pfield.np:
                 pfield_download(pfield)

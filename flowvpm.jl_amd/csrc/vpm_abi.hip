@@ -255,6 +255,51 @@ int vpm_set_particles(vpm_field f, const double* particles, int64_t np)
     return sync(f);
 }
 
+int vpm_add_particles(vpm_field f, const double* particles, int64_t n)
+{
+    VPM_FIELD_GUARD(f);
+    if (n < 0 || (n > 0 && !particles)) return fail(VPM_EINVAL, "bad particle array");
+    if (f->np + n > f->cap)
+        return fail(VPM_ERANGE, "PARTICLE OVERFLOW. Max number of particles " + std::to_string(f->cap) +
+                                    " has been reached");
+    for (int64_t i = 0; i < n; ++i) {
+        const double* p = particles + i * NF;
+        for (int k = 0; k < 7; ++k)
+            if (!std::isfinite(p[k]))
+                return fail(VPM_EINVAL, "non-finite X/Gamma/sigma at particle " + std::to_string(i));
+        if (!(p[RSIG] > 0)) return fail(VPM_EINVAL, "sigma <= 0 at particle " + std::to_string(i));
+    }
+    if (n == 0) return VPM_OK;
+    hipError_t e = hipMemcpyAsync(f->P + f->np * NF, particles, sizeof(double) * NF * (size_t)n,
+                                  hipMemcpyHostToDevice, f->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(H2D)");
+    f->np += n;
+    f->perm_valid = false;
+    return sync(f);
+}
+
+int vpm_remove_particle(vpm_field f, int64_t i)
+{
+    VPM_FIELD_GUARD(f);
+    if (i < 0) return fail(VPM_EINVAL, "Requested removal of invalid particle index " + std::to_string(i));
+    if (i >= f->np)
+        return fail(VPM_EINVAL, "Requested removal of particle " + std::to_string(i) + ", but there is only " +
+                                    std::to_string(f->np) + " particles in the field.");
+    const int64_t last = f->np - 1;
+    hipError_t e;
+    if (i != last) {   // particlefield.jl:409-412
+        e = hipMemcpyAsync(f->P + i * NF, f->P + last * NF, sizeof(double) * NF, hipMemcpyDeviceToDevice,
+                           f->stream);
+        if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(D2D)");
+    }
+    // _reset_particle (particlefield.jl:485-490): U, vorticity, J, PSE rows
+    e = hipMemsetAsync(f->P + last * NF + RU, 0, sizeof(double) * (RM - RU), f->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+    f->np = last;
+    f->perm_valid = false;
+    return sync(f);
+}
+
 int vpm_get_particles(vpm_field f, double* particles, int64_t np, int32_t row0, int32_t row1)
 {
     VPM_FIELD_GUARD(f);

@@ -97,6 +97,14 @@ int vpm_set_particles(vpm_field f, const double* particles, int64_t np);
 /* Copy rows [row0, row1) of the first np records out, into a 46-stride array
  * (other rows of `particles` are left untouched). */
 int vpm_get_particles(vpm_field f, double* particles, int64_t np, int32_t row0, int32_t row1);
+/* add_particle (particlefield.jl:167-205) for n whole 46-row records, appended
+ * after the current ones; the checks of vpm_set_particles apply, and the
+ * maxparticles overflow is VPM_ERANGE. */
+int vpm_add_particles(vpm_field f, const double* particles, int64_t n);
+/* remove_particle (particlefield.jl:401-419), 0-based i: the last particle
+ * moves into slot i, the last slot's U, vorticity, J and PSE are zeroed,
+ * np decreases by one. */
+int vpm_remove_particle(vpm_field f, int64_t i);
 int vpm_get_np(vpm_field f, int64_t* np);
 int vpm_synchronize(vpm_field f);
 

@@ -191,3 +191,50 @@ def test_adjoint_golden(vpm):
         np.testing.assert_allclose(Xb, ref["Xbar"], rtol=1e-9, atol=1e-12 * np.abs(ref["Xbar"]).max())
         np.testing.assert_allclose(Gb, ref["Gbar"], rtol=1e-9, atol=1e-12 * np.abs(ref["Gbar"]).max())
         np.testing.assert_allclose(Sb, ref["Sbar"], rtol=1e-9, atol=1e-12 * max(np.abs(ref["Sbar"]).max(), 1e-30))
+
+
+@pytest.mark.parametrize("resident", [True, False])
+def test_run_vpm_static_particles_and_relax_cadence_vs_oracle(vpm, orc, resident):
+    """run_vpm! (utils.jl:86-119): static particles appended before each
+    step and removed after it (remove_particle from the end, :106-111),
+    relaxation every nsteps_relax = 2 steps from step 1, and a runtime
+    function that sees the field after each step and ends the run.  The
+    oracle replays the same sequence; F64 state to 1e-10."""
+    api = vpm.api
+    P = vpm.workloads.ring_cfg0()[::8].copy()          # 1,000 particles
+    n = P.shape[0]
+    rng = np.random.default_rng(40)
+    rl = api.Relaxation(1, 2, 0.3)
+    pf = api.ParticleField(n + 8, kernel="gaussianerf", formulation=api.rVPM, relaxation=rl, precision="f64")
+    api.add_particles(pf, P)
+    dt = 0.01
+
+    def statics(k):
+        X = P[rng.integers(0, n, 3), 0:3] + 0.05
+        S = vpm.fields.make_particles(X, np.full((3, 3), 1e-3 * (k + 1)), np.full(3, float(P[0, 6])))
+        S[:, 42] = 1.0
+        return S
+
+    added = [statics(k) for k in range(4)]
+    seen = []
+
+    def static_fn(pfield, t, dt_):
+        api.add_particles(pfield, added[pfield.nt])
+        return None
+
+    def runtime(pfield, t, dt_):
+        seen.append((pfield.nt, pfield.np))
+        return pfield.nt == 3
+
+    api.run_vpm(pf, dt, 10, runtime_function=runtime, static_particles_function=static_fn, resident=resident)
+    assert seen == [(0, n), (1, n), (2, n), (3, n)]
+    s = orc.make_settings(kernel="gaussianerf", f=0.0, g=0.2, relax=1, rlxf=0.3)
+    R = P.copy()
+    for i in range(1, 4):
+        Q = np.ascontiguousarray(np.vstack([R, added[i - 1]]))
+        orc.rk3_step(Q, s, dt, relax_now=(i % 2 == 0))
+        R = Q[:n].copy()
+    e = _state_err(vpm, pf.particles[:pf.np], R)
+    _say("run_vpm statics resident=%s" % resident, e)
+    assert max(e.values()) < 1e-10, e
+    assert np.all(pf.particles[n:n + 3, 9:27] == 0.0)   # removed slots reset
