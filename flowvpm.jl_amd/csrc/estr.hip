@@ -21,7 +21,7 @@
 // storage order, contraction off -- the GPU oracle for E_str.
 #include <cfloat>
 
-#include "vpm_internal.h"
+#include "p2p_pair.h"
 
 namespace vpm {
 
@@ -98,13 +98,14 @@ __device__ __forceinline__ void split_rel_e(double x, double o, float& hi, float
 template <int K>
 __global__ __launch_bounds__(kBlock) void estr_f32c_kernel(
     const double* __restrict__ pos, const double* __restrict__ J, int64_t ld,
-    const float4* __restrict__ gz, const float2* __restrict__ bc, int64_t np, int transposed,
-    int64_t chunk, double* __restrict__ part, int64_t pld)
+    const float4* __restrict__ gz, const float2* __restrict__ bc, const double* __restrict__ tbox, int64_t np,
+    int transposed, int64_t chunk, double* __restrict__ part, int64_t pld)
 {
     __shared__ float4 lp[kTile];   // s'_hi x, y, z, a
     __shared__ float4 lg[kTile];   // Gamma x, y, z, b
     __shared__ float4 lq[kTile];   // J'^T Gamma (or J' Gamma), cut
     __shared__ float4 ll[kTile];   // s'_lo x, y, z
+    __shared__ float wb[kBlock / 64][6];
 
     const int tid = threadIdx.x;
     const int64_t i = (int64_t)blockIdx.x * kBlock + tid;
@@ -118,6 +119,9 @@ __global__ __launch_bounds__(kBlock) void estr_f32c_kernel(
     split_rel_e(pos[it], ox, tx, tlx);
     split_rel_e(pos[ld + it], oy, ty, tly);
     split_rel_e(pos[2 * ld + it], oz, tz, tlz);
+    float blo[3], bhi[3];
+    block_box(wave_box(tx, ty, tz), wb, blo, bhi);
+    const double org[3] = {ox, oy, oz};
 
     const int64_t sb = (int64_t)blockIdx.y * chunk;
     const int64_t se = min(sb + chunk, np);
@@ -125,7 +129,11 @@ __global__ __launch_bounds__(kBlock) void estr_f32c_kernel(
 #pragma unroll
     for (int c = 0; c < 6; ++c) { sum[c] = 0.f; err[c] = 0.f; }
 
-    for (int64_t base = sb; base < se; base += kTile) {
+    __shared__ int nlist[kBlock];
+    __shared__ int ncnt[kBlock / 64];
+    NearTiles near(tbox, sb / kTile, (se + kTile - 1) / kTile, org, blo, bhi, 1.0f, nlist, ncnt);
+    for (int64_t tile; (tile = near.next()) >= 0;) {
+        const int64_t base = tile * kTile;
         const int64_t j = base + tid;
         float4 p, g, q, l;
         if (j < se) {
@@ -335,20 +343,24 @@ int launch_estr_impl(vpm_field f, int kernel, int transposed, int precision, con
         return VPM_OK;
     }
     const int64_t ld = f->cap;
-    // [3][ld] positions, [9][ld] J (double), [ld] float4, [ld] float2
+    // [3][ld] positions, [9][ld] J (double), [ld] float4, [ld] float2, tile boxes
+    const int64_t ntile = (np + kTile - 1) / kTile;
     if ((rc = ensure(f->adj, sizeof(double) * 12 * (size_t)ld + sizeof(float4) * (size_t)ld +
-                                 sizeof(float2) * (size_t)ld)))
+                                 sizeof(float2) * (size_t)ld + 64 * (size_t)ntile)))
         return rc;
     double* pos = (double*)f->adj.p;
     double* J = pos + 3 * ld;
     float4* gz = (float4*)(J + 9 * ld);
     float2* bc = (float2*)(gz + ld);
+    double* tbox = (double*)(bc + ld);
     switch (kernel) {
 #define VPM_PE(KID) case KID: hipLaunchKernelGGL(pack_estr_kernel<KID>, dim3(nblk(np)), dim3(kBlock), 0, f->stream, f->P, np, perm, pos, J, ld, gz, bc); break;
         VPM_PE(VPM_KERNEL_GAUSSIAN) VPM_PE(VPM_KERNEL_GAUSSIANERF) VPM_PE(VPM_KERNEL_WINCKELMANS)
 #undef VPM_PE
     default: return fail(VPM_EINVAL, "unknown kernel id");
     }
+    hipLaunchKernelGGL(tile_box_kernel<2>, dim3((unsigned)ntile), dim3(kBlock), 0, f->stream, (const double*)pos, ld,
+                       (const float*)bc + 1, np, tbox);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "pack_estr");
     // source splits, as for U/J
@@ -362,7 +374,7 @@ int launch_estr_impl(vpm_field f, int kernel, int transposed, int precision, con
     double* part = (double*)f->part.p;
     timing_begin(f);
     switch (kernel) {
-#define VPM_ES(KID) case KID: hipLaunchKernelGGL(estr_f32c_kernel<KID>, dim3(nblk(np), (unsigned)K), dim3(kBlock), 0, f->stream, pos, J, ld, gz, bc, np, transposed, chunk, part, np); break;
+#define VPM_ES(KID) case KID: hipLaunchKernelGGL(estr_f32c_kernel<KID>, dim3(nblk(np), (unsigned)K), dim3(kBlock), 0, f->stream, pos, J, ld, gz, bc, (const double*)tbox, np, transposed, chunk, part, np); break;
         VPM_ES(VPM_KERNEL_GAUSSIAN) VPM_ES(VPM_KERNEL_GAUSSIANERF) VPM_ES(VPM_KERNEL_WINCKELMANS)
 #undef VPM_ES
     default: return fail(VPM_EINVAL, "unknown kernel id");

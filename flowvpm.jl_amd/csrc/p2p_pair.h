@@ -434,4 +434,138 @@ __host__ __device__ __forceinline__ float zeta_cut(double sigma)
     return FLT_MAX;
 }
 
+// ---- source tiles beyond every cut ----
+// The near-field-only kernels (zeta, E_str, the SFS differences) drop a
+// source when its FP32 hi distance to the target (or to the wave's target
+// box) is at least its cut.  A block skips a whole tile of kTile sorted
+// sources, without loading it, when the FP32 gap between the block's target
+// box and the tile's source box is already at least the tile's largest cut.
+// Both boxes bound the FP32 hi coordinates the per-source tests use (x - o
+// is rounded monotonically, and the double box extremes are attained), and
+// every operation of the gap is monotone in those coordinates, so the gap
+// never exceeds a tested distance: a skipped tile holds only sources the
+// per-source test would drop, and results are unchanged bit for bit.
+
+// FP64 bounding box and largest cut of each tile of kTile consecutive
+// sources: tb[8 t + 0..2] = lo, [3..5] = hi, [6] = max cut[j * CS].
+template <int CS>
+__global__ __launch_bounds__(kBlock) void tile_box_kernel(const double* __restrict__ pos, int64_t ld,
+                                                          const float* __restrict__ cut, int64_t np,
+                                                          double* __restrict__ tb)
+{
+    static_assert(kTile == kBlock, "one thread per source of a tile");
+    __shared__ double red[7][kBlock];
+    const int tid = threadIdx.x;
+    const int64_t j = (int64_t)blockIdx.x * kTile + tid;
+    if (j < np) {
+        for (int c = 0; c < 3; ++c) red[c][tid] = red[3 + c][tid] = pos[c * ld + j];
+        red[6][tid] = (double)cut[j * CS];
+    } else {
+        for (int c = 0; c < 3; ++c) { red[c][tid] = DBL_MAX; red[3 + c][tid] = -DBL_MAX; }
+        red[6][tid] = 0.0;
+    }
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (tid < w) {
+            for (int c = 0; c < 3; ++c) {
+                red[c][tid] = fmin(red[c][tid], red[c][tid + w]);
+                red[3 + c][tid] = fmax(red[3 + c][tid], red[3 + c][tid + w]);
+            }
+            red[6][tid] = fmax(red[6][tid], red[6][tid + w]);
+        }
+        __syncthreads();
+    }
+    if (tid < 7) tb[8 * (int64_t)blockIdx.x + tid] = red[tid][0];
+}
+
+// Union of the wave boxes of a block, in every thread.  wb is __shared__
+// float[kBlock / 64][6]; contains a barrier (call from every thread).
+__device__ __forceinline__ void block_box(const WaveBox& box, float (*wb)[6], float (&blo)[3], float (&bhi)[3])
+{
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int c = 0; c < 3; ++c) { wb[wave][c] = box.lo[c]; wb[wave][3 + c] = box.hi[c]; }
+    __syncthreads();
+    for (int c = 0; c < 3; ++c) {
+        blo[c] = wb[0][c];
+        bhi[c] = wb[0][3 + c];
+        for (int w = 1; w < kBlock / 64; ++w) {
+            blo[c] = fminf(blo[c], wb[w][c]);
+            bhi[c] = fmaxf(bhi[c], wb[w][3 + c]);
+        }
+    }
+}
+
+// Squared FP32 gap between the block's target box and tile box T, both
+// relative to the block origin org (same evaluation order as the per-source
+// distances: x, then y, then z by fma).
+__device__ __forceinline__ float tile_gap2(const double* __restrict__ T, const double (&org)[3],
+                                           const float (&blo)[3], const float (&bhi)[3])
+{
+    float g2 = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float lo = (float)(T[c] - org[c]);
+        const float hi = (float)(T[3 + c] - org[c]);
+        const float g = fmaxf(fmaxf(blo[c] - hi, lo - bhi[c]), 0.f);
+        g2 = c == 0 ? g * g : fmaf(g, g, g2);
+    }
+    return g2;
+}
+
+// The near tiles of [t0, t1) for a block, in ascending order: kBlock tiles
+// are tested at once (one per thread, independent loads), the near ones are
+// compacted into the __shared__ list, and next() walks the list.  A tile is
+// near when tile_gap2 < its largest cut times margin (the margin of the
+// kernel's per-source test).  All threads must call next() together: it
+// holds barriers and returns the same tile (or -1 at the end) to each.
+struct NearTiles {
+    const double* tbox;
+    int64_t c0, t1;
+    const double* org;
+    const float* blo;
+    const float* bhi;
+    float margin;
+    int* list;      // __shared__ int[kBlock]
+    int* wcnt;      // __shared__ int[kBlock / 64]
+    int cnt = 0, k = 0;
+    int64_t cbase = 0;
+
+    __device__ NearTiles(const double* tb, int64_t first, int64_t end, const double (&o)[3], const float (&lo)[3],
+                         const float (&hi)[3], float m, int* l, int* w)
+        : tbox(tb), c0(first), t1(end), org(o), blo(lo), bhi(hi), margin(m), list(l), wcnt(w) {}
+
+    __device__ int64_t next()
+    {
+        while (k >= cnt) {
+            if (c0 >= t1) return -1;
+            const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+            const int64_t t = c0 + tid;
+            bool nr = false;
+            if (t < t1) {
+                const double* T = tbox + 8 * t;
+                const double o3[3] = {org[0], org[1], org[2]};
+                const float l3[3] = {blo[0], blo[1], blo[2]}, h3[3] = {bhi[0], bhi[1], bhi[2]};
+                nr = tile_gap2(T, o3, l3, h3) < (float)T[6] * margin;
+            }
+            const uint64_t m = __ballot(nr);
+            __syncthreads();   // readers of the previous list are done
+            if (lane == 0) wcnt[wave] = __popcll(m);
+            __syncthreads();
+            int off = 0, tot = 0;
+            for (int w = 0; w < kBlock / 64; ++w) {
+                off += w < wave ? wcnt[w] : 0;
+                tot += wcnt[w];
+            }
+            if (nr) list[off + __popcll(m & ((1ull << lane) - 1ull))] = tid;
+            __syncthreads();
+            cbase = c0;
+            c0 += kBlock;
+            cnt = tot;
+            k = 0;
+        }
+        return cbase + list[k++];
+    }
+};
+
 }  // namespace vpm

@@ -136,20 +136,22 @@ __global__ __launch_bounds__(kBlock) void rbf_final_kernel(double* __restrict__ 
     for (int c = 0; c < 3; ++c) p[RG + c] = p[RM + c];
 }
 
-// sum of the block partials in block order (one thread per component)
-__global__ void rbf_reduce_kernel(const double* __restrict__ part, int nb, double* __restrict__ out)
+// sum of the block partials in a fixed order: thread t adds partials
+// t, t + kBlock, ... in turn, then a fixed tree over the threads
+__global__ __launch_bounds__(kBlock) void rbf_reduce_kernel(const double* __restrict__ part, int nb,
+                                                            double* __restrict__ out)
 {
-    const int c = threadIdx.x;
-    if (c >= 3) return;
-    double s = 0.0;
-    for (int b = 0; b < nb; ++b) s += part[3 * b + c];
-    out[c] = s;
+    const int t = threadIdx.x;
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int b = t; b < nb; b += kBlock)
+        for (int c = 0; c < 3; ++c) s[c] += part[3 * b + c];
+    block_sum3(s[0], s[1], s[2], out);   // one block: out[0..2]
 }
 #pragma clang fp contract(on)
 
 static int reduce3(vpm_field f, double* part, int nb, double* dout, double out[3])
 {
-    hipLaunchKernelGGL(rbf_reduce_kernel, dim3(1), dim3(64), 0, f->stream, (const double*)part, nb, dout);
+    hipLaunchKernelGGL(rbf_reduce_kernel, dim3(1), dim3(kBlock), 0, f->stream, (const double*)part, nb, dout);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "rbf_reduce");
     if ((e = hipMemcpyAsync(out, dout, 3 * sizeof(double), hipMemcpyDeviceToHost, f->stream)) != hipSuccess)

@@ -101,12 +101,14 @@ __device__ __forceinline__ void zeta_and_rate(float r2, float s, float& z, float
 template <int K>
 __global__ __launch_bounds__(kBlock) void dj_f32c_kernel(const double* __restrict__ pos, int64_t ld,
                                                          const float4* __restrict__ sgs,
-                                                         const float* __restrict__ scut, int64_t np, float delta,
+                                                         const float* __restrict__ scut,
+                                                         const double* __restrict__ tbox, int64_t np, float delta,
                                                          double* __restrict__ dj)
 {
     __shared__ float4 lpos[kTile];
     __shared__ float4 lgs[kTile];
     __shared__ float4 llo[kTile];
+    __shared__ float wb[kBlock / 64][6];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int64_t i = (int64_t)blockIdx.x * kBlock + tid;
@@ -118,11 +120,18 @@ __global__ __launch_bounds__(kBlock) void dj_f32c_kernel(const double* __restric
     split_rel(pos[ld + it], oy, ty, tly);
     split_rel(pos[2 * ld + it], oz, tz, tlz);
     const WaveBox box = wave_box(tx, ty, tz);
+    float blo[3], bhi[3];
+    block_box(box, wb, blo, bhi);
+    const double org[3] = {ox, oy, oz};
     constexpr int NA = 11;
     float sum[NA], err[NA];
 #pragma unroll
     for (int c = 0; c < NA; ++c) { sum[c] = 0.f; err[c] = 0.f; }
-    for (int64_t base = 0; base < np; base += kTile) {
+    __shared__ int nlist[kBlock];
+    __shared__ int ncnt[kBlock / 64];
+    NearTiles near(tbox, 0, (np + kTile - 1) / kTile, org, blo, bhi, 1.0001f, nlist, ncnt);
+    for (int64_t tile; (tile = near.next()) >= 0;) {
+        const int64_t base = tile * kTile;
         const int n = (int)min((int64_t)kTile, np - base);
         __syncthreads();
         if (tid < n) {
@@ -251,14 +260,15 @@ template <int K>
 __global__ __launch_bounds__(kBlock) void de_f32c_kernel(const double* __restrict__ pos, const double* __restrict__ J,
                                                          const double* __restrict__ dj, int64_t ld,
                                                          const float4* __restrict__ gz, const float* __restrict__ scut,
-                                                         int64_t np, float alpha, int transposed,
-                                                         double* __restrict__ out)
+                                                         const double* __restrict__ tbox, int64_t np, float alpha,
+                                                         int transposed, double* __restrict__ out)
 {
     __shared__ float4 lp[kTile];    // s'_hi, sigma (0: static)
     __shared__ float4 lg[kTile];    // Gamma, cut
     __shared__ float4 l1[kTile];    // S(J'_j, Gamma_j)
     __shared__ float4 l2[kTile];    // S(dJ'_j, Gamma_j)
     __shared__ float4 ll[kTile];    // s'_lo
+    __shared__ float wb[kBlock / 64][6];
     const int tid = threadIdx.x;
     const int64_t i = (int64_t)blockIdx.x * kBlock + tid;
     const int64_t ib = (int64_t)blockIdx.x * kBlock;
@@ -271,11 +281,18 @@ __global__ __launch_bounds__(kBlock) void de_f32c_kernel(const double* __restric
     split_rel(pos[it], ox, tx, tlx);
     split_rel(pos[ld + it], oy, ty, tly);
     split_rel(pos[2 * ld + it], oz, tz, tlz);
+    float blo[3], bhi[3];
+    block_box(wave_box(tx, ty, tz), wb, blo, bhi);
+    const double org[3] = {ox, oy, oz};
     float sum[12], err[12];
 #pragma unroll
     for (int c = 0; c < 12; ++c) { sum[c] = 0.f; err[c] = 0.f; }
     const float oma = 1.f - alpha;
-    for (int64_t base = 0; base < np; base += kTile) {
+    __shared__ int nlist[kBlock];
+    __shared__ int ncnt[kBlock / 64];
+    NearTiles near(tbox, 0, (np + kTile - 1) / kTile, org, blo, bhi, 1.0f, nlist, ncnt);
+    for (int64_t tile; (tile = near.next()) >= 0;) {
+        const int64_t base = tile * kTile;
         const int n = (int)min((int64_t)kTile, np - base);
         __syncthreads();
         if (tid < n) {
@@ -420,9 +437,11 @@ int launch_sfs_delta(vpm_field f, int kernel, int transposed, double alpha)
     int rc;
     const uint32_t* perm = nullptr;
     if (np > 1 && (rc = sort_field(f, perm))) return rc;
-    // scratch: pos 3, J 9, dJ 9, dE 3 doubles; gs, gz float4; cut float
+    // scratch: pos 3, J 9, dJ 9, dE 3 doubles; gs, gz float4; tile boxes; cut float
     const size_t nd = 24 * (size_t)np;
-    const size_t bytes = sizeof(double) * nd + (2 * sizeof(float4) + sizeof(float)) * (size_t)np + 64;
+    const int64_t ntile = (np + kTile - 1) / kTile;
+    const size_t bytes = sizeof(double) * nd + (2 * sizeof(float4) + sizeof(float)) * (size_t)np +
+                         64 * (size_t)ntile + 64;
     if ((rc = ensure(f->adj, bytes))) return rc;
     double* pos = (double*)f->adj.p;
     double* J = pos + 3 * np;
@@ -430,7 +449,8 @@ int launch_sfs_delta(vpm_field f, int kernel, int transposed, double alpha)
     double* de = dj + 9 * np;
     float4* gs = (float4*)(de + 3 * np);
     float4* gz = gs + np;
-    float* cut = (float*)(gz + np);
+    double* tbox = (double*)(gz + np);
+    float* cut = (float*)(tbox + 8 * ntile);
     const unsigned nb = (unsigned)((np + kBlock - 1) / kBlock);
     const float delta = (float)(1.0 / alpha - 1.0);
     switch (kernel) {
@@ -438,11 +458,13 @@ int launch_sfs_delta(vpm_field f, int kernel, int transposed, double alpha)
     case KID:                                                                                                    \
         hipLaunchKernelGGL(pack_sfsd_kernel<KID>, dim3(nb), dim3(kBlock), 0, f->stream, f->P, np, perm, pos, J, ld, \
                            gs, cut, gz);                                                                         \
+        hipLaunchKernelGGL(tile_box_kernel<1>, dim3((unsigned)ntile), dim3(kBlock), 0, f->stream,                \
+                           (const double*)pos, ld, (const float*)cut, np, tbox);                               \
         hipLaunchKernelGGL(dj_f32c_kernel<KID>, dim3(nb), dim3(kBlock), 0, f->stream, (const double*)pos, ld,     \
-                           (const float4*)gs, (const float*)cut, np, delta, dj);                                 \
+                           (const float4*)gs, (const float*)cut, (const double*)tbox, np, delta, dj);            \
         hipLaunchKernelGGL(de_f32c_kernel<KID>, dim3(nb), dim3(kBlock), 0, f->stream, (const double*)pos,         \
-                           (const double*)J, (const double*)dj, ld, (const float4*)gz, (const float*)cut, np,     \
-                           (float)alpha, transposed, de);                                                        \
+                           (const double*)J, (const double*)dj, ld, (const float4*)gz, (const float*)cut,         \
+                           (const double*)tbox, np, (float)alpha, transposed, de);                               \
         break;
         VPM_SD(VPM_KERNEL_GAUSSIAN) VPM_SD(VPM_KERNEL_GAUSSIANERF) VPM_SD(VPM_KERNEL_WINCKELMANS)
 #undef VPM_SD

@@ -12,7 +12,9 @@
 // b exp2(a r^3) (Gaussian), b (1 + a r^2)^-3.5 (Winckelmans).  Sources whose
 // distance to the wave's target box exceeds q_far sigma (Gaussian-erf 7,
 // Gaussian 3: zeta below FP32 resolution of the self term) are skipped per
-// wave with one ballot per 64 sources; Winckelmans evaluates every pair.
+// wave with one ballot per 64 sources, and a block visits only the source
+// tiles within reach of its targets (NearTiles, p2p_pair.h); Winckelmans
+// evaluates every pair.
 // The singular kernel's zeta is nonzero only at r == 0 and runs on the FP64
 // kernel.
 //
@@ -50,12 +52,14 @@ __global__ __launch_bounds__(kBlock) void pack_zeta_kernel(const double* __restr
 template <int K>
 __global__ __launch_bounds__(kBlock) void zeta_f32c_kernel(const double* __restrict__ pos, int64_t ld,
                                                            const float4* __restrict__ gz,
-                                                           const float2* __restrict__ bc, int64_t np,
+                                                           const float2* __restrict__ bc,
+                                                           const double* __restrict__ tbox, int64_t np,
                                                            int64_t chunk, double* __restrict__ part, int64_t pld)
 {
     __shared__ float4 lp[kTile];   // s'_hi x, y, z, cut
     __shared__ float4 lg[kTile];   // Gamma x, y, z, b
     __shared__ float4 ll[kTile];   // s'_lo x, y, z, a
+    __shared__ float wb[kBlock / 64][6];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -68,11 +72,18 @@ __global__ __launch_bounds__(kBlock) void zeta_f32c_kernel(const double* __restr
     split_rel(pos[ld + it], oy, ty, tly);
     split_rel(pos[2 * ld + it], oz, tz, tlz);
     const WaveBox box = wave_box(tx, ty, tz);
+    float blo[3], bhi[3];
+    block_box(box, wb, blo, bhi);
+    const double org[3] = {ox, oy, oz};
 
     const int64_t sb = (int64_t)blockIdx.y * chunk;
     const int64_t se = min(sb + chunk, np);
     float sum[3] = {0.f, 0.f, 0.f}, err[3] = {0.f, 0.f, 0.f};
-    for (int64_t base = sb; base < se; base += kTile) {
+    __shared__ int nlist[kBlock];
+    __shared__ int ncnt[kBlock / 64];
+    NearTiles near(tbox, sb / kTile, (se + kTile - 1) / kTile, org, blo, bhi, 1.0001f, nlist, ncnt);
+    for (int64_t tile; (tile = near.next()) >= 0;) {
+        const int64_t base = tile * kTile;
         const int64_t j = base + tid;
         const int n = (int)min((int64_t)kTile, se - base);
         __syncthreads();
@@ -245,18 +256,23 @@ int launch_zeta(vpm_field f, int kernel, int precision)
     } else {
         if (np > 1 && (rc = sort_field(f, perm))) return rc;
         if ((rc = ensure(f->pos, sizeof(double) * 3 * (size_t)ld))) return rc;
-        if ((rc = ensure(f->adj, (sizeof(float4) + sizeof(float2)) * (size_t)np + 16))) return rc;
+        const int64_t ntile = (np + kTile - 1) / kTile;
+        if ((rc = ensure(f->adj, (sizeof(float4) + sizeof(float2)) * (size_t)np + 64 * (size_t)ntile + 16)))
+            return rc;
         float4* gz = (float4*)f->adj.p;
         float2* bc = (float2*)(gz + np);
+        double* tbox = (double*)(bc + np);
         double* pos = (double*)f->pos.p;
         switch (kernel) {
 #define VPM_Z32(KID)                                                                                           \
     case KID:                                                                                                  \
         hipLaunchKernelGGL(pack_zeta_kernel<KID>, dim3(nbt), dim3(kBlock), 0, f->stream, f->P, np, perm, pos, ld, \
                            gz, bc);                                                                            \
+        hipLaunchKernelGGL(tile_box_kernel<2>, dim3((unsigned)ntile), dim3(kBlock), 0, f->stream,                \
+                           (const double*)pos, ld, (const float*)bc + 1, np, tbox);                            \
         timing_begin(f);                                                                                       \
         hipLaunchKernelGGL(zeta_f32c_kernel<KID>, grid, dim3(kBlock), 0, f->stream, (const double*)pos, ld,     \
-                           (const float4*)gz, (const float2*)bc, np, chunk, part, np);                        \
+                           (const float4*)gz, (const float2*)bc, (const double*)tbox, np, chunk, part, np);  \
         break;
             VPM_Z32(VPM_KERNEL_GAUSSIAN) VPM_Z32(VPM_KERNEL_GAUSSIANERF) VPM_Z32(VPM_KERNEL_WINCKELMANS)
 #undef VPM_Z32

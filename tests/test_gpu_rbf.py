@@ -6,10 +6,13 @@ arithmetic; only the summation order of zeta and of the dot products
 differs, so the iteration count matches and the state agrees to
 RBF_F64_TOL relative to max |Gamma|.
 F32C: the zeta field has FP32 pair arithmetic (scaled error < 1e-5), so the
-CG takes its own path to the same tolerance; the property checked is the
-one the reset exists for: the new strengths, evaluated by the oracle's
-FP64 zeta, reproduce the oracle's target vorticity to the CG tolerance
-(relative residual per component, RBF_F32C_RESID).
+CG takes its own path to the same tolerance.  Its strengths are within
+RBF_F32C_SOL of the oracle's (relative norm per component), and without
+static particles they reproduce the oracle's target vorticity, through the
+oracle's FP64 zeta, to the CG tolerance (RBF_F32C_RESID).  With static
+particles the reference's CG controls only its recursive residual (the
+statics add a constant to every A p), so the true residual is not bounded
+there and is not checked.
 """
 import numpy as np
 import pytest
@@ -18,6 +21,7 @@ pytestmark = pytest.mark.gpu
 
 RBF_F64_TOL = 1e-10
 RBF_F32C_RESID = 1.5e-3
+RBF_F32C_SOL = 2e-3
 
 
 def _say(*a):
@@ -51,10 +55,10 @@ def _resid(orc, R, omega, kernel, static):
 
 
 @pytest.mark.parametrize("kernel", ["gaussianerf", "gaussian", "winckelmans"])
-@pytest.mark.parametrize("m", [4, 11])
-def test_core_spreading_reset_vs_oracle(vpm, orc, kernel, m):
+@pytest.mark.parametrize("m,statics", [(4, True), (11, False), (11, True)])
+def test_core_spreading_reset_vs_oracle(vpm, orc, kernel, m, statics):
     api = vpm.api
-    P = _lattice(vpm, m, statics=(1, m ** 3 // 2))
+    P = _lattice(vpm, m, statics=(1, m ** 3 // 2) if statics else ())
     n = P.shape[0]
     sg0 = 0.07
     R = P.copy()
@@ -83,9 +87,15 @@ def test_core_spreading_reset_vs_oracle(vpm, orc, kernel, m):
             np.testing.assert_allclose(gres, res, rtol=1e-6)
             assert np.abs(G[:, 3:6] - R[:, 3:6]).max() <= RBF_F64_TOL * scale
         else:
-            r = _resid(orc, G, omega, kernel, static)
-            _say("rbf f32c residual vs oracle target", r)
-            assert np.all(r <= RBF_F32C_RESID), r
+            live = ~static
+            d = np.array([np.linalg.norm(G[live, 3 + c] - R[live, 3 + c]) / np.linalg.norm(R[live, 3 + c])
+                          for c in range(3)])
+            _say("rbf f32c strengths vs oracle", d)
+            assert np.all(d <= RBF_F32C_SOL), d
+            if not statics:
+                r = _resid(orc, G, omega, kernel, static)
+                _say("rbf f32c residual vs oracle target", r)
+                assert np.all(r <= RBF_F32C_RESID), r
     f.close()
 
 
