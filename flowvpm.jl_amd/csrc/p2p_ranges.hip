@@ -23,8 +23,7 @@ struct RangeItem {
 
 template <int K>
 __global__ __launch_bounds__(kBlock) void ranges_f32c_kernel(const RangeItem* __restrict__ items,
-                                                             const int64_t* __restrict__ lsb,
-                                                             const int64_t* __restrict__ lse,
+                                                             const int2* __restrict__ lsrc,
                                                              const double* __restrict__ pos, int64_t ld,
                                                              const float4* __restrict__ gs,
                                                              const float* __restrict__ cut,
@@ -49,7 +48,8 @@ __global__ __launch_bounds__(kBlock) void ranges_f32c_kernel(const RangeItem* __
 #pragma unroll
     for (int c = 0; c < kAccF32; ++c) { sum[c] = 0.f; err[c] = 0.f; }
     for (int64_t l = it.l_begin; l < it.l_end; ++l) {
-        const int64_t sb = lsb[l], se = lse[l];
+        const int2 rs = lsrc[l];
+        const int64_t sb = rs.x, se = rs.y;
         for (int64_t base = sb; base < se; base += kTile) {
             const int n = (int)((se - base) < kTile ? (se - base) : kTile);
             __syncthreads();
@@ -90,8 +90,7 @@ __global__ __launch_bounds__(kBlock) void ranges_f32c_kernel(const RangeItem* __
 template <int K>
 __global__ __launch_bounds__(kBlock) void ranges_wave_f32c_kernel(const RangeItem* __restrict__ items,
                                                                   int64_t nitems,
-                                                                  const int64_t* __restrict__ lsb,
-                                                                  const int64_t* __restrict__ lse,
+                                                                  const int2* __restrict__ lsrc,
                                                                   const double* __restrict__ pos, int64_t ld,
                                                                   const float4* __restrict__ gs,
                                                                   const float* __restrict__ cut,
@@ -122,7 +121,8 @@ __global__ __launch_bounds__(kBlock) void ranges_wave_f32c_kernel(const RangeIte
 #pragma unroll
     for (int c = 0; c < kAccF32; ++c) { sum[c] = 0.f; err[c] = 0.f; }
     for (int64_t l = it.l_begin; l < it.l_end; ++l) {
-        const int64_t sb = lsb[l], se = lse[l];
+        const int2 rs = lsrc[l];
+        const int64_t sb = rs.x, se = rs.y;
         for (int64_t base = sb; base < se; base += 64) {
             const int n = (int)((se - base) < 64 ? (se - base) : 64);
             __builtin_amdgcn_wave_barrier();
@@ -159,8 +159,7 @@ __global__ __launch_bounds__(kBlock) void ranges_wave_f32c_kernel(const RangeIte
 #pragma clang fp contract(off)
 template <int K>
 __global__ __launch_bounds__(kBlock) void ranges_f64_kernel(const RangeItem* __restrict__ items,
-                                                            const int64_t* __restrict__ lsb,
-                                                            const int64_t* __restrict__ lse,
+                                                            const int2* __restrict__ lsrc,
                                                             const double* __restrict__ pos, int64_t ld,
                                                             const double4* __restrict__ g64,
                                                             double* __restrict__ P)
@@ -177,7 +176,8 @@ __global__ __launch_bounds__(kBlock) void ranges_f64_kernel(const RangeItem* __r
 #pragma unroll
     for (int c = 0; c < kAccF64; ++c) acc[c] = 0.0;
     for (int64_t l = it.l_begin; l < it.l_end; ++l) {
-        const int64_t sb = lsb[l], se = lse[l];
+        const int2 rs = lsrc[l];
+        const int64_t sb = rs.x, se = rs.y;
         for (int64_t base = sb; base < se; base += kTile) {
             const int n = (int)((se - base) < kTile ? (se - base) : kTile);
             __syncthreads();
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void ranges_f64_kernel(const RangeItem* __r
 // at most 256 targets.
 struct RangePlan {
     std::vector<RangeItem> items;
-    std::vector<int64_t> lsb, lse;
+    std::vector<int2> src;     // source ranges (begin, end); positions fit 32 bits
 };
 
 // Fast plan for the usual FMM direct list: every target range is a leaf, so
@@ -232,8 +232,8 @@ static bool plan_ranges_leaves(const int64_t* tb, const int64_t* te, const int64
         const Run& q = runs[order[r]];
         if (q.tb < p.te || (q.tb == p.tb && q.te > q.tb && p.te > p.tb)) return false;   // overlap or split run
     }
-    plan.lsb.assign(sb, sb + npairs);
-    plan.lse.assign(se, se + npairs);
+    plan.src.resize((size_t)npairs);
+    for (int64_t k = 0; k < npairs; ++k) plan.src[(size_t)k] = make_int2((int)sb[k], (int)se[k]);
     for (const Run& r : runs) {
         if (r.te <= r.tb) continue;
         for (int64_t t = r.tb; t < r.te; t += chunk)
@@ -247,8 +247,7 @@ static void plan_ranges(const int64_t* tb, const int64_t* te, const int64_t* sb,
 {
     if (plan_ranges_leaves(tb, te, sb, se, npairs, chunk, plan)) return;
     plan.items.clear();
-    plan.lsb.clear();
-    plan.lse.clear();
+    plan.src.clear();
     std::vector<int64_t> cuts;
     cuts.reserve(2 * (size_t)npairs);
     for (int64_t k = 0; k < npairs; ++k)
@@ -265,16 +264,14 @@ static void plan_ranges(const int64_t* tb, const int64_t* te, const int64_t* sb,
         for (size_t m = lo; m < hi; ++m) count[m + 1]++;
     }
     for (size_t m = 0; m < nint; ++m) count[m + 1] += count[m];
-    plan.lsb.assign((size_t)count[nint], 0);
-    plan.lse.assign((size_t)count[nint], 0);
+    plan.src.assign((size_t)count[nint], make_int2(0, 0));
     std::vector<int64_t> fill(count.begin(), count.end() - 1);
     for (int64_t k = 0; k < npairs; ++k) {
         if (!(tb[k] < te[k] && sb[k] < se[k])) continue;
         const size_t lo = std::lower_bound(cuts.begin(), cuts.end(), tb[k]) - cuts.begin();
         const size_t hi = std::lower_bound(cuts.begin(), cuts.end(), te[k]) - cuts.begin();
         for (size_t m = lo; m < hi; ++m) {
-            plan.lsb[(size_t)fill[m]] = sb[k];
-            plan.lse[(size_t)fill[m]] = se[k];
+            plan.src[(size_t)fill[m]] = make_int2((int)sb[k], (int)se[k]);
             fill[m]++;
         }
     }
@@ -285,19 +282,17 @@ static void plan_ranges(const int64_t* tb, const int64_t* te, const int64_t* sb,
     }
 }
 
-static int upload_plan(vpm_field f, const RangePlan& plan, RangeItem*& d_items, int64_t*& d_lsb, int64_t*& d_lse)
+static int upload_plan(vpm_field f, const RangePlan& plan, RangeItem*& d_items, int2*& d_src)
 {
     const size_t bytes_items = sizeof(RangeItem) * plan.items.size();
-    const size_t bytes_list = sizeof(int64_t) * plan.lsb.size();
-    if (int rc = ensure(f->rng, bytes_items + 2 * bytes_list + 64)) return rc;
+    const size_t bytes_list = sizeof(int2) * plan.src.size();
+    if (int rc = ensure(f->rng, bytes_items + bytes_list + 64)) return rc;
     char* base = (char*)f->rng.p;
     d_items = (RangeItem*)base;
-    d_lsb = (int64_t*)(base + bytes_items);
-    d_lse = d_lsb + plan.lsb.size();
+    d_src = (int2*)(base + bytes_items);
     hipError_t e;
     if ((e = hipMemcpyAsync(d_items, plan.items.data(), bytes_items, hipMemcpyHostToDevice, f->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(d_lsb, plan.lsb.data(), bytes_list, hipMemcpyHostToDevice, f->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(d_lse, plan.lse.data(), bytes_list, hipMemcpyHostToDevice, f->stream)) != hipSuccess)
+        (e = hipMemcpyAsync(d_src, plan.src.data(), bytes_list, hipMemcpyHostToDevice, f->stream)) != hipSuccess)
         return hip_fail(e, "hipMemcpyAsync(range lists)");
     return VPM_OK;
 }
@@ -321,14 +316,14 @@ int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb,
     int rc;
     if ((rc = stage_field(f, kernel, precision, false, st))) return rc;
     RangeItem* d_items;
-    int64_t *d_lsb, *d_lse;
-    if ((rc = upload_plan(f, plan, d_items, d_lsb, d_lse))) return rc;
+    int2* d_src;
+    if ((rc = upload_plan(f, plan, d_items, d_src))) return rc;
     const dim3 grid((unsigned)plan.items.size());
     hipError_t e;
     timing_begin(f);
     if (precision == VPM_PREC_F64) {
         switch (kernel) {
-#define VPM_R64(KID) case KID: hipLaunchKernelGGL(ranges_f64_kernel<KID>, grid, dim3(kBlock), 0, f->stream, d_items, d_lsb, d_lse, st.pos, st.ld, st.g64, f->P); break;
+#define VPM_R64(KID) case KID: hipLaunchKernelGGL(ranges_f64_kernel<KID>, grid, dim3(kBlock), 0, f->stream, d_items, d_src, st.pos, st.ld, st.g64, f->P); break;
             VPM_R64(VPM_KERNEL_SINGULAR) VPM_R64(VPM_KERNEL_GAUSSIAN) VPM_R64(VPM_KERNEL_GAUSSIANERF) VPM_R64(VPM_KERNEL_WINCKELMANS)
 #undef VPM_R64
         default: return fail(VPM_EINVAL, "unknown kernel id");
@@ -337,14 +332,14 @@ int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb,
         const int64_t ni = (int64_t)plan.items.size();
         const dim3 wgrid((unsigned)((ni + kBlock / 64 - 1) / (kBlock / 64)));
         switch (kernel) {
-#define VPM_RW(KID) case KID: hipLaunchKernelGGL(ranges_wave_f32c_kernel<KID>, wgrid, dim3(kBlock), 0, f->stream, d_items, ni, d_lsb, d_lse, st.pos, st.ld, st.gs, st.cut, f->P); break;
+#define VPM_RW(KID) case KID: hipLaunchKernelGGL(ranges_wave_f32c_kernel<KID>, wgrid, dim3(kBlock), 0, f->stream, d_items, ni, d_src, st.pos, st.ld, st.gs, st.cut, f->P); break;
             VPM_RW(VPM_KERNEL_SINGULAR) VPM_RW(VPM_KERNEL_GAUSSIAN) VPM_RW(VPM_KERNEL_GAUSSIANERF) VPM_RW(VPM_KERNEL_WINCKELMANS)
 #undef VPM_RW
         default: return fail(VPM_EINVAL, "unknown kernel id");
         }
     } else {
         switch (kernel) {
-#define VPM_R32(KID) case KID: hipLaunchKernelGGL(ranges_f32c_kernel<KID>, grid, dim3(kBlock), 0, f->stream, d_items, d_lsb, d_lse, st.pos, st.ld, st.gs, st.cut, f->P); break;
+#define VPM_R32(KID) case KID: hipLaunchKernelGGL(ranges_f32c_kernel<KID>, grid, dim3(kBlock), 0, f->stream, d_items, d_src, st.pos, st.ld, st.gs, st.cut, f->P); break;
             VPM_R32(VPM_KERNEL_SINGULAR) VPM_R32(VPM_KERNEL_GAUSSIAN) VPM_R32(VPM_KERNEL_GAUSSIANERF) VPM_R32(VPM_KERNEL_WINCKELMANS)
 #undef VPM_R32
         default: return fail(VPM_EINVAL, "unknown kernel id");
@@ -372,8 +367,7 @@ int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb,
 // wave (Gaussian-erf, Gaussian), sums folded into TwoSum every 32 sources.
 template <int K>
 __global__ __launch_bounds__(kBlock) void estr_ranges_f32c_kernel(const RangeItem* __restrict__ items,
-                                                                  const int64_t* __restrict__ lsb,
-                                                                  const int64_t* __restrict__ lse,
+                                                                  const int2* __restrict__ lsrc,
                                                                   double* __restrict__ P, int transposed)
 {
     __shared__ float4 lp[kTile];   // s'_hi, a
@@ -399,7 +393,8 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_f32c_kernel(const RangeIte
     for (int c = 0; c < 6; ++c) { sum[c] = 0.f; err[c] = 0.f; }
     const double l2e = 1.4426950408889634;
     for (int64_t l = it.l_begin; l < it.l_end; ++l) {
-        const int64_t sb = lsb[l], se = lse[l];
+        const int2 rs = lsrc[l];
+        const int64_t sb = rs.x, se = rs.y;
         for (int64_t base = sb; base < se; base += kTile) {
             const int n = (int)((se - base) < kTile ? (se - base) : kTile);
             __syncthreads();
@@ -517,8 +512,7 @@ __device__ __forceinline__ double zeta_ref_r(double r)
 // onto SFS directly as the reference does.
 template <int K>
 __global__ __launch_bounds__(kBlock) void estr_ranges_f64_kernel(const RangeItem* __restrict__ items,
-                                                                 const int64_t* __restrict__ lsb,
-                                                                 const int64_t* __restrict__ lse,
+                                                                 const int2* __restrict__ lsrc,
                                                                  double* __restrict__ P, int transposed)
 {
     __shared__ double lx[kTile][4];
@@ -534,7 +528,8 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_f64_kernel(const RangeItem
     for (int c = 0; c < 9; ++c) JT[c] = tr[RJ + c];
     double s1 = tr[RSFS], s2 = tr[RSFS + 1], s3 = tr[RSFS + 2];
     for (int64_t l = it.l_begin; l < it.l_end; ++l) {
-        const int64_t sb = lsb[l], se = lse[l];
+        const int2 rs = lsrc[l];
+        const int64_t sb = rs.x, se = rs.y;
         for (int64_t base = sb; base < se; base += kTile) {
             const int n = (int)((se - base) < kTile ? (se - base) : kTile);
             __syncthreads();
@@ -585,20 +580,20 @@ int launch_estr_ranges(vpm_field f, int kernel, int precision, int transposed, c
     if (plan.items.empty()) return VPM_OK;
     int rc;
     RangeItem* d_items;
-    int64_t *d_lsb, *d_lse;
-    if ((rc = upload_plan(f, plan, d_items, d_lsb, d_lse))) return rc;
+    int2* d_src;
+    if ((rc = upload_plan(f, plan, d_items, d_src))) return rc;
     const dim3 grid((unsigned)plan.items.size());
     timing_begin(f);
     if (precision == VPM_PREC_F64 || kernel == VPM_KERNEL_SINGULAR) {
         switch (kernel) {
-#define VPM_E64(KID) case KID: hipLaunchKernelGGL(estr_ranges_f64_kernel<KID>, grid, dim3(kBlock), 0, f->stream, d_items, d_lsb, d_lse, f->P, transposed); break;
+#define VPM_E64(KID) case KID: hipLaunchKernelGGL(estr_ranges_f64_kernel<KID>, grid, dim3(kBlock), 0, f->stream, d_items, d_src, f->P, transposed); break;
             VPM_E64(VPM_KERNEL_SINGULAR) VPM_E64(VPM_KERNEL_GAUSSIAN) VPM_E64(VPM_KERNEL_GAUSSIANERF) VPM_E64(VPM_KERNEL_WINCKELMANS)
 #undef VPM_E64
         default: return fail(VPM_EINVAL, "unknown kernel id");
         }
     } else {
         switch (kernel) {
-#define VPM_E32(KID) case KID: hipLaunchKernelGGL(estr_ranges_f32c_kernel<KID>, grid, dim3(kBlock), 0, f->stream, d_items, d_lsb, d_lse, f->P, transposed); break;
+#define VPM_E32(KID) case KID: hipLaunchKernelGGL(estr_ranges_f32c_kernel<KID>, grid, dim3(kBlock), 0, f->stream, d_items, d_src, f->P, transposed); break;
             VPM_E32(VPM_KERNEL_GAUSSIAN) VPM_E32(VPM_KERNEL_GAUSSIANERF) VPM_E32(VPM_KERNEL_WINCKELMANS)
 #undef VPM_E32
         default: return fail(VPM_EINVAL, "unknown kernel id");
@@ -623,8 +618,7 @@ int launch_estr_ranges(vpm_field f, int kernel, int precision, int transposed, c
 // is the block size: 64 (one wave per FMM leaf) for short items, else 256.
 template <int K, int BS>
 __global__ __launch_bounds__(BS) void zeta_ranges_f32c_kernel(const RangeItem* __restrict__ items,
-                                                              const int64_t* __restrict__ lsb,
-                                                              const int64_t* __restrict__ lse,
+                                                              const int2* __restrict__ lsrc,
                                                               double* __restrict__ P)
 {
     __shared__ float4 lp[BS];   // s'_hi, cut
@@ -643,7 +637,8 @@ __global__ __launch_bounds__(BS) void zeta_ranges_f32c_kernel(const RangeItem* _
     split_rel(tr[RX + 2], oz, tz, tlz);
     float sum[3] = {0.f, 0.f, 0.f}, err[3] = {0.f, 0.f, 0.f};
     for (int64_t l = it.l_begin; l < it.l_end; ++l) {
-        const int64_t sb = lsb[l], se = lse[l];
+        const int2 rs = lsrc[l];
+        const int64_t sb = rs.x, se = rs.y;
         for (int64_t base = sb; base < se; base += BS) {
             const int n = (int)((se - base) < BS ? (se - base) : BS);
             __syncthreads();
@@ -705,8 +700,7 @@ __global__ __launch_bounds__(BS) void zeta_ranges_f32c_kernel(const RangeItem* _
 // as the reference does.
 template <int K>
 __global__ __launch_bounds__(kBlock) void zeta_ranges_f64_kernel(const RangeItem* __restrict__ items,
-                                                                 const int64_t* __restrict__ lsb,
-                                                                 const int64_t* __restrict__ lse,
+                                                                 const int2* __restrict__ lsrc,
                                                                  double* __restrict__ P)
 {
     __shared__ double4 ls[kTile];   // x, y, z, sigma
@@ -719,7 +713,8 @@ __global__ __launch_bounds__(kBlock) void zeta_ranges_f64_kernel(const RangeItem
     const double tx = tr[RX], ty = tr[RX + 1], tz = tr[RX + 2];
     double acc[3] = {tr[RJ], tr[RJ + 1], tr[RJ + 2]};
     for (int64_t l = it.l_begin; l < it.l_end; ++l) {
-        const int64_t sb = lsb[l], se = lse[l];
+        const int2 rs = lsrc[l];
+        const int64_t sb = rs.x, se = rs.y;
         for (int64_t base = sb; base < se; base += kTile) {
             const int n = (int)((se - base) < kTile ? (se - base) : kTile);
             __syncthreads();
@@ -763,27 +758,27 @@ int launch_zeta_ranges(vpm_field f, int kernel, int precision, const int64_t* tb
     if (plan.items.empty()) return VPM_OK;
     int rc;
     RangeItem* d_items;
-    int64_t *d_lsb, *d_lse;
-    if ((rc = upload_plan(f, plan, d_items, d_lsb, d_lse))) return rc;
+    int2* d_src;
+    if ((rc = upload_plan(f, plan, d_items, d_src))) return rc;
     const dim3 grid((unsigned)plan.items.size());
     timing_begin(f);
     if (f64) {
         switch (kernel) {
-#define VPM_Z64(KID) case KID: hipLaunchKernelGGL(zeta_ranges_f64_kernel<KID>, grid, dim3(kBlock), 0, f->stream, d_items, d_lsb, d_lse, f->P); break;
+#define VPM_Z64(KID) case KID: hipLaunchKernelGGL(zeta_ranges_f64_kernel<KID>, grid, dim3(kBlock), 0, f->stream, d_items, d_src, f->P); break;
             VPM_Z64(VPM_KERNEL_SINGULAR) VPM_Z64(VPM_KERNEL_GAUSSIAN) VPM_Z64(VPM_KERNEL_GAUSSIANERF) VPM_Z64(VPM_KERNEL_WINCKELMANS)
 #undef VPM_Z64
         default: return fail(VPM_EINVAL, "unknown kernel id");
         }
     } else if (wave) {
         switch (kernel) {
-#define VPM_ZW(KID) case KID: hipLaunchKernelGGL((zeta_ranges_f32c_kernel<KID, 64>), grid, dim3(64), 0, f->stream, d_items, d_lsb, d_lse, f->P); break;
+#define VPM_ZW(KID) case KID: hipLaunchKernelGGL((zeta_ranges_f32c_kernel<KID, 64>), grid, dim3(64), 0, f->stream, d_items, d_src, f->P); break;
             VPM_ZW(VPM_KERNEL_GAUSSIAN) VPM_ZW(VPM_KERNEL_GAUSSIANERF) VPM_ZW(VPM_KERNEL_WINCKELMANS)
 #undef VPM_ZW
         default: return fail(VPM_EINVAL, "unknown kernel id");
         }
     } else {
         switch (kernel) {
-#define VPM_Z32(KID) case KID: hipLaunchKernelGGL((zeta_ranges_f32c_kernel<KID, kBlock>), grid, dim3(kBlock), 0, f->stream, d_items, d_lsb, d_lse, f->P); break;
+#define VPM_Z32(KID) case KID: hipLaunchKernelGGL((zeta_ranges_f32c_kernel<KID, kBlock>), grid, dim3(kBlock), 0, f->stream, d_items, d_src, f->P); break;
             VPM_Z32(VPM_KERNEL_GAUSSIAN) VPM_Z32(VPM_KERNEL_GAUSSIANERF) VPM_Z32(VPM_KERNEL_WINCKELMANS)
 #undef VPM_Z32
         default: return fail(VPM_EINVAL, "unknown kernel id");

@@ -181,8 +181,10 @@ int vpm_create(int device, int64_t maxparticles, vpm_field* out)
 {
     if (!out) return fail(VPM_EINVAL, "null output handle");
     *out = nullptr;
-    if (maxparticles < 1 || maxparticles > (int64_t)UINT32_MAX)
-        return fail(VPM_EINVAL, "maxparticles must be in [1, 2^32)");
+    // indices are 32-bit on the device (sort permutation, range lists);
+    // 2^31 records would take 790 GB, beyond one GPU's memory
+    if (maxparticles < 1 || maxparticles > (int64_t)INT32_MAX)
+        return fail(VPM_EINVAL, "maxparticles must be in [1, 2^31)");
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
     if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");

@@ -88,7 +88,8 @@ const char* vpm_last_error(void);
 int vpm_device_count(int* count);
 
 /* ---- particle fields: ParticleField storage (particlefield.jl:67-141) --- */
-/* Allocate a field for up to maxparticles particles on `device`. */
+/* Allocate a field for up to maxparticles particles on `device`
+ * (1 <= maxparticles < 2^31). */
 int vpm_create(int device, int64_t maxparticles, vpm_field* out);
 int vpm_destroy(vpm_field f);
 /* Copy np whole 46-row records in (np <= maxparticles).  Rejects sigma <= 0
