@@ -1,0 +1,69 @@
+"""Host-side argument checks of the driver entry points (vpm_add_particles,
+vpm_remove_particle, vpm_zeta_ranges, vpm_core_spreading_reset) and their
+error codes (particlefield.jl:171-173, 402-407 raise the same conditions).
+Every case is rejected before any kernel launch."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_add_and_remove_particle_errors(vpm):
+    api = vpm.api
+    P = vpm.workloads.random_cloud(10, seed=1)
+    f = api.Field(12, 0)
+    f.set_particles(P)
+    with pytest.raises(api.VPMError) as ei:
+        f.add_particles(vpm.workloads.random_cloud(3, seed=2))      # 13 > 12
+    assert ei.value.status == -2
+    bad = vpm.workloads.random_cloud(1, seed=3)
+    bad[0, 6] = 0.0
+    with pytest.raises(api.VPMError) as ei:
+        f.add_particles(bad)                                        # sigma <= 0
+    assert ei.value.status == -1
+    assert f.np == 10
+    for i in (-1, 10):
+        with pytest.raises(api.VPMError) as ei:
+            f.remove_particle(i)
+        assert ei.value.status == -1
+    f.add_particles(vpm.workloads.random_cloud(2, seed=4))
+    assert f.np == 12
+    f.remove_particle(0)                                            # the last one moves into slot 0
+    G = np.zeros((11, 46))
+    f.get_particles(G)
+    Q = np.vstack([P, vpm.workloads.random_cloud(2, seed=4)])
+    assert np.array_equal(G[0], Q[11]) and np.array_equal(G[1:10], Q[1:10]) and np.array_equal(G[10], Q[10])
+    f.close()
+
+
+def test_zeta_ranges_and_reset_argument_errors(vpm):
+    api = vpm.api
+    n = 50
+    P = vpm.workloads.random_cloud(n, seed=5)
+    with pytest.raises(api.VPMError) as ei:
+        api.zeta_ranges(P.copy(), [0], [n + 1], [0], [n], kernel="gaussianerf", precision="f32c")
+    assert ei.value.status == -1
+    with pytest.raises(api.VPMError) as ei:
+        api.zeta_ranges(P.copy(), [0, 1], [n], [0], [n], kernel="gaussianerf", precision="f32c")
+    assert ei.value.status == -1
+    A = P.copy()
+    api.zeta_ranges(A, [], [], [], [], kernel="gaussianerf", precision="f32c")   # empty list: no-op
+    assert np.array_equal(A, P)
+    f = api.Field(n, 0)
+    f.set_particles(P)
+    for sg0 in (0.0, -1.0, float("nan")):
+        with pytest.raises(api.VPMError) as ei:
+            f.core_spreading_reset(sg0, "gaussianerf", "f64")
+        assert ei.value.status == -1
+    with pytest.raises(api.VPMError) as ei:
+        f.core_spreading_reset(0.05, "gaussianerf", "f64", itmax=-1)
+    assert ei.value.status == -1
+    G = np.zeros_like(P)
+    f.get_particles(G)
+    assert np.array_equal(G, P)                                     # nothing ran
+    # itmax = 0: target and sigma are set, Gamma = target * vol, no iteration
+    it, res = f.core_spreading_reset(0.05, "gaussianerf", "f64", itmax=0)
+    f.get_particles(G)
+    assert it == 0 and np.all(G[:, 6] == 0.05)
+    np.testing.assert_array_equal(G[:, 3:6], G[:, 33:36] * G[:, 7:8])
+    f.close()
