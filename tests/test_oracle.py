@@ -519,3 +519,44 @@ def test_rk3_step_runs_core_spreading_reset(orc, vpm):
     R0 = P.copy()
     assert orc.rk3_step(R0, s0, 0.01, relax_now=False) == 1
     assert s0.t_sgm == pytest.approx(0.01) and np.all(R0[:, 6] > sg0)
+
+
+def _adjoint_identities(P, U, J, Ubar, Jbar, Xb, Gb, Sb):
+    tu, tj = np.sum(Ubar * U, axis=1), np.sum(Jbar * J, axis=1)
+    L = np.sum(tu) + np.sum(tj)
+    lhs = np.sum(Gb * P[:, 3:6], axis=1)
+    e_lin = abs(np.sum(lhs) - L) / (np.sum(np.abs(lhs)) + np.sum(np.abs(tu)) + np.sum(np.abs(tj)))
+    e_tr = (np.abs(np.sum(Xb, axis=0)) / np.sum(np.abs(Xb), axis=0)).max()
+    Xc = P[:, 0:3] - P[:, 0:3].mean(axis=0)
+    hx, hs, rhs = np.sum(Xb * Xc, axis=1), Sb * P[:, 6], -2.0 * tu - 3.0 * tj
+    e_hom = abs(np.sum(hx) + np.sum(hs) - np.sum(rhs)) / (np.sum(np.abs(hx)) + np.sum(np.abs(hs)) + np.sum(np.abs(rhs)))
+    return e_lin, e_tr, e_hom
+
+
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans", "gaussian", "singular"])
+def test_adjoint_identities_hold_and_catch_mistakes(orc, vpm, kernel):
+    """The size-independent identities the full-size GPU test applies to the
+    adjoint (linearity in Gamma, translation invariance, homogeneity of
+    degree -2 / -3 under (X, sigma) scaling) hold for the oracle to rounding,
+    and each fails when one part of the gradient is planted wrong.  The
+    weights are the outputs themselves (Ubar = U, Jbar = J, held fixed), so
+    that L = sum |U|^2 + sum |J|^2 has no cancellation and a planted error
+    shows at its own size."""
+    n = 300
+    P = vpm.workloads.random_cloud(n, seed=9)
+    P[:, 6] *= 5.0
+    Q = P.copy()
+    orc.uj_direct(Q, kernel)
+    U, J = Q[:, 9:12], Q[:, 15:24]
+    Ubar, Jbar = U.copy(), J.copy()
+    Xb, Gb, Sb = orc.uj_adjoint(P, kernel, Ubar, Jbar)
+    ok = _adjoint_identities(P, U, J, Ubar, Jbar, Xb, Gb, Sb)
+    assert max(ok) < 1e-12, ok
+    bad = []
+    if kernel != "singular":                                       # no sigma dependence there
+        bad.append(_adjoint_identities(P, U, J, Ubar, Jbar, Xb, Gb, 0 * Sb)[2])       # sigma term dropped
+    bad.append(_adjoint_identities(P, U, J, Ubar, Jbar, Xb, 0.99 * Gb, Sb)[0])        # Gamma gradient scaled
+    Xd = Xb.copy()
+    Xd[np.argmax(np.linalg.norm(Xb, axis=1))] = 0.0
+    bad.append(_adjoint_identities(P, U, J, Ubar, Jbar, Xd, Gb, Sb)[1])               # one particle's dX dropped
+    assert min(bad) > 1e-4, bad
