@@ -560,3 +560,121 @@ def test_adjoint_identities_hold_and_catch_mistakes(orc, vpm, kernel):
     Xd[np.argmax(np.linalg.norm(Xb, axis=1))] = 0.0
     bad.append(_adjoint_identities(P, U, J, Ubar, Jbar, Xd, Gb, Sb)[1])               # one particle's dX dropped
     assert min(bad) > 1e-4, bad
+
+
+# ---- per-particle schemes: properties fixed by the mathematics -----------
+# (timeintegration.jl:388-532, relaxation.jl:62-142, viscous.jl:167-180,
+# subfilterscale.jl:541-673 and 204-250)
+
+def test_rk3_uniform_advection_is_exact(orc, vpm):
+    """A lone particle (no self-interaction: U = 0, J = 0) in a free stream
+    moves by exactly dt U_inf per step: the low-storage coefficients sum to
+    one (1/3 + 15/16 4/9 + 8/15 15/32 = 1); Gamma and sigma do not change."""
+    P = vpm.fields.make_particles([[0.1, -0.2, 0.3]], [[0.4, 0.5, -0.6]], 0.05)
+    Uinf = (0.3, -0.2, 0.1)
+    s = orc.make_settings(kernel="gaussianerf", f=0.0, g=0.2, relax=0, Uinf=Uinf)
+    R = P.copy()
+    for _ in range(4):
+        orc.rk3_step(R, s, 0.05, relax_now=False)
+    np.testing.assert_allclose(R[0, 0:3], P[0, 0:3] + 4 * 0.05 * np.array(Uinf), rtol=0, atol=1e-15)
+    assert np.array_equal(R[0, 3:7], P[0, 3:7])
+
+
+def test_rk3_third_order_convergence(orc, vpm):
+    """On 60 strongly interacting particles (rVPM, stretching on, no
+    relaxation; they move about a tenth of the box by T = 1), the RK3 error
+    of X and Gamma at fixed time falls by 2^3 each time dt halves (measured
+    8.2, 8.1, 8.1); a wrong stage coefficient or a missed register reset
+    loses an order."""
+    rng = np.random.default_rng(15)
+    n = 60
+    P = vpm.fields.make_particles(rng.uniform(0, 1, (n, 3)), rng.normal(0, 2e-2, (n, 3)), 0.15)
+    s = orc.make_settings(kernel="gaussianerf", f=0.0, g=0.2, relax=0)
+    T = 1.0
+
+    def run(nsteps):
+        R = P.copy()
+        for _ in range(nsteps):
+            orc.rk3_step(R, s, T / nsteps, relax_now=False)
+        return R
+
+    ref = run(512)
+    err = [np.abs(run(k)[:, 0:6] - ref[:, 0:6]).max() for k in (4, 8, 16, 32)]
+    ratios = [err[i] / err[i + 1] for i in range(3)]
+    assert all(7.0 < r < 9.0 for r in ratios), (err, ratios)
+
+
+def test_relaxation_properties(orc, vpm):
+    """Pedrizzetti (relaxation.jl:62-81): rlxf = 0 changes nothing and
+    rlxf = 1 aligns Gamma with the vorticity omega = (J32 - J23, J13 - J31,
+    J21 - J12) keeping |Gamma|; the corrected scheme (:117-142) keeps |Gamma|
+    for any rlxf; omega = 0 (symmetric J) leaves Gamma as it is."""
+    rng = np.random.default_rng(12)
+    n = 50
+    P = vpm.workloads.random_cloud(n, seed=13)
+    P[:, 3:6] = rng.normal(size=(n, 3))
+    P[:, 15:24] = rng.normal(size=(n, 9))
+    S = P[0, 15:24].reshape(3, 3)
+    P[0, 15:24] = (S + S.T).reshape(-1)                   # omega = 0 at particle 0
+    J = P[:, 15:24]
+    w = np.stack([J[:, 5] - J[:, 7], J[:, 6] - J[:, 2], J[:, 1] - J[:, 3]], axis=1)
+    g0 = np.linalg.norm(P[:, 3:6], axis=1)
+    R = P.copy()
+    orc.relax(R, 0.0)
+    np.testing.assert_allclose(R[:, 3:6], P[:, 3:6], rtol=1e-15, atol=0)
+    R = P.copy()
+    orc.relax(R, 1.0)
+    wn = np.linalg.norm(w[1:], axis=1)
+    np.testing.assert_allclose(R[1:, 3:6], g0[1:, None] * w[1:] / wn[:, None], rtol=1e-13, atol=1e-15)
+    assert np.array_equal(R[0, 3:6], P[0, 3:6])
+    for rlxf in (0.05, 0.3, 0.7):
+        R = P.copy()
+        orc.relax(R, rlxf, corrected=True)
+        np.testing.assert_allclose(np.linalg.norm(R[:, 3:6], axis=1), g0, rtol=1e-13)
+
+
+def test_core_spreading_advances_sigma_squared_linearly(orc, vpm):
+    """With f = g = 0 (no stretching term in the sigma update), a
+    CoreSpreading rk3 step advances sigma^2 by exactly 2 nu dt (the RK3
+    registers integrate the constant rate d sigma^2/dt = 2 nu); static
+    particles keep their sigma; t_sgm advances by dt."""
+    P = vpm.workloads.random_cloud(120, seed=14)
+    P[:, 6] *= 3.0
+    P[[5, 50], 42] = 1.0
+    nu, dt = 1e-3, 0.01
+    s = orc.make_settings(kernel="gaussianerf", f=0.0, g=0.0, relax=0, viscous=1, nu=nu, sgm0=0.05,
+                          beta=100.0)
+    R = P.copy()
+    assert orc.rk3_step(R, s, dt, relax_now=False) == 0
+    live = P[:, 42] == 0
+    np.testing.assert_allclose(R[live, 6] ** 2 - P[live, 6] ** 2, 2 * nu * dt, rtol=1e-10)
+    assert np.array_equal(R[~live, 6], P[~live, 6])
+    assert s.t_sgm == pytest.approx(dt)
+
+
+def test_dynamic_sfs_coefficient_rules(orc, vpm):
+    """dynamicprocedure_pseudo3level_afterUJ (subfilterscale.jl:541-673) and
+    clipping_backscatter (:225-247, 287-296) on a two-ring cloud: every
+    non-static coefficient satisfies |C| <= maxC, |C| >= minC unless its
+    numerator vanished, C >= 0 with force_positive, C = C2/C3 up to the sign
+    taken by force_positive (or 0 when clipped), C (Gamma . SFS) >= 0 after
+    clipping, and M is flushed."""
+    P = vpm.workloads.leapfrog_cfg2(Nphi=24)
+    n = P.shape[0]
+    P[[3, 40], 42] = 1.0
+    minC, maxC = 0.01, 0.6
+    R = P.copy()
+    assert orc.sfs_before(R, "gaussianerf", True, 0.999, minC, maxC) == 0
+    orc.uj_direct(R, "gaussianerf", reset=True, reset_sfs=True, sfs=True)
+    assert orc.sfs_after(R, "gaussianerf", True, 0.999, 0.005, minC, maxC, force_positive=True, clip=True) == 0
+    live = R[:, 42] == 0
+    C = R[live, 36]
+    ratio = R[live, 37] / R[live, 38]
+    gs = np.sum(R[live, 3:6] * R[live, 39:42], axis=1)
+    clipped = C == 0
+    assert np.all(C >= 0) and np.all(C <= maxC * (1 + 1e-14))
+    assert np.all(C[~clipped] >= minC * (1 - 1e-14))
+    np.testing.assert_allclose(C[~clipped], np.abs(ratio[~clipped]), rtol=1e-15)
+    assert np.all(C * gs >= 0)
+    assert np.all(R[live, 27:36] == 0)
+    assert 0 < np.count_nonzero(~clipped) < n
