@@ -78,6 +78,7 @@ def lib():
         L.vpmo_core_spreading_reset.argtypes = [_dp, C.c_int64, C.c_int, C.c_double, C.c_int, C.c_double,
                                                 C.c_int, C.c_int, C.POINTER(C.c_int), _dp]
         L.vpmo_zeta_direct.argtypes = [_dp, C.c_int64, C.c_int, C.c_int]
+        L.vpmo_uj_adjoint_rrules.argtypes = [_dp, C.c_int64, C.c_int, _dp, _dp, _dp, _dp, _dp]
         L.vpmo_zeta_ranges.argtypes = [_dp, C.c_int64, C.c_int, _ip, _ip, _ip, _ip, C.c_int64]
         L.vpmo_rk3_step_adjoint.restype = C.c_int
         L.vpmo_rk3_step_adjoint.argtypes = [_dp, C.c_int64, C.c_void_p, C.c_double, _dp]
@@ -252,6 +253,17 @@ def uj_adjoint(P, kernel, Ubar, Jbar, nthreads=None):
                           np.ascontiguousarray(Ubar, dtype=np.float64).reshape(-1),
                           np.ascontiguousarray(Jbar, dtype=np.float64).reshape(-1),
                           Xb, Gb, Sb, nthreads or nthreads_default())
+    return Xb.reshape(n, 3), Gb.reshape(n, 3), Sb
+
+
+def uj_adjoint_rrules(P, kernel, Ubar, Jbar):
+    """The reference's UJS_pullback (rrules.jl:183-326) restated literally:
+    (dL/dX, dL/dGamma, dL/dsigma) for a field acting on itself."""
+    n = P.shape[0]
+    Xb, Gb, Sb = np.zeros(3 * n), np.zeros(3 * n), np.zeros(n)
+    lib().vpmo_uj_adjoint_rrules(_check(P), n, _kid(kernel),
+                                 np.ascontiguousarray(Ubar, dtype=np.float64).reshape(-1),
+                                 np.ascontiguousarray(Jbar, dtype=np.float64).reshape(-1), Xb, Gb, Sb)
     return Xb.reshape(n, 3), Gb.reshape(n, 3), Sb
 
 

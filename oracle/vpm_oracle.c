@@ -900,6 +900,112 @@ void vpmo_zeta_direct(double* P, int64_t np, int kernel, int nthreads)
     }
 }
 
+/* ---- the reference's own CPU gradient: UJS_pullback (rrules.jl:183-326) --
+ * A literal restatement of the ChainRules pullback of fmm.direct!, for a
+ * field acting on itself (the target and source cotangents of each particle
+ * are summed).  Cross-check only: the reference marks parts of it as
+ * inexact (rrules.jl:233, 237), so it is compared with vpmo_uj_adjoint (which
+ * central differences pin) and its differences are reported, not forced.
+ * eps3(a,x,y) = (x cross y)_a and lc(a,b,c) is the Levi-Civita symbol
+ * (rrules.jl:22-26); J-bar of target i is read column-major, Jb(c,b) =
+ * Jbar[9 i + c + 3 b] (0-based; rrules.jl:262). */
+static double eps3(int a, const double* x, const double* y)
+{
+    if (a == 0) return x[1] * y[2] - x[2] * y[1];
+    if (a == 1) return x[2] * y[0] - x[0] * y[2];
+    return x[0] * y[1] - x[1] * y[0];
+}
+static double lc(int a, int b, int c)
+{
+    if (a == b || b == c || c == a) return 0.0;
+    return ((b - a + 3) % 3 == 1) ? 1.0 : -1.0;
+}
+
+void vpmo_uj_adjoint_rrules(const double* P, int64_t np, int kernel,
+                            const double* Ubar, const double* Jbar,
+                            double* Xbar, double* Gbar, double* Sbar)
+{
+    const double c4 = 1 / (4 * PI_D);
+    for (int64_t k = 0; k < 3 * np; k++) { Xbar[k] = 0.0; Gbar[k] = 0.0; }
+    for (int64_t k = 0; k < np; k++) Sbar[k] = 0.0;
+    /* contributions from U-bar (rrules.jl:213-243) */
+    for (int64_t j = 0; j < np; j++) {
+        const double* sj = P + j * VPMO_NFIELDS;
+        const double* G = sj + RG;
+        const double sg = sj[RSIG];
+        for (int64_t i = 0; i < np; i++) {
+            const double* ti = P + i * VPMO_NFIELDS;
+            const double* Ub = Ubar + 3 * i;
+            double dx[3];
+            for (int e = 0; e < 3; e++) dx[e] = ti[RX + e] - sj[RX + e];
+            const double r = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+            if (!(r > 0)) continue;
+            double g, dg, d2g;
+            vpmo_g_derivs(kernel, r / sg, &g, &dg, &d2g);
+            const double A = g / (r * r * r);
+            const double B = dg / (r * r * r * r * sg) - 3 * g / (r * r * r * r * r);
+            for (int e = 0; e < 3; e++) {
+                double x_term = -A * eps3(e, Ub, G);
+                for (int a = 0; a < 3; a++) {
+                    double GU[3] = {G[0] * Ub[a], G[1] * Ub[a], G[2] * Ub[a]};
+                    x_term += B * dx[e] * eps3(a, dx, GU);
+                }
+                x_term *= -c4;
+                Xbar[3 * i + e] += x_term;
+                Xbar[3 * j + e] -= x_term;
+            }
+            for (int a = 0; a < 3; a++) {
+                Sbar[j] += c4 * dg / (r * r * sg * sg) * eps3(a, dx, G) * Ub[a];
+                Gbar[3 * j + a] += c4 * g / (r * r * r) * eps3(a, dx, Ub);
+            }
+        }
+    }
+    /* contributions from J-bar (rrules.jl:252-314) */
+    for (int64_t i = 0; i < np; i++) {
+        const double* ti = P + i * VPMO_NFIELDS;
+        const double* Jb = Jbar + 9 * i;
+        for (int64_t j = 0; j < np; j++) {
+            const double* sj = P + j * VPMO_NFIELDS;
+            const double* G = sj + RG;
+            const double sg = sj[RSIG];
+            double dx[3];
+            for (int e = 0; e < 3; e++) dx[e] = ti[RX + e] - sj[RX + e];
+            const double r = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+            if (!(r > 0.0)) continue;
+            double g, dg, ddg;
+            vpmo_g_derivs(kernel, r / sg, &g, &dg, &ddg);
+            const double alpha = dg / (sg * r) - 3 * g / (r * r);
+            const double beta = -c4 * g / (r * r * r);
+            double crss[3];
+            for (int a = 0; a < 3; a++) crss[a] = -c4 / (r * r * r) * eps3(a, dx, G);
+            for (int a = 0; a < 3; a++) {
+                for (int b = 0; b < 3; b++) {
+                    double sigma_term = 0.0;
+                    for (int c = 0; c < 3; c++) {
+                        sigma_term += c4 * dg * lc(a, b, c) * G[c] / (r * r);
+                        double gamma_term = -beta * lc(a, b, c) * Jb[c + 3 * b];
+                        double xyz_term = 0.0;
+                        for (int d = 0; d < 3; d++) {
+                            xyz_term += (dx[b] * lc(c, a, d) + dx[a] * lc(c, b, d)) * G[d];
+                            gamma_term += alpha * c4 / (r * r * r) * lc(a, c, d) * dx[c] * dx[b] * Jb[d + 3 * b];
+                        }
+                        xyz_term *= -alpha * c4 / r;
+                        xyz_term += (ddg / (sg * sg) - 7 * dg / (r * sg) + 15 * g / (r * r)) * crss[c] * dx[b] * dx[a];
+                        xyz_term *= Jb[c + 3 * b] / (r * r);
+                        Gbar[3 * j + a] += gamma_term;
+                        Xbar[3 * i + a] += xyz_term;
+                        Xbar[3 * j + a] -= xyz_term;
+                    }
+                    sigma_term += (-ddg / sg + 2 * dg / r) * crss[a] * dx[b];
+                    Sbar[j] += Jb[a + 3 * b] / (sg * sg) * sigma_term;
+                    Xbar[3 * i + a] += Jb[b + 3 * a] * alpha * crss[b];
+                    Xbar[3 * j + a] -= Jb[b + 3 * a] * alpha * crss[b];
+                }
+            }
+        }
+    }
+}
+
 /* ---- zeta_fmm's pair loop over a direct list (viscous.jl:523-558) ----
  * For every (target range, source range) pair in list order, every target i
  * of the range gets, over the sources j in order,

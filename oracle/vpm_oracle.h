@@ -123,6 +123,11 @@ void vpmo_estr_ranges(double* P, int64_t stride, int kernel, int transposed,
  * (:314-478) and zeta_direct; returns 1 when unconverged with iterror. */
 int vpmo_core_spreading_reset(double* P, int64_t np, int kernel, double sgm0, int itmax, double tol,
                               int iterror, int nthreads, int* iters, double res[3]);
+/* The reference's own pullback UJS_pullback (rrules.jl:183-326), restated
+ * literally for a field acting on itself; a cross-check of vpmo_uj_adjoint
+ * (the reference marks parts of it inexact).  Outputs are overwritten. */
+void vpmo_uj_adjoint_rrules(const double* P, int64_t np, int kernel, const double* Ubar, const double* Jbar,
+                            double* Xbar, double* Gbar, double* Sbar);
 /* zeta_direct (viscous.jl:486-515): J[1:3] of every particle (static
  * included) <- sum over all particles j (self included) of
  * Gamma_j zeta(r/sigma_j)/sigma_j^3. */

@@ -62,6 +62,18 @@ def _check(vpm, tag, got, ref, scale):
         assert scaled.max() < tol and normwise < ADJ_NORM_TOL, (tag, name, scaled.max(), normwise)
 
 
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans"])
+def test_adjoint_f32c_vs_reference_pullback(vpm, orc, kernel):
+    """Against the reference's own CPU gradient: UJS_pullback
+    (rrules.jl:183-326) restated literally in the oracle, on the cloud of
+    test_adjoint_f32c_vs_oracle, to the same summation-scale bounds."""
+    P = vpm.workloads.random_cloud(1500, seed=21)
+    w, W = vpm.workloads.adjoint_weights(1500)
+    ref = orc.uj_adjoint_rrules(P, kernel, w, W)
+    got = vpm.uj_adjoint(P, w, W, kernel=kernel, precision="f32c")
+    _check(vpm, "rrules " + kernel, got, ref, orc.uj_adjoint_scale(P, kernel, w, W))
+
+
 @pytest.mark.parametrize("kernel", KERNELS)
 def test_adjoint_f32c_vs_oracle(vpm, orc, kernel):
     P = vpm.workloads.random_cloud(1500, seed=21)

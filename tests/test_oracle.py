@@ -678,3 +678,26 @@ def test_dynamic_sfs_coefficient_rules(orc, vpm):
     assert np.all(C * gs >= 0)
     assert np.all(R[live, 27:36] == 0)
     assert 0 < np.count_nonzero(~clipped) < n
+
+
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans", "gaussian", "singular"])
+def test_uj_adjoint_equals_reference_pullback(orc, vpm, kernel):
+    """The closed-form adjoint (pinned by central differences and mpmath)
+    equals the reference's own CPU gradient, UJS_pullback of rrules.jl:183-326
+    restated literally, to FP64 rounding for every output, although the
+    reference's comments doubt parts of it (rrules.jl:233, 237)."""
+    n = 40
+    P = vpm.workloads.random_cloud(n, seed=31)
+    P[:, 6] *= 8.0
+    P[7, 0:3] = P[8, 0:3]                     # a coincident pair (skipped by both)
+    rng = np.random.default_rng(32)
+    Ub, Jb = rng.normal(size=(n, 3)), rng.normal(size=(n, 9))
+    a = orc.uj_adjoint(P, kernel, Ub, Jb)
+    b = orc.uj_adjoint_rrules(P, kernel, Ub, Jb)
+    for name, x, y in zip(("X", "Gamma", "sigma"), a, b):
+        x, y = np.asarray(x).reshape(n, -1), np.asarray(y).reshape(n, -1)
+        if kernel == "singular" and name == "sigma":
+            assert np.all(x == 0) and np.all(y == 0)
+            continue
+        e = np.linalg.norm(x - y) / np.linalg.norm(x)
+        assert e < 1e-13, (name, e)
