@@ -78,6 +78,9 @@ def lib():
         L.vpmo_core_spreading_reset.argtypes = [_dp, C.c_int64, C.c_int, C.c_double, C.c_int, C.c_double,
                                                 C.c_int, C.c_int, C.POINTER(C.c_int), _dp]
         L.vpmo_zeta_direct.argtypes = [_dp, C.c_int64, C.c_int, C.c_int]
+        L.vpmo_update_particle_states_pullback_rrules.argtypes = [_dp, _dp, C.c_int64, C.c_double, C.c_double,
+                                                                  C.c_double, C.c_double, C.c_double, C.c_double,
+                                                                  C.c_int]
         L.vpmo_uj_adjoint_rrules.argtypes = [_dp, C.c_int64, C.c_int, _dp, _dp, _dp, _dp, _dp]
         L.vpmo_zeta_ranges.argtypes = [_dp, C.c_int64, C.c_int, _ip, _ip, _ip, _ip, C.c_int64]
         L.vpmo_rk3_step_adjoint.restype = C.c_int
@@ -254,6 +257,13 @@ def uj_adjoint(P, kernel, Ubar, Jbar, nthreads=None):
                           np.ascontiguousarray(Jbar, dtype=np.float64).reshape(-1),
                           Xb, Gb, Sb, nthreads or nthreads_default())
     return Xb.reshape(n, 3), Gb.reshape(n, 3), Sb
+
+
+def update_particle_states_pullback_rrules(P, B, a, b, dt, f, g, zeta0, full=False):
+    """The reference's state_pullback (rrules.jl:436-547) restated literally,
+    in place on the cotangent records B (layout of update_particle_states_adjoint)."""
+    lib().vpmo_update_particle_states_pullback_rrules(_check(P), _check(B), P.shape[0], a, b, dt, f, g, zeta0,
+                                                      int(bool(full)))
 
 
 def uj_adjoint_rrules(P, kernel, Ubar, Jbar):

@@ -811,6 +811,90 @@ void vpmo_update_particle_states_adjoint(const double* P, double* B, int64_t np,
     }
 }
 
+/* ---- the reference's own update pullback: state_pullback (rrules.jl:436-547)
+ * A literal restatement (transposed scheme, the only one it covers) for the
+ * record layout of vpmo_update_particle_states_adjoint: B holds on entry the
+ * cotangents of the updated X, Gamma, sigma and registers M1 = M[0:3],
+ * M2 = M[3:6], M23 = M[7], and on exit those of the state before the update,
+ * with Ubar in the U rows and Jbar in the J rows.  C and SFS come from P.
+ * As written, the reference differentiates the -3 MM4 Gamma_zeta term of the
+ * Gamma register only for zeta = eta, in Jbar (:521) and in Gammabar (:524);
+ * full != 0 uses the whole sum over zeta there instead.  Its Gammabar also
+ * reads M2outbar[xi] of the first particle (:526); that is kept, so calls
+ * with np > 1 reproduce it.  Cross-check only. */
+void vpmo_update_particle_states_pullback_rrules(const double* P, double* B, int64_t np, double a, double b,
+                                                 double dt, double f, double g, double zeta0, int full)
+{
+    const double A = (f + g) / (1 + 3 * f);
+    const double Bc = f / (1 + 3 * f);
+    const double B2 = Bc / zeta0;
+    const double M2out0[3] = {B[RM + 3], B[RM + 4], B[RM + 5]};   /* M2outbar[1:3] (particle 1) */
+    for (int64_t i = 0; i < np; i++) {
+        const double* p = P + i * VPMO_NFIELDS;
+        double* q = B + i * VPMO_NFIELDS;
+        const double* G = p + RG;
+        const double* Jv = p + RJ;                 /* J_mat[xi][eta] = Jv[xi + 3 eta] */
+        const double sg = p[RSIG];
+        const double Ci = p[RC];
+        const double* S = p + RSFS;
+        double M1out[3], xout[3], M2out[3], Gout[3];
+        for (int k = 0; k < 3; k++) {
+            M1out[k] = q[RM + k]; xout[k] = q[RX + k]; M2out[k] = q[RM + 3 + k]; Gout[k] = q[RG + k];
+        }
+        const double M23out = q[RM + 7], sout = q[RSIG];
+        double MM[4] = {0, 0, 0, 0}, G2 = 0.0, dMM4dJ[3][3];
+        for (int e = 0; e < 3; e++) {                                      /* :484-492 */
+            G2 += G[e] * G[e];
+            for (int x = 0; x < 3; x++) MM[e] += Jv[x + 3 * e] * G[x];
+        }
+        for (int e = 0; e < 3; e++) {                                      /* :493-498 */
+            MM[3] += (A * MM[e] * G[e] - B2 * Ci * S[e] * G[e] * sg * sg * sg) / G2;
+            for (int x = 0; x < 3; x++) dMM4dJ[x][e] = 1 / G2 * A * G[x] * G[e];
+        }
+        double dMM4dC = 0.0, dMM4dsigma = 0.0;
+        const double sigmabar_term = M23out + b * sout;                    /* :501-502 */
+        double gdot = 0.0;                                                 /* sum_zeta Gamma_zeta m2_zeta */
+        for (int e = 0; e < 3; e++) gdot += G[e] * (M2out[e] + b * Gout[e]);
+        double Gin[3], Jb[9], M1in[3], Ub[3], M2in[3], Sb[3], sin_ = 0.0;
+        for (int e = 0; e < 3; e++) {                                      /* :503-533 */
+            double dMM4dGamma = -B2 * Ci * S[e] * sg * sg * sg - 2 * MM[3] * G[e] + A * MM[e];
+            for (int x = 0; x < 3; x++) dMM4dGamma += A * Jv[e + 3 * x] * G[x];
+            dMM4dGamma /= G2;
+            dMM4dC += -B2 * S[e] * G[e] * sg * sg * sg / G2;
+            const double dMM4dS = -B2 * Ci * G[e] * sg * sg * sg / G2;
+            dMM4dsigma += -3 * B2 * Ci * S[e] * G[e] * sg * sg / G2;
+            const double xbar_term = M1out[e] + b * xout[e];
+            M1in[e] = a * xbar_term;
+            Ub[e] = dt * xbar_term;
+            const double gammabar_term = M2out[e] + b * Gout[e];
+            M2in[e] = a * gammabar_term;
+            const double gm = full ? gdot : G[e] * gammabar_term;          /* Gamma_eta m2_eta (:521, :524) */
+            for (int x = 0; x < 3; x++)
+                Jb[x + 3 * e] = dt * G[x] * gammabar_term - 3 * dt * dMM4dJ[x][e] * gm
+                                - dt * sg * dMM4dJ[x][e] * sigmabar_term;
+            sin_ -= 3 * dt * (G[e] * dMM4dsigma + Ci * S[e] * sg * sg / zeta0) * gammabar_term;
+            Gin[e] = Gout[e] - 3 * dt * (dMM4dGamma * gm + MM[3] * gammabar_term)
+                     - dt * sg * dMM4dGamma * sigmabar_term;
+            for (int x = 0; x < 3; x++) Gin[e] += dt * Jv[e + 3 * x] * (M2out0[x] + b * Gout[x]);
+            Sb[e] = -dt * (3 * G[e] * dMM4dS + Ci * sg * sg * sg / zeta0) * gammabar_term
+                    - dt * sg * dMM4dS * sigmabar_term;
+        }
+        (void)sin_;                                                        /* overwritten at :535 */
+        const double sin_final = sout - dt * (MM[3] + sg * dMM4dsigma) * sigmabar_term;
+        (void)Sb;
+        for (int k = 0; k < 3; k++) {
+            q[RX + k] = xout[k];                                           /* :480 */
+            q[RG + k] = Gin[k];
+            q[RM + k] = M1in[k];
+            q[RM + 3 + k] = M2in[k];
+            q[RU + k] = Ub[k];
+        }
+        for (int k = 0; k < 9; k++) q[RJ + k] = Jb[k];
+        q[RM + 7] = a * sigmabar_term;
+        q[RSIG] = sin_final;
+    }
+}
+
 /* Pullback of one vpmo_rk3_step (no SFS, inviscid, no relaxation): B holds
  * the cotangent records of the state after the step and is replaced by those
  * of the state before it (rows X, Gamma, sigma; M rows are zero because the

@@ -123,6 +123,13 @@ void vpmo_estr_ranges(double* P, int64_t stride, int kernel, int transposed,
  * (:314-478) and zeta_direct; returns 1 when unconverged with iterror. */
 int vpmo_core_spreading_reset(double* P, int64_t np, int kernel, double sgm0, int itmax, double tol,
                               int iterror, int nthreads, int* iters, double res[3]);
+/* The reference's own update pullback state_pullback (rrules.jl:436-547),
+ * restated literally in the record layout of
+ * vpmo_update_particle_states_adjoint (transposed scheme).  full != 0 uses
+ * the whole sum over zeta where the reference keeps zeta = eta (:521, :524).
+ * Cross-check only. */
+void vpmo_update_particle_states_pullback_rrules(const double* P, double* B, int64_t np, double a, double b,
+                                                 double dt, double f, double g, double zeta0, int full);
 /* The reference's own pullback UJS_pullback (rrules.jl:183-326), restated
  * literally for a field acting on itself; a cross-check of vpmo_uj_adjoint
  * (the reference marks parts of it inexact).  Outputs are overwritten. */

@@ -701,3 +701,39 @@ def test_uj_adjoint_equals_reference_pullback(orc, vpm, kernel):
             continue
         e = np.linalg.norm(x - y) / np.linalg.norm(x)
         assert e < 1e-13, (name, e)
+
+
+def test_update_pullback_vs_reference_state_pullback(orc, vpm):
+    """The update pullback (derived from the rVPM update, pinned by central
+    differences) against the reference's own state_pullback
+    (rrules.jl:436-547) restated literally, particle by particle (its
+    Gammabar reads the first particle's register cotangent, :526, so each
+    call holds one particle): X, U, the registers and sigma agree to
+    rounding; Gamma and J differ, because the reference differentiates the
+    -3 MM4 Gamma_zeta term only at zeta = eta (:521, :524); with the whole
+    sum there the two agree to rounding on every output."""
+    n = 12
+    rng = np.random.default_rng(41)
+    P = vpm.workloads.random_cloud(n, seed=42)
+    P[:, 3:6] = rng.normal(size=(n, 3))
+    P[:, 15:24] = rng.normal(size=(n, 9))
+    B0 = np.zeros_like(P)
+    for rows in (slice(0, 7), slice(27, 33), slice(34, 35)):
+        B0[:, rows] = rng.normal(size=(n, rows.stop - rows.start))
+    a, b, dt, f, g = -5.0 / 9.0, 15.0 / 16.0, 0.01, 0.0, 0.2
+    zeta0 = orc.zeta("gaussianerf", 0.0)   # unused without SFS (C = 0)
+    ours = B0.copy()
+    orc.update_particle_states_adjoint(P, ours, a, b, dt, f, g, transposed=True)
+    lit, fix = B0.copy(), B0.copy()
+    for i in range(n):
+        for R, full in ((lit, False), (fix, True)):
+            Pi, Bi = P[i:i + 1].copy(), R[i:i + 1].copy()
+            orc.update_particle_states_pullback_rrules(Pi, Bi, a, b, dt, f, g, zeta0, full=full)
+            R[i] = Bi[0]
+    same = np.r_[0:3, 6, 9:12, 27:33, 34]
+    np.testing.assert_allclose(lit[:, same], ours[:, same], rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(fix[:, np.r_[0:12, 15:24, 27:33, 34]], ours[:, np.r_[0:12, 15:24, 27:33, 34]],
+                               rtol=1e-12, atol=1e-14)
+    dG = np.linalg.norm(lit[:, 3:6] - ours[:, 3:6]) / np.linalg.norm(ours[:, 3:6])
+    dJ = np.linalg.norm(lit[:, 15:24] - ours[:, 15:24]) / np.linalg.norm(ours[:, 15:24])
+    assert dG > 1e-4 and dJ > 1e-4, (dG, dJ)
