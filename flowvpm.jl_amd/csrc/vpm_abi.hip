@@ -225,6 +225,12 @@ int vpm_destroy(vpm_field f)
         DevBuf* bufs[] = {&f->pos, &f->gs, &f->cut, &f->g64, &f->part, &f->keys, &f->keys2, &f->perm,
                           &f->iota, &f->sorttmp, &f->bbox, &f->flag, &f->adj, &f->rng, &f->bar, &f->xfer};
         for (DevBuf* b : bufs) release(*b);
+        for (vpm_tape t : f->tapes) {   // detach: the tape stays valid only for vpm_tape_destroy
+            release(t->rec);
+            t->f = nullptr;
+            t->recorded = false;
+        }
+        f->tapes.clear();
         if (f->P) (void)hipFree(f->P);
         if (f->ev0) (void)hipEventDestroy(f->ev0);
         if (f->ev1) (void)hipEventDestroy(f->ev1);
@@ -637,6 +643,10 @@ int vpm_tape_create(vpm_field f, vpm_tape* out)
     vpm_tape t = new (std::nothrow) vpm_tape_s;
     if (!t) return fail(VPM_ENOMEM, "vpm_tape_create");
     t->f = f;
+    {
+        std::lock_guard<std::mutex> lock(f->mu);
+        f->tapes.push_back(t);
+    }
     *out = t;
     return VPM_OK;
 }
@@ -644,10 +654,13 @@ int vpm_tape_create(vpm_field f, vpm_tape* out)
 int vpm_tape_destroy(vpm_tape t)
 {
     if (!t) return VPM_OK;
-    if (t->rec.p) {
-        std::lock_guard<std::mutex> lock(t->f->mu);
-        if (int rc = set_device(t->f)) return rc;
-        release(t->rec);
+    if (vpm_field f = t->f) {   // null once the field was destroyed (rec already released)
+        std::lock_guard<std::mutex> lock(f->mu);
+        if (t->rec.p) {
+            if (int rc = set_device(f)) return rc;
+            release(t->rec);
+        }
+        f->tapes.erase(std::remove(f->tapes.begin(), f->tapes.end(), t), f->tapes.end());
     }
     delete t;
     return VPM_OK;
@@ -667,6 +680,7 @@ int vpm_rk3_step_taped(vpm_field f, vpm_step_config* s, double dt, vpm_tape t)
 int vpm_rk3_step_adjoint(vpm_tape t, double* bar)
 {
     if (!t) return fail(VPM_EINVAL, "null vpm_tape");
+    if (!t->f) return fail(VPM_EINVAL, "the tape's field was destroyed");
     vpm_field f = t->f;
     VPM_FIELD_GUARD(f);
     if (!t->recorded) return fail(VPM_EINVAL, "tape holds no recorded step");

@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/flowvpm_hip.h"
 
@@ -66,6 +67,7 @@ struct vpm_field_s {
     bool profiling = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     vpm::Timing timing[5];      // U/J F32C, U/J F64, E_str, adjoint, zeta
+    std::vector<vpm_tape> tapes;     // live tapes of this field (released with it)
 };
 
 // Stage records of one rk3 step (state after each stage's UJ evaluation),

@@ -228,6 +228,9 @@ int vpm_uj_adjoint(vpm_field f, const vpm_config* cfg, const double* Ubar, const
  * with their U and J. */
 typedef struct vpm_tape_s* vpm_tape;
 int vpm_tape_create(vpm_field f, vpm_tape* out);
+/* Destroying a field releases the device memory of its tapes; such a tape
+ * must still be passed to vpm_tape_destroy, and any other use of it returns
+ * VPM_EINVAL. */
 int vpm_tape_destroy(vpm_tape t);
 /* vpm_rk3_step (ReformulatedVPM, no SFS, inviscid, no relaxation) that also
  * records its stage states into t (overwriting what t held). */

@@ -139,3 +139,24 @@ def test_taped_step_rejects_sfs_and_empty_tape(vpm):
     with pytest.raises(api.VPMError):
         api.rungekutta3_taped(pf, 0.01, tape)
     tape.close()
+
+
+def test_tape_outlives_its_field(vpm):
+    """Destroying a field releases its tapes' device memory and detaches
+    them: a detached tape's adjoint returns VPM_EINVAL, it can no longer
+    record, and destroying it afterwards is safe."""
+    api = vpm.api
+    P = _cloud(vpm, 200, seed=5)
+    f = api.Field(200, 0)
+    f.set_particles(P)
+    s = api._lib.StepConfig()
+    s.uj = api.make_config("gaussianerf", "f32c", True)
+    s.g = 0.2
+    t1, t2 = api.Tape(f), api.Tape(f)
+    api.check(api.lib().vpm_rk3_step_taped(f.handle, api.C.byref(s), 0.01, t1.handle))
+    t2.close()                       # a tape closed before its field
+    f.close()
+    with pytest.raises(api.VPMError) as ei:
+        api.rk3_adjoint(t1, np.zeros((200, 46)))
+    assert ei.value.status == -1
+    t1.close()
