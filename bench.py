@@ -39,23 +39,29 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(P, kernel, budget_s=12.0):
+def cpu_baseline(P, kernel, target_s=12.0):
     """FP64 oracle (reference-algorithm restatement) on a bounded sample of
-    targets against all sources, OpenMP over targets."""
+    targets against all sources, OpenMP over targets: a short calibration
+    run sizes one timed sample of about target_s seconds of CPU work."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     orc.lib()
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
     n = P.shape[0]
     nt = 64
-    while True:
+    while True:                                   # calibration: at least 0.5 s
         Q = P.copy()
         t0 = time.perf_counter()
         orc.direct(Q, kernel, t_range=(0, nt), nthreads=nthreads)
         dt = time.perf_counter() - t0
-        if dt > budget_s / 4 or nt * 2 > n:
+        if dt > 0.5 or nt * 2 > n:
             break
-        nt = min(n, int(nt * max(2.0, min(8.0, (budget_s / 4) / max(dt, 1e-3)))))
+        nt = min(n, nt * 4)
+    nt = max(1, min(n, int(nt * target_s / max(dt, 1e-3))))
+    Q = P.copy()
+    t0 = time.perf_counter()
+    orc.direct(Q, kernel, t_range=(0, nt), nthreads=nthreads)
+    dt = time.perf_counter() - t0
     rate = nt * n / dt
     return {"value": rate, "unit": "pair-interactions/s", "cores": nthreads, "kind": "port",
             "sample": "%d targets x %d sources of the workload, FP64 oracle (oracle/vpm_oracle.c), "
