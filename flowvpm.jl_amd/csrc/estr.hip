@@ -363,11 +363,14 @@ int launch_estr_impl(vpm_field f, int kernel, int transposed, int precision, con
                        (const float*)bc + 1, np, tbox);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "pack_estr");
-    // source splits, as for U/J
+    // source splits, chunks of whole tiles: enough blocks to fill the chip,
+    // and up to 32 chunks of at least 16 tiles to spread the near-field work
+    // of blocks whose targets straddle a jump of the sort order (as for zeta)
     const int64_t nb = (np + kBlock - 1) / kBlock;
-    int64_t K = std::max<int64_t>(1, (2560 + nb - 1) / nb);
     const int64_t tiles = (np + kTile - 1) / kTile;
-    K = std::min(K, tiles);
+    const int64_t k_fill = std::min<int64_t>((2560 + nb - 1) / nb, tiles);
+    const int64_t k_spread = std::min<int64_t>(32, tiles / 16);
+    int64_t K = std::max<int64_t>(1, std::max(k_fill, k_spread));
     const int64_t chunk = ((tiles + K - 1) / K) * kTile;
     K = (np + chunk - 1) / chunk;
     if ((rc = ensure(f->part, sizeof(double) * 6 * (size_t)K * (size_t)np))) return rc;

@@ -227,10 +227,16 @@ int launch_zeta(vpm_field f, int kernel, int precision)
     const int64_t ld = f->cap;
     int rc;
     const unsigned nbt = (unsigned)((np + kBlock - 1) / kBlock);
-    // source splits: enough blocks to fill the chip, chunks of whole tiles
-    const int64_t want = 2560;
+    // source splits, chunks of whole tiles: enough blocks to fill the chip,
+    // and up to 32 chunks of at least 16 tiles.  Near-field work per block is
+    // uneven (a block whose targets straddle a jump of the sort order reaches
+    // most tiles and finishes last); the chunks spread such a block's tiles
+    // over that many blocks.  At N = 2^20 on the jet column: 21 ms with one
+    // chunk, 12.4 with 8, 8.6 with 16, 6.6 with 32.
     const int64_t tiles = (np + kTile - 1) / kTile;
-    int64_t K = std::max<int64_t>(1, std::min<int64_t>((want + nbt - 1) / nbt, tiles));
+    const int64_t k_fill = std::min<int64_t>((2560 + nbt - 1) / nbt, tiles);
+    const int64_t k_spread = std::min<int64_t>(32, tiles / 16);
+    int64_t K = std::max<int64_t>(1, std::max(k_fill, k_spread));
     const int64_t chunk = ((tiles + K - 1) / K) * kTile;
     K = (np + chunk - 1) / chunk;
     if ((rc = ensure(f->part, sizeof(double) * 3 * (size_t)K * (size_t)np))) return rc;
