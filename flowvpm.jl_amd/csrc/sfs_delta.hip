@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void dj_f32c_kernel(const double* __restric
                                                          const float4* __restrict__ sgs,
                                                          const float* __restrict__ scut,
                                                          const double* __restrict__ tbox, int64_t np, float delta,
-                                                         double* __restrict__ dj)
+                                                         int64_t chunk, double* __restrict__ part)
 {
     __shared__ float4 lpos[kTile];
     __shared__ float4 lgs[kTile];
@@ -129,10 +129,11 @@ __global__ __launch_bounds__(kBlock) void dj_f32c_kernel(const double* __restric
     for (int c = 0; c < NA; ++c) { sum[c] = 0.f; err[c] = 0.f; }
     __shared__ int nlist[kBlock];
     __shared__ int ncnt[kBlock / 64];
-    NearTiles near(tbox, 0, (np + kTile - 1) / kTile, org, blo, bhi, 1.0001f, nlist, ncnt);
+    const int64_t s0 = (int64_t)blockIdx.y * chunk, s1 = min(s0 + chunk, np);
+    NearTiles near(tbox, s0 / kTile, (s1 + kTile - 1) / kTile, org, blo, bhi, 1.0001f, nlist, ncnt);
     for (int64_t tile; (tile = near.next()) >= 0;) {
         const int64_t base = tile * kTile;
-        const int n = (int)min((int64_t)kTile, np - base);
+        const int n = (int)min((int64_t)kTile, s1 - base);
         __syncthreads();
         if (tid < n) {
             const int64_t j = base + tid;
@@ -205,9 +206,27 @@ __global__ __launch_bounds__(kBlock) void dj_f32c_kernel(const double* __restric
         }
     }
     if (i < np) {
+        double* out = part + (int64_t)blockIdx.y * NA * np + i;
+#pragma unroll
+        for (int c = 0; c < NA; ++c) out[c * np] = (double)sum[c] + (double)err[c];
+    }
+}
+
+// dJ from the chunk partials of dj_f32c_kernel, summed in chunk order.
+__global__ __launch_bounds__(kBlock) void dj_finalize_kernel(const double* __restrict__ part, int nchunk, int64_t np,
+                                                             double* __restrict__ dj)
+{
+    constexpr int NA = 11;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < np) {
         double a[NA];
 #pragma unroll
-        for (int c = 0; c < NA; ++c) a[c] = (double)sum[c] + (double)err[c];
+        for (int c = 0; c < NA; ++c) a[c] = 0.0;
+        for (int k = 0; k < nchunk; ++k) {
+            const double* pk = part + (int64_t)k * NA * np + i;
+#pragma unroll
+            for (int c = 0; c < NA; ++c) a[c] += pk[c * np];
+        }
         // dJ = -3 dJhv + eps . dW' (column-major), dJhv[2][2] = -dJhv00 - dJhv11
         const double Wx = a[0], Wy = a[1], Wz = a[2];
         double J[9];
@@ -221,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void dj_f32c_kernel(const double* __restric
         J[7] = -3.0 * a[10] + Wx;
         J[8] = 3.0 * (a[3] + a[7]);
 #pragma unroll
-        for (int c = 0; c < 9; ++c) dj[c * ld + i] = J[c];
+        for (int c = 0; c < 9; ++c) dj[c * np + i] = J[c];
     }
 }
 
@@ -261,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void de_f32c_kernel(const double* __restric
                                                          const double* __restrict__ dj, int64_t ld,
                                                          const float4* __restrict__ gz, const float* __restrict__ scut,
                                                          const double* __restrict__ tbox, int64_t np, float alpha,
-                                                         int transposed, double* __restrict__ out)
+                                                         int transposed, int64_t chunk, double* __restrict__ part)
 {
     __shared__ float4 lp[kTile];    // s'_hi, sigma (0: static)
     __shared__ float4 lg[kTile];    // Gamma, cut
@@ -290,10 +309,11 @@ __global__ __launch_bounds__(kBlock) void de_f32c_kernel(const double* __restric
     const float oma = 1.f - alpha;
     __shared__ int nlist[kBlock];
     __shared__ int ncnt[kBlock / 64];
-    NearTiles near(tbox, 0, (np + kTile - 1) / kTile, org, blo, bhi, 1.0f, nlist, ncnt);
+    const int64_t s0 = (int64_t)blockIdx.y * chunk, s1 = min(s0 + chunk, np);
+    NearTiles near(tbox, s0 / kTile, (s1 + kTile - 1) / kTile, org, blo, bhi, 1.0f, nlist, ncnt);
     for (int64_t tile; (tile = near.next()) >= 0;) {
         const int64_t base = tile * kTile;
-        const int n = (int)min((int64_t)kTile, np - base);
+        const int n = (int)min((int64_t)kTile, s1 - base);
         __syncthreads();
         if (tid < n) {
             const int64_t j = base + tid;
@@ -381,9 +401,34 @@ __global__ __launch_bounds__(kBlock) void de_f32c_kernel(const double* __restric
         }
     }
     if (i < np) {
+        double* o = part + (int64_t)blockIdx.y * 12 * np + i;
+#pragma unroll
+        for (int c = 0; c < 12; ++c) o[c * np] = (double)sum[c] + (double)err[c];
+    }
+}
+
+// dE from the chunk partials of de_f32c_kernel, summed in chunk order; J'
+// is relative to the J of the first target of the 256-target block, as in
+// the kernel.
+__global__ __launch_bounds__(kBlock) void de_finalize_kernel(const double* __restrict__ part, int nchunk,
+                                                             const double* __restrict__ J,
+                                                             const double* __restrict__ dj, int64_t ld, int64_t np,
+                                                             int transposed, double* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t ib = (int64_t)blockIdx.x * kBlock;
+    if (i < np) {
         double A[12];
 #pragma unroll
-        for (int c = 0; c < 12; ++c) A[c] = (double)sum[c] + (double)err[c];
+        for (int c = 0; c < 12; ++c) A[c] = 0.0;
+        for (int k = 0; k < nchunk; ++k) {
+            const double* pk = part + (int64_t)k * 12 * np + i;
+#pragma unroll
+            for (int c = 0; c < 12; ++c) A[c] += pk[c * np];
+        }
+        double Jr[9], dJr[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) { Jr[c] = J[c * ld + ib]; dJr[c] = dj[c * ld + ib]; }
         double Jp[9], dJp[9];
 #pragma unroll
         for (int c = 0; c < 9; ++c) { Jp[c] = J[c * ld + i] - Jr[c]; dJp[c] = dj[c * ld + i] - dJr[c]; }
@@ -453,6 +498,15 @@ int launch_sfs_delta(vpm_field f, int kernel, int transposed, double alpha)
     float* cut = (float*)(tbox + 8 * ntile);
     const unsigned nb = (unsigned)((np + kBlock - 1) / kBlock);
     const float delta = (float)(1.0 / alpha - 1.0);
+    // source chunks as for zeta and E_str (up to 32 of at least 16 tiles)
+    const int64_t k_fill = std::min<int64_t>((2560 + nb - 1) / nb, ntile);
+    const int64_t k_spread = std::min<int64_t>(32, ntile / 16);
+    int64_t nchunk = std::max<int64_t>(1, std::max(k_fill, k_spread));
+    const int64_t chunk = ((ntile + nchunk - 1) / nchunk) * kTile;
+    nchunk = (np + chunk - 1) / chunk;
+    if ((rc = ensure(f->part, sizeof(double) * 12 * (size_t)nchunk * (size_t)np))) return rc;
+    double* part = (double*)f->part.p;
+    const dim3 grid(nb, (unsigned)nchunk);
     switch (kernel) {
 #define VPM_SD(KID)                                                                                              \
     case KID:                                                                                                    \
@@ -460,11 +514,15 @@ int launch_sfs_delta(vpm_field f, int kernel, int transposed, double alpha)
                            gs, cut, gz);                                                                         \
         hipLaunchKernelGGL(tile_box_kernel<1>, dim3((unsigned)ntile), dim3(kBlock), 0, f->stream,                \
                            (const double*)pos, ld, (const float*)cut, np, tbox);                               \
-        hipLaunchKernelGGL(dj_f32c_kernel<KID>, dim3(nb), dim3(kBlock), 0, f->stream, (const double*)pos, ld,     \
-                           (const float4*)gs, (const float*)cut, (const double*)tbox, np, delta, dj);            \
-        hipLaunchKernelGGL(de_f32c_kernel<KID>, dim3(nb), dim3(kBlock), 0, f->stream, (const double*)pos,         \
+        hipLaunchKernelGGL(dj_f32c_kernel<KID>, grid, dim3(kBlock), 0, f->stream, (const double*)pos, ld,         \
+                           (const float4*)gs, (const float*)cut, (const double*)tbox, np, delta, chunk, part);   \
+        hipLaunchKernelGGL(dj_finalize_kernel, dim3(nb), dim3(kBlock), 0, f->stream, (const double*)part,         \
+                           (int)nchunk, np, dj);                                                                 \
+        hipLaunchKernelGGL(de_f32c_kernel<KID>, grid, dim3(kBlock), 0, f->stream, (const double*)pos,             \
                            (const double*)J, (const double*)dj, ld, (const float4*)gz, (const float*)cut,         \
-                           (const double*)tbox, np, (float)alpha, transposed, de);                               \
+                           (const double*)tbox, np, (float)alpha, transposed, chunk, part);                      \
+        hipLaunchKernelGGL(de_finalize_kernel, dim3(nb), dim3(kBlock), 0, f->stream, (const double*)part,         \
+                           (int)nchunk, (const double*)J, (const double*)dj, ld, np, transposed, de);           \
         break;
         VPM_SD(VPM_KERNEL_GAUSSIAN) VPM_SD(VPM_KERNEL_GAUSSIANERF) VPM_SD(VPM_KERNEL_WINCKELMANS)
 #undef VPM_SD
