@@ -203,8 +203,12 @@ int vpm_create(int device, int64_t maxparticles, vpm_field* out)
         delete f;
         return hip_fail(e, "hipMalloc(particles)");
     }
-    if ((e = hipMemset(f->P, 0, sizeof(double) * NF * (size_t)maxparticles)) != hipSuccess ||
-        (e = hipEventCreate(&f->ev0)) != hipSuccess || (e = hipEventCreate(&f->ev1)) != hipSuccess) {
+    // the zero fill goes on the field's own stream and is waited for: the
+    // stream is non-blocking, so a fill on the null stream would not be
+    // ordered before the first upload and could land on top of it
+    if ((e = hipMemsetAsync(f->P, 0, sizeof(double) * NF * (size_t)maxparticles, f->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(f->stream)) != hipSuccess || (e = hipEventCreate(&f->ev0)) != hipSuccess ||
+        (e = hipEventCreate(&f->ev1)) != hipSuccess) {
         int rc = hip_fail(e, "field setup");
         (void)hipFree(f->P);
         (void)hipStreamDestroy(f->stream);

@@ -67,3 +67,21 @@ def test_zeta_ranges_and_reset_argument_errors(vpm):
     assert it == 0 and np.all(G[:, 6] == 0.05)
     np.testing.assert_array_equal(G[:, 3:6], G[:, 33:36] * G[:, 7:8])
     f.close()
+
+
+def test_first_upload_survives_the_zero_fill(vpm):
+    """vpm_create zeroes the record buffer on the field's own stream and
+    waits for it, so an upload right after creation is never overwritten by
+    the fill (the fill on the null stream was not ordered with the field's
+    non-blocking stream).  Large capacities make the fill the longer
+    operation."""
+    api = vpm.api
+    rng = np.random.default_rng(77)
+    P = vpm.fields.make_particles(rng.normal(size=(1000, 3)), rng.normal(size=(1000, 3)), 0.1)
+    for cap in (1 << 20, 1 << 22, 3000):
+        f = api.Field(cap, 0)
+        f.set_particles(P)
+        G = np.zeros_like(P)
+        f.get_particles(G)
+        f.close()
+        assert np.array_equal(G, P), cap
