@@ -12,7 +12,8 @@ Reference interfaces mirrored here (src/ paths of FLOWVPM.jl):
   Estr_fmm!(... direct_list)                  FLOWVPM_subfilterscale_models.jl:157-188 -> estr_ranges
   rungekutta3(pfield, dt; relax) / nextstep   FLOWVPM_timeintegration.jl:388-461,
                                               FLOWVPM_particlefield.jl:435-460
-  run_vpm!(pfield, dt, nsteps)                FLOWVPM_utils.jl:41-139 (relax cadence)
+  run_vpm!(pfield, dt, nsteps)                FLOWVPM_utils.jl:41-139 (relax and save cadence)
+  save(pfield, name) / read!(pfield, h5)      FLOWVPM_utils.jl:148-328,493-571 -> h5io
   zeta_direct(pfield)                         FLOWVPM_viscous.jl:486-515
   kernel / formulation / relaxation / SFS / viscous aliases   FLOWVPM.jl:80-256
 
@@ -27,7 +28,8 @@ from dataclasses import dataclass
 import numpy
 import numpy as np
 
-from . import _lib
+from . import _lib, h5io
+from .h5io import read_into, save  # noqa: F401  (save / read! of FLOWVPM_utils.jl)
 from .fields import NFIELDS, STATIC_INDEX, make_particles
 from ._lib import VPMError, check, lib  # noqa: F401
 
@@ -533,8 +535,10 @@ def nextstep(pfield, dt, *, relax=False, resident=False):
     pfield.nt += 1
 
 
-def run_vpm(pfield, dt, nsteps, *, runtime_function=None, static_particles_function=None, resident=True):
-    """The stepping loop of run_vpm! (FLOWVPM_utils.jl:86-119): relaxation
+def run_vpm(pfield, dt, nsteps, *, runtime_function=None, static_particles_function=None, resident=True,
+            save_path=None, save_pfield=True, create_savepath=True, run_name="pfield", nsteps_save=1,
+            prompt=True, save_time=True):
+    """The stepping loop of run_vpm! (FLOWVPM_utils.jl:86-136): relaxation
     every nsteps_relax steps, starting at step 1; before each step
     static_particles_function(pfield, t, dt) may append particles, which are
     removed after the step unless it returns False (:100-112); then
@@ -542,10 +546,18 @@ def run_vpm(pfield, dt, nsteps, *, runtime_function=None, static_particles_funct
     viscous scheme must be compatible with the kernel (:56-64,
     FLOWVPM.jl:259-271).  State stays
     on the device between steps when resident=True: appended particles are
-    uploaded with vpm_add_particles and removed with vpm_remove_particle."""
+    uploaded with vpm_add_particles and removed with vpm_remove_particle.
+
+    With save_path, the folder is made (an existing one is replaced when
+    create_savepath, :66-68) and the field is saved as <run_name>.<nt>.h5 +
+    .xmf at step 0, every nsteps_save steps, at the last step and at a
+    runtime_function break (:124-129); save_time=False stamps the files
+    with nt instead of t.  The settings file (BSON) is not written."""
     if isinstance(pfield.viscous, CoreSpreading) and pfield.kernel != "gaussianerf":
         raise VPMError(_lib.VPM_EINVAL, "Kernel %s is not compatible with viscous scheme CoreSpreading; "
                        "compatible kernels are ['gaussianerf']" % pfield.kernel)
+    if save_path is not None and create_savepath:
+        h5io.create_path(save_path, prompt)
     if resident:
         pfield_upload(pfield)
     for i in range(0, nsteps + 1):
@@ -564,11 +576,20 @@ def run_vpm(pfield, dt, nsteps, *, runtime_function=None, static_particles_funct
                     remove_particle(pfield, pi)
                     if resident:
                         pfield.device_field().remove_particle(pi)
+        downloaded = False
+        breakflag = False
         if runtime_function is not None:
             if resident and pfield.np:
                 pfield_download(pfield)
-            if runtime_function(pfield, pfield.t, dt):
-                break
+                downloaded = True
+            breakflag = bool(runtime_function(pfield, pfield.t, dt))
+        if save_pfield and save_path is not None and (i % nsteps_save == 0 or i == nsteps or breakflag):
+            if resident and pfield.np and not downloaded:
+                pfield_download(pfield)
+            h5io.save(pfield, run_name, path=save_path, add_num=True,
+                      overwrite_time=None if save_time else pfield.nt)
+        if breakflag:
+            break
     if resident and pfield.np:
         pfield_download(pfield)
 

@@ -6,6 +6,7 @@ path a whole rungekutta3 step must agree with the oracle to FP64 rounding
 FP32c U/J error (bounded here by 1e-5 relative per particle, vector norms).
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -238,3 +239,41 @@ def test_run_vpm_static_particles_and_relax_cadence_vs_oracle(vpm, orc, resident
     _say("run_vpm statics resident=%s" % resident, e)
     assert max(e.values()) < 1e-10, e
     assert np.all(pf.particles[n:n + 3, 9:27] == 0.0)   # removed slots reset
+
+
+def test_run_vpm_saves_and_restarts(vpm, tmp_path):
+    """run_vpm with save_path on a device-resident field (FLOWVPM_utils.jl:
+    124-129): each saved file holds the field as it was at that step, and a
+    restart with read! (load_time, overwrite) from step 2 takes the next step
+    to the same state, bit for bit, as the run that went on (a step reads
+    only X, Gamma, sigma, vol and static, all in the file).  Skipped without
+    libhdf5."""
+    from flowvpm_amd import h5io
+    if h5io.hdf5_library_path() is None:
+        pytest.skip("no libhdf5 on this machine")
+    api = vpm.api
+    P = vpm.workloads.ring_cfg0()[::8].copy()
+    n = P.shape[0]
+    out = str(tmp_path / "run")
+    kw = dict(kernel="gaussianerf", formulation=api.rVPM, relaxation=api.pedrizzetti, precision="f32c")
+    pf = _pfield(vpm, P, **kw)
+    snaps = {}
+
+    def runtime(pfield, t, dt_):
+        snaps[pfield.nt] = pfield.particles[:pfield.np].copy()
+        return False
+
+    api.run_vpm(pf, 0.01, 3, runtime_function=runtime, save_path=out, resident=True)
+    assert sorted(os.listdir(out)) == sorted("pfield.%d.%s" % (k, e) for k in range(4) for e in ("h5", "xmf"))
+    for k in range(4):
+        q = api.ParticleField(n)
+        h5io.read_into(q, "pfield.%d" % k, path=out)
+        assert q.nt == k and abs(q.t - 0.01 * k) < 1e-15
+        cols = [0, 1, 2, 3, 4, 5, 6, 7, 8, 42]
+        assert np.array_equal(q.particles[:, cols], snaps[k][:, cols]), k
+    r = api.ParticleField(n, **kw)
+    h5io.read_into(r, "pfield.2", path=out, overwrite=True, load_time=True)
+    api.nextstep(r, 0.01, relax=True)
+    assert r.nt == 3
+    _say("restart from step 2 vs the run: max |dX| %.1e" % np.abs(r.particles[:n, 0:3] - snaps[3][:, 0:3]).max())
+    assert np.array_equal(r.particles[:n, 0:9], snaps[3][:, 0:9])
