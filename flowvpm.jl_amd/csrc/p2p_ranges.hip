@@ -10,6 +10,7 @@
 // targets' records: no two blocks write the same particle, and each target's
 // sum runs in pair order.  The pair arithmetic is p2p_pair.h's.
 #include <algorithm>
+#include <string>
 #include <vector>
 
 #include "p2p_pair.h"
@@ -202,57 +203,95 @@ __global__ __launch_bounds__(kBlock) void ranges_f64_kernel(const RangeItem* __r
 #pragma clang fp contract(on)
 
 // Host plan of a pair list: elementary target intervals, each with the
-// source ranges of every pair covering it (pair order), cut into blocks of
-// at most 256 targets.
+// source ranges of every pair covering it (pair order), cut into items of at
+// most `chunk` targets.  Items and source ranges are written straight into
+// the field's pinned staging buffer (items first) and reach the device in
+// one copy.
 struct RangePlan {
-    std::vector<RangeItem> items;
-    std::vector<int2> src;     // source ranges (begin, end); positions fit 32 bits
+    RangeItem* items = nullptr;   // in f->hstage
+    int2* src = nullptr;          // source ranges (begin, end); positions fit 32 bits
+    int64_t nitems = 0, nsrc = 0;
+    bool wave = false;            // F32C over short intervals: one wave per <= 64 targets
 };
 
-// Fast plan for the usual FMM direct list: every target range is a leaf, so
-// ranges are identical or disjoint, and the pairs of one target range are
-// listed together.  Then the list itself is the per-interval source list.
-// Returns false (plan untouched) when the list is not of that form.
-static bool plan_ranges_leaves(const int64_t* tb, const int64_t* te, const int64_t* sb, const int64_t* se,
-                               int64_t npairs, int64_t chunk, RangePlan& plan)
+static int stage_plan(vpm_field f, int64_t nitems, int64_t nsrc, RangePlan& plan)
 {
+    const size_t bytes = sizeof(RangeItem) * (size_t)nitems + sizeof(int2) * (size_t)nsrc;
+    if (int rc = ensure_pinned(f->hstage, bytes)) return rc;
+    plan.items = (RangeItem*)f->hstage.p;
+    plan.src = (int2*)(plan.items + nitems);
+    plan.nitems = nitems;
+    plan.nsrc = nsrc;
+    return VPM_OK;
+}
+
+static int range_error(int64_t k)
+{
+    return fail(VPM_EINVAL, "range pair " + std::to_string(k) + " outside [0, np)");
+}
+
+// allow_wave: the caller has a wave kernel, taken when the mean target range
+// of the list is at most 128 targets (FMM leaves).  Every pair is checked
+// (0 <= begin <= end <= np) in the passes that read it, before any device
+// work.
+static int plan_ranges(vpm_field f, const int64_t* tb, const int64_t* te, const int64_t* sb, const int64_t* se,
+                       int64_t npairs, bool allow_wave, RangePlan& plan)
+{
+    plan = RangePlan();
+    const int64_t np = f->np;
+    // runs of consecutive pairs with one target range
     struct Run { int64_t tb, te, l0, l1; };
     std::vector<Run> runs;
+    int64_t tot = 0, cnt = 0;
     for (int64_t k = 0; k < npairs;) {
+        if (tb[k] < 0 || te[k] > np || tb[k] > te[k]) return range_error(k);
         int64_t m = k + 1;
         while (m < npairs && tb[m] == tb[k] && te[m] == te[k]) ++m;
         runs.push_back({tb[k], te[k], k, m});
+        if (te[k] > tb[k]) { tot += te[k] - tb[k]; ++cnt; }
         k = m;
     }
-    std::vector<size_t> order(runs.size());
-    for (size_t r = 0; r < runs.size(); ++r) order[r] = r;
-    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return runs[a].tb < runs[b].tb; });
-    for (size_t r = 1; r < order.size(); ++r) {
-        const Run& p = runs[order[r - 1]];
-        const Run& q = runs[order[r]];
-        if (q.tb < p.te || (q.tb == p.tb && q.te > q.tb && p.te > p.tb)) return false;   // overlap or split run
-    }
-    plan.src.resize((size_t)npairs);
-    for (int64_t k = 0; k < npairs; ++k) plan.src[(size_t)k] = make_int2((int)sb[k], (int)se[k]);
-    for (const Run& r : runs) {
-        if (r.te <= r.tb) continue;
-        for (int64_t t = r.tb; t < r.te; t += chunk)
-            plan.items.push_back({t, std::min<int64_t>(t + chunk, r.te), r.l0, r.l1});
-    }
-    return true;
-}
+    plan.wave = allow_wave && cnt > 0 && tot <= 128 * cnt;
+    const int64_t chunk = plan.wave ? 64 : kBlock;
+    auto pieces = [chunk](int64_t a, int64_t b) { return b > a ? (b - a + chunk - 1) / chunk : 0; };
 
-static void plan_ranges(const int64_t* tb, const int64_t* te, const int64_t* sb, const int64_t* se, int64_t npairs,
-                        RangePlan& plan, int64_t chunk = kBlock)
-{
-    if (plan_ranges_leaves(tb, te, sb, se, npairs, chunk, plan)) return;
-    plan.items.clear();
-    plan.src.clear();
+    // The usual FMM direct list: every target range is a leaf, so ranges are
+    // identical or disjoint, and the pairs of one target range are listed
+    // together.  Then the list itself is the per-interval source list.
+    bool leaves = true;
+    {
+        std::vector<size_t> order(runs.size());
+        for (size_t r = 0; r < runs.size(); ++r) order[r] = r;
+        std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return runs[a].tb < runs[b].tb; });
+        for (size_t r = 1; r < order.size() && leaves; ++r) {
+            const Run& p = runs[order[r - 1]];
+            const Run& q = runs[order[r]];
+            if (q.tb < p.te || (q.tb == p.tb && q.te > q.tb && p.te > p.tb)) leaves = false;   // overlap or split run
+        }
+    }
+    if (leaves) {
+        int64_t nitems = 0;
+        for (const Run& r : runs) nitems += pieces(r.tb, r.te);
+        if (int rc = stage_plan(f, nitems, npairs, plan)) return rc;
+        for (int64_t k = 0; k < npairs; ++k) {
+            if (sb[k] < 0 || se[k] > np || sb[k] > se[k]) return range_error(k);
+            plan.src[k] = make_int2((int)sb[k], (int)se[k]);
+        }
+        int64_t m = 0;
+        for (const Run& r : runs)
+            for (int64_t t = r.tb; t < r.te; t += chunk)
+                plan.items[m++] = {t, std::min<int64_t>(t + chunk, r.te), r.l0, r.l1};
+        return VPM_OK;
+    }
+
+    // general lists: elementary intervals between all range ends
     std::vector<int64_t> cuts;
     cuts.reserve(2 * (size_t)npairs);
-    for (int64_t k = 0; k < npairs; ++k)
+    for (int64_t k = 0; k < npairs; ++k) {
+        if (sb[k] < 0 || se[k] > np || sb[k] > se[k]) return range_error(k);
         if (tb[k] < te[k] && sb[k] < se[k]) { cuts.push_back(tb[k]); cuts.push_back(te[k]); }
-    if (cuts.empty()) return;
+    }
+    if (cuts.empty()) return VPM_OK;
     std::sort(cuts.begin(), cuts.end());
     cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
     const size_t nint = cuts.size() - 1;
@@ -264,36 +303,39 @@ static void plan_ranges(const int64_t* tb, const int64_t* te, const int64_t* sb,
         for (size_t m = lo; m < hi; ++m) count[m + 1]++;
     }
     for (size_t m = 0; m < nint; ++m) count[m + 1] += count[m];
-    plan.src.assign((size_t)count[nint], make_int2(0, 0));
+    int64_t nitems = 0;
+    for (size_t m = 0; m < nint; ++m)
+        if (count[m + 1] != count[m]) nitems += pieces(cuts[m], cuts[m + 1]);
+    if (int rc = stage_plan(f, nitems, count[nint], plan)) return rc;
     std::vector<int64_t> fill(count.begin(), count.end() - 1);
     for (int64_t k = 0; k < npairs; ++k) {
         if (!(tb[k] < te[k] && sb[k] < se[k])) continue;
         const size_t lo = std::lower_bound(cuts.begin(), cuts.end(), tb[k]) - cuts.begin();
         const size_t hi = std::lower_bound(cuts.begin(), cuts.end(), te[k]) - cuts.begin();
-        for (size_t m = lo; m < hi; ++m) {
-            plan.src[(size_t)fill[m]] = make_int2((int)sb[k], (int)se[k]);
-            fill[m]++;
-        }
+        for (size_t m = lo; m < hi; ++m) plan.src[fill[m]++] = make_int2((int)sb[k], (int)se[k]);
     }
+    int64_t q = 0;
     for (size_t m = 0; m < nint; ++m) {
         if (count[m + 1] == count[m]) continue;
         for (int64_t t = cuts[m]; t < cuts[m + 1]; t += chunk)
-            plan.items.push_back({t, std::min<int64_t>(t + chunk, cuts[m + 1]), count[m], count[m + 1]});
+            plan.items[q++] = {t, std::min<int64_t>(t + chunk, cuts[m + 1]), count[m], count[m + 1]};
     }
+    return VPM_OK;
 }
 
+// One copy of the staged plan to the device.  The pinned source stays in use
+// until the stream is synchronized, which every entry point does before it
+// returns.
 static int upload_plan(vpm_field f, const RangePlan& plan, RangeItem*& d_items, int2*& d_src)
 {
-    const size_t bytes_items = sizeof(RangeItem) * plan.items.size();
-    const size_t bytes_list = sizeof(int2) * plan.src.size();
-    if (int rc = ensure(f->rng, bytes_items + bytes_list + 64)) return rc;
+    const size_t bytes_items = sizeof(RangeItem) * (size_t)plan.nitems;
+    const size_t bytes = bytes_items + sizeof(int2) * (size_t)plan.nsrc;
+    if (int rc = ensure(f->rng, bytes + 64)) return rc;
     char* base = (char*)f->rng.p;
     d_items = (RangeItem*)base;
     d_src = (int2*)(base + bytes_items);
-    hipError_t e;
-    if ((e = hipMemcpyAsync(d_items, plan.items.data(), bytes_items, hipMemcpyHostToDevice, f->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(d_src, plan.src.data(), bytes_list, hipMemcpyHostToDevice, f->stream)) != hipSuccess)
-        return hip_fail(e, "hipMemcpyAsync(range lists)");
+    hipError_t e = hipMemcpyAsync(base, f->hstage.p, bytes, hipMemcpyHostToDevice, f->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(range lists)");
     return VPM_OK;
 }
 
@@ -302,23 +344,17 @@ int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb,
 {
     // F32C over short target intervals (FMM leaves, mean <= 128 targets):
     // one wave per <= 64 targets; otherwise one block per <= 256 targets
-    bool wave = false;
-    if (precision != VPM_PREC_F64 && npairs > 0) {
-        int64_t tot = 0, cnt = 0;
-        for (int64_t k = 0; k < npairs; ++k)
-            if (te[k] > tb[k] && (k == 0 || tb[k] != tb[k - 1] || te[k] != te[k - 1])) { tot += te[k] - tb[k]; ++cnt; }
-        wave = cnt > 0 && tot <= 128 * cnt;
-    }
     RangePlan plan;
-    plan_ranges(tb, te, sb, se, npairs, plan, wave ? 64 : kBlock);
-    if (plan.items.empty()) return VPM_OK;
-    Staged st;
     int rc;
+    if ((rc = plan_ranges(f, tb, te, sb, se, npairs, precision != VPM_PREC_F64, plan))) return rc;
+    if (plan.nitems == 0) return VPM_OK;
+    const bool wave = plan.wave;
+    Staged st;
     if ((rc = stage_field(f, kernel, precision, false, st))) return rc;
     RangeItem* d_items;
     int2* d_src;
     if ((rc = upload_plan(f, plan, d_items, d_src))) return rc;
-    const dim3 grid((unsigned)plan.items.size());
+    const dim3 grid((unsigned)plan.nitems);
     hipError_t e;
     timing_begin(f);
     if (precision == VPM_PREC_F64) {
@@ -329,7 +365,7 @@ int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb,
         default: return fail(VPM_EINVAL, "unknown kernel id");
         }
     } else if (wave) {
-        const int64_t ni = (int64_t)plan.items.size();
+        const int64_t ni = plan.nitems;
         const dim3 wgrid((unsigned)((ni + kBlock / 64 - 1) / (kBlock / 64)));
         switch (kernel) {
 #define VPM_RW(KID) case KID: hipLaunchKernelGGL(ranges_wave_f32c_kernel<KID>, wgrid, dim3(kBlock), 0, f->stream, d_items, ni, d_src, st.pos, st.ld, st.gs, st.cut, f->P); break;
@@ -348,7 +384,7 @@ int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb,
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "p2p_ranges kernel");
     timing_end(f, precision == VPM_PREC_F64 ? 1 : 0);
-    // the host vectors must outlive the asynchronous copies
+    // the pinned plan must outlive the asynchronous copy
     e = hipStreamSynchronize(f->stream);
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     return VPM_OK;
@@ -576,13 +612,13 @@ int launch_estr_ranges(vpm_field f, int kernel, int precision, int transposed, c
                        const int64_t* te, const int64_t* sb, const int64_t* se, int64_t npairs)
 {
     RangePlan plan;
-    plan_ranges(tb, te, sb, se, npairs, plan);
-    if (plan.items.empty()) return VPM_OK;
     int rc;
+    if ((rc = plan_ranges(f, tb, te, sb, se, npairs, false, plan))) return rc;
+    if (plan.nitems == 0) return VPM_OK;
     RangeItem* d_items;
     int2* d_src;
     if ((rc = upload_plan(f, plan, d_items, d_src))) return rc;
-    const dim3 grid((unsigned)plan.items.size());
+    const dim3 grid((unsigned)plan.nitems);
     timing_begin(f);
     if (precision == VPM_PREC_F64 || kernel == VPM_KERNEL_SINGULAR) {
         switch (kernel) {
@@ -746,21 +782,15 @@ int launch_zeta_ranges(vpm_field f, int kernel, int precision, const int64_t* tb
                        const int64_t* sb, const int64_t* se, int64_t npairs)
 {
     const bool f64 = precision == VPM_PREC_F64 || kernel == VPM_KERNEL_SINGULAR;
-    bool wave = false;
-    if (!f64 && npairs > 0) {
-        int64_t tot = 0, cnt = 0;
-        for (int64_t k = 0; k < npairs; ++k)
-            if (te[k] > tb[k] && (k == 0 || tb[k] != tb[k - 1] || te[k] != te[k - 1])) { tot += te[k] - tb[k]; ++cnt; }
-        wave = cnt > 0 && tot <= 128 * cnt;
-    }
     RangePlan plan;
-    plan_ranges(tb, te, sb, se, npairs, plan, wave ? 64 : kBlock);
-    if (plan.items.empty()) return VPM_OK;
     int rc;
+    if ((rc = plan_ranges(f, tb, te, sb, se, npairs, !f64, plan))) return rc;
+    if (plan.nitems == 0) return VPM_OK;
+    const bool wave = plan.wave;
     RangeItem* d_items;
     int2* d_src;
     if ((rc = upload_plan(f, plan, d_items, d_src))) return rc;
-    const dim3 grid((unsigned)plan.items.size());
+    const dim3 grid((unsigned)plan.nitems);
     timing_begin(f);
     if (f64) {
         switch (kernel) {

@@ -49,6 +49,28 @@ int ensure(DevBuf& b, size_t bytes)
     return VPM_OK;
 }
 
+int ensure_pinned(DevBuf& b, size_t bytes)
+{
+    if (b.bytes >= bytes) return VPM_OK;
+    const size_t had = b.bytes;
+    if (b.p) {
+        hipError_t e = hipHostFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+        if (e != hipSuccess) return hip_fail(e, "hipHostFree");
+    }
+    // grow by at least a half, so that lists growing step by step reallocate rarely
+    size_t want = std::max<size_t>(bytes, 1 << 16);
+    want = std::max(want, had + had / 2);
+    hipError_t e = hipHostMalloc(&b.p, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return hip_fail(e, "hipHostMalloc");
+    }
+    b.bytes = want;
+    return VPM_OK;
+}
+
 static void release(DevBuf& b)
 {
     if (b.p) (void)hipFree(b.p);
@@ -229,6 +251,9 @@ int vpm_destroy(vpm_field f)
         DevBuf* bufs[] = {&f->pos, &f->gs, &f->cut, &f->g64, &f->part, &f->keys, &f->keys2, &f->perm,
                           &f->iota, &f->sorttmp, &f->bbox, &f->flag, &f->adj, &f->rng, &f->bar, &f->xfer};
         for (DevBuf* b : bufs) release(*b);
+        if (f->hstage.p) (void)hipHostFree(f->hstage.p);
+        f->hstage.p = nullptr;
+        f->hstage.bytes = 0;
         for (vpm_tape t : f->tapes) {   // detach: the tape stays valid only for vpm_tape_destroy
             release(t->rec);
             t->f = nullptr;
@@ -396,10 +421,7 @@ int vpm_p2p_ranges(vpm_field f, const vpm_config* cfg, const int64_t* tb, const 
     if (npairs < 0) return fail(VPM_EINVAL, "npairs < 0");
     if (npairs == 0) return VPM_OK;
     if (!tb || !te || !sb || !se) return fail(VPM_EINVAL, "null range array");
-    for (int64_t k = 0; k < npairs; ++k) {
-        if (tb[k] < 0 || te[k] > f->np || tb[k] > te[k] || sb[k] < 0 || se[k] > f->np || sb[k] > se[k])
-            return fail(VPM_EINVAL, "range pair " + std::to_string(k) + " outside [0, np)");
-    }
+    // each pair is checked against [0, np) while its plan is built
     if (int rc = launch_p2p_ranges(f, cfg->kernel, cfg->precision, tb, te, sb, se, npairs)) return rc;
     return sync(f);
 }
@@ -412,10 +434,7 @@ int vpm_estr_ranges(vpm_field f, const vpm_config* cfg, const int64_t* tb, const
     if (npairs < 0) return fail(VPM_EINVAL, "npairs < 0");
     if (npairs == 0) return VPM_OK;
     if (!tb || !te || !sb || !se) return fail(VPM_EINVAL, "null range array");
-    for (int64_t k = 0; k < npairs; ++k) {
-        if (tb[k] < 0 || te[k] > f->np || tb[k] > te[k] || sb[k] < 0 || se[k] > f->np || sb[k] > se[k])
-            return fail(VPM_EINVAL, "range pair " + std::to_string(k) + " outside [0, np)");
-    }
+    // each pair is checked against [0, np) while its plan is built
     if (int rc = launch_estr_ranges(f, cfg->kernel, cfg->precision, cfg->transposed, tb, te, sb, se, npairs))
         return rc;
     return sync(f);
@@ -429,10 +448,7 @@ int vpm_zeta_ranges(vpm_field f, const vpm_config* cfg, const int64_t* tb, const
     if (npairs < 0) return fail(VPM_EINVAL, "npairs < 0");
     if (npairs == 0) return VPM_OK;
     if (!tb || !te || !sb || !se) return fail(VPM_EINVAL, "null range array");
-    for (int64_t k = 0; k < npairs; ++k) {
-        if (tb[k] < 0 || te[k] > f->np || tb[k] > te[k] || sb[k] < 0 || se[k] > f->np || sb[k] > se[k])
-            return fail(VPM_EINVAL, "range pair " + std::to_string(k) + " outside [0, np)");
-    }
+    // each pair is checked against [0, np) while its plan is built
     if (int rc = launch_zeta_ranges(f, cfg->kernel, cfg->precision, tb, te, sb, se, npairs)) return rc;
     return sync(f);
 }

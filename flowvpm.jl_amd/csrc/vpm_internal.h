@@ -64,6 +64,7 @@ struct vpm_field_s {
     vpm::DevBuf rng;                 // leaf-pair range lists
     vpm::DevBuf bar;                 // cotangent records of the step adjoint
     vpm::DevBuf xfer;                // gathered rows for partial downloads
+    vpm::DevBuf hstage;              // pinned host staging of range plans (hipHostMalloc)
     bool profiling = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     vpm::Timing timing[5];      // U/J F32C, U/J F64, E_str, adjoint, zeta
@@ -88,6 +89,9 @@ int fail(int status, const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 // ensure a grow-only buffer holds at least `bytes`
 int ensure(DevBuf& b, size_t bytes);
+// the same for pinned host memory; the caller has synchronized the field's
+// stream since the buffer's last use as a copy source
+int ensure_pinned(DevBuf& b, size_t bytes);
 
 // Launchers (stream-ordered on f->stream).  Return a vpm_status.
 struct Staged {

@@ -85,3 +85,35 @@ def test_first_upload_survives_the_zero_fill(vpm):
         f.get_particles(G)
         f.close()
         assert np.array_equal(G, P), cap
+
+
+@pytest.mark.parametrize("entry", ["p2p", "estr", "zeta"])
+def test_range_lists_checked_before_any_work(vpm, entry):
+    """Each pair of a list is checked (0 <= begin <= end <= np) on both plan
+    paths (leaf form: equal or disjoint target ranges listed together;
+    general form: overlapping target ranges); a bad pair anywhere fails the
+    call with VPM_EINVAL naming it, and nothing is computed."""
+    api = vpm.api
+    n = 300
+    P = vpm.workloads.random_cloud(n, seed=12)
+    P[:, 9:24] = 0.25
+    P[:, 39:42] = 0.5
+    fn = {"p2p": lambda A, *r: vpm.p2p_ranges(A, *r, kernel="gaussianerf", precision="f32c"),
+          "estr": lambda A, *r: api.estr_ranges(A, *r, kernel="gaussianerf", precision="f32c"),
+          "zeta": lambda A, *r: api.zeta_ranges(A, *r, kernel="gaussianerf", precision="f32c")}[entry]
+    leaf = ([0, 0, 100, 100], [100, 100, 200, 200], [0, 100, 0, 100], [100, 200, 100, 200])
+    general = ([0, 50, 100], [150, 250, 300], [0, 100, 200], [100, 200, 300])
+    for tb, te, sb, se in (leaf, general):
+        A = P.copy()
+        fn(A, tb, te, sb, se)          # the valid list runs
+        for k, col, bad in ((len(tb) - 1, 3, n + 1), (1, 2, -1), (len(tb) - 1, 1, n + 5), (1, 0, -3)):
+            lists = [list(tb), list(te), list(sb), list(se)]
+            lists[col][k] = bad
+            A = P.copy()
+            with pytest.raises(vpm.VPMError) as ei:
+                fn(A, *lists)
+            assert ei.value.status == -1 and ("range pair %d" % k) in str(ei.value), str(ei.value)
+            assert np.array_equal(A, P)
+            G = np.zeros_like(P)
+            api._scratch_field(n, 0).get_particles(G)       # the device copy the call was given
+            assert np.array_equal(G, P)
