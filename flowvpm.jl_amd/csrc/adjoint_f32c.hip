@@ -87,7 +87,9 @@ __device__ __forceinline__ void adj_coef<VPM_KERNEL_GAUSSIANERF>(float r2, float
     const float E = ex2(-0.5f * kLog2e * t);
     const float c2E = (float)kConst2 * E;
     const float c2qE = c2E * q;
-    const float g = erff(q * 0.70710678118654752f) - c2qE;
+    // erf(q/sqrt2) = 1 - E S(1/q)/q above the series (kErfcS)
+    const float w = rsq(t);
+    const float g = fmaf(-E, fmaf(horner(kErfcS, w), w, (float)kConst2 * q), 1.f);
     const float hm = fmaf(-(1.f / 3.f) * t, c2qE, g);          // g - q g'/3
     const float kn = fmaf(-c2qE * t, 5.f + t, 15.f * g);        // q^2 g'' - 7 q g' + 15 g
     const bool sm1 = t < kErfFitTMax, sm2 = t < kErfPhi2TMax;

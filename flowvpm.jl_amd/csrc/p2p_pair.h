@@ -63,7 +63,10 @@ __device__ __forceinline__ void coef_full<VPM_KERNEL_SINGULAR>(float r2, float, 
 template <>
 __device__ __forceinline__ void coef_full<VPM_KERNEL_GAUSSIANERF>(float r2, float sinv, float& Fp, float& Hpp)
 {
-    // g = erf(q/sqrt2) - sqrt(2/pi) q e^{-q^2/2},  q g' = sqrt(2/pi) q^3 e^{-q^2/2}
+    // g = erf(q/sqrt2) - sqrt(2/pi) q e^{-q^2/2},  q g' = sqrt(2/pi) q^3 e^{-q^2/2};
+    // above the series, erfc(q/sqrt2) = E S(1/q)/q with E = e^{-q^2/2}, so
+    // g = 1 - E (S(1/q)/q + sqrt(2/pi) q) (kErfcS; the unselected value at
+    // small q may be non-finite)
     const float ri = r2 > 0.f ? rsq(r2) : 0.f;
     const float q = r2 * ri * sinv;
     const float t = q * q;
@@ -72,9 +75,10 @@ __device__ __forceinline__ void coef_full<VPM_KERNEL_GAUSSIANERF>(float r2, floa
     const float Fs = horner(kErfPhi0, t) * s3;
     const float Hs = horner(kErfPhi1, t) * s5;
     const float E = ex2(-0.5f * kLog2e * t);
-    const float c2qE = (float)kConst2 * q * E;
-    const float g = erff(q * 0.70710678118654752f) - c2qE;
-    const float hm = fmaf(-(1.f / 3.f) * t, c2qE, g);       // g - q g'/3
+    const float c2q = (float)kConst2 * q;
+    const float w = rsq(t);
+    const float g = fmaf(-E, fmaf(horner(kErfcS, w), w, c2q), 1.f);
+    const float hm = fmaf(-(1.f / 3.f) * t, c2q * E, g);    // g - q g'/3
     const float ri2 = ri * ri;
     const float ri3 = ri2 * ri;
     const bool small = t < kErfFitTMax;
