@@ -159,3 +159,57 @@ def test_leaf_direct_list_shape(vpm):
         first.setdefault(t, k)
     assert all(tb[first[t]:first[t] + int(np.sum(tb == t))].tolist() == [t] * int(np.sum(tb == t)) for t in first)
     assert np.all(te - tb <= 64) and np.all(te > tb)
+
+
+def _header_array(path, name):
+    import re
+    with open(path) as fh:
+        text = fh.read()
+    body = re.search(r"%s\[\d+\] = \{([^}]*)\}" % name, text).group(1)
+    return np.array([float(v.strip().rstrip("f")) for v in body.split(",")], dtype=np.float32)
+
+
+def _g_fp32(S, q, drop_erfc=False):
+    """The device form of g = erf(q/sqrt2) - sqrt(2/pi) q e^{-q^2/2} above the
+    series: 1 - E (S(1/q)/q + sqrt(2/pi) q), E = e^{-q^2/2}, evaluated in
+    float32 (Horner in float32, each fma one rounding)."""
+    f32 = np.float32
+    q = q.astype(f32)
+    t = (q * q).astype(f32)
+    E = np.exp((f32(-0.5) * t).astype(f32)).astype(f32)
+    w = (f32(1) / np.sqrt(t)).astype(f32)
+    acc = np.full(q.shape, S[-1], dtype=f32)
+    for c in S[-2::-1]:
+        acc = (acc.astype(np.float64) * w + c).astype(f32)
+    c2q = (f32(np.sqrt(2 / np.pi)) * q).astype(f32)
+    inner = (acc.astype(np.float64) * w + c2q).astype(f32)
+    if drop_erfc:
+        inner = c2q
+    g = (1.0 - E.astype(np.float64) * inner).astype(f32)
+    hm = (g.astype(np.float64) - (f32(1 / 3) * t).astype(np.float64) * (c2q * E).astype(np.float64)).astype(f32)
+    return g, hm
+
+
+def test_erfc_fit_gives_g_to_fp32(vpm):
+    """kErfcS (kernel_fits.h) in the device's closed form of the Gaussian-erf
+    near field reproduces g and g - q g'/3 (the numerators of F' and H'') to
+    FP32 accuracy against 30-digit mpmath, for every q from the series'
+    threshold (1.5) past the far cut (7), where the fit is only extrapolated
+    and E < 2.3e-11.  Dropping the erfc term fails the check."""
+    import mpmath as mp
+    mp.mp.dps = 30
+    csrc = os.path.join(os.path.dirname(vpm.__file__), "csrc")
+    S = _header_array(os.path.join(csrc, "kernel_fits.h"), "kErfcS")
+    q = np.concatenate([np.linspace(1.5, 7.0, 1500), np.linspace(7.0, 12.0, 200)])
+    g32, hm32 = _g_fp32(S, q)
+    c2 = mp.sqrt(2 / mp.pi)
+    g_ref = np.array([float(mp.erf(mp.mpf(float(x)) / mp.sqrt(2)) - c2 * float(x) * mp.exp(-mp.mpf(float(x)) ** 2 / 2))
+                      for x in q.astype(np.float32)])
+    hm_ref = np.array([float(gr - c2 * mp.mpf(float(x)) ** 3 * mp.exp(-mp.mpf(float(x)) ** 2 / 2) / 3)
+                       for gr, x in zip(g_ref, q.astype(np.float32))])
+    eg = np.abs(g32 - g_ref) / np.abs(g_ref)
+    eh = np.abs(hm32 - hm_ref) / np.abs(hm_ref)
+    assert eg.max() < 4e-7, eg.max()
+    assert eh.max() < 8e-7, eh.max()
+    gbad, _ = _g_fp32(S, q, drop_erfc=True)
+    assert (np.abs(gbad - g_ref) / np.abs(g_ref)).max() > 1e-3
