@@ -117,3 +117,82 @@ def test_range_lists_checked_before_any_work(vpm, entry):
             G = np.zeros_like(P)
             api._scratch_field(n, 0).get_particles(G)       # the device copy the call was given
             assert np.array_equal(G, P)
+
+
+def _ranges_call(vpm, f, tb, te, sb, se, kernel="gaussianerf"):
+    import ctypes as C
+    arrs = [np.ascontiguousarray(a, dtype=np.int64) for a in (tb, te, sb, se)]
+    cfg = vpm.api.make_config(kernel, "f32c", True)
+    ptr = [a.ctypes.data_as(vpm._lib._i64p) for a in arrs]
+    vpm._lib.check(vpm._lib.lib().vpm_p2p_ranges(f.handle, C.byref(cfg), *ptr, len(arrs[0])))
+
+
+def test_concurrent_calls(vpm):
+    """Threads calling vpm_p2p_ranges on one field at once, each with its own
+    target leaves (as FastMultipole's threaded direct! does), give the same
+    records bit for bit as one call over the whole list: each call holds the
+    field's lock, and a target's pairs are summed in list order within one
+    call.  Meanwhile other threads run whole U/J evaluations on fields of
+    their own, which match a sequential run bit for bit."""
+    import threading
+    api = vpm.api
+    P = vpm.workloads.morton_order(vpm.workloads.random_cloud(4096, seed=21))
+    n = P.shape[0]
+    tb, te, sb, se = vpm.workloads.leaf_direct_list(P, 64)
+    ref = api.Field(n, 0)
+    ref.set_particles(P)
+    _ranges_call(vpm, ref, tb, te, sb, se)
+    R = np.zeros_like(P)
+    ref.get_particles(R)
+    ref.close()
+    others = [vpm.workloads.random_cloud(3000 + 500 * k, seed=30 + k) for k in range(3)]
+    seq = []
+    for Q in others:
+        g = api.Field(Q.shape[0], 0)
+        g.set_particles(Q)
+        g.uj_direct("gaussianerf", "f32c")
+        S = np.zeros_like(Q)
+        g.get_particles(S)
+        g.close()
+        seq.append(S)
+
+    f = api.Field(n, 0)
+    f.set_particles(P)
+    leaf = tb // 64
+    errors = []
+    outs = [None] * len(others)
+
+    def leaves(k):
+        try:
+            m = leaf % 4 == k
+            _ranges_call(vpm, f, tb[m], te[m], sb[m], se[m])
+        except Exception as e:          # pragma: no cover - reported below
+            errors.append(e)
+
+    def whole(k):
+        try:
+            Q = others[k]
+            g = api.Field(Q.shape[0], 0)
+            g.set_particles(Q)
+            g.uj_direct("gaussianerf", "f32c")
+            S = np.zeros_like(Q)
+            g.get_particles(S)
+            g.close()
+            outs[k] = S
+        except Exception as e:          # pragma: no cover
+            errors.append(e)
+
+    threads = [threading.Thread(target=leaves, args=(k,)) for k in range(4)]
+    threads += [threading.Thread(target=whole, args=(k,)) for k in range(len(others))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in threads)
+    assert not errors, errors
+    G = np.zeros_like(P)
+    f.get_particles(G)
+    f.close()
+    assert np.array_equal(G, R)
+    for S, Z in zip(outs, seq):
+        assert np.array_equal(S, Z)
