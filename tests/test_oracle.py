@@ -737,3 +737,55 @@ def test_update_pullback_vs_reference_state_pullback(orc, vpm):
     dG = np.linalg.norm(lit[:, 3:6] - ours[:, 3:6]) / np.linalg.norm(ours[:, 3:6])
     dJ = np.linalg.norm(lit[:, 15:24] - ours[:, 15:24]) / np.linalg.norm(ours[:, 15:24])
     assert dG > 1e-4 and dJ > 1e-4, (dG, dJ)
+
+
+@pytest.mark.parametrize("transposed", [True, False])
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans"])
+def test_dynamic_sfs_coefficient_from_independent_filters(vpm, orc, transposed, kernel):
+    """The oracle's dynamic procedure (sfs_before, the domain-filter UJ with
+    E_str, sfs_after; subfilterscale.jl:447-673) against the coefficient
+    rebuilt from independent evaluations of the two filter widths: U/J and
+    E_str of the field with every non-static sigma scaled by alpha (test
+    filter) and unscaled (domain filter), stretching from metrics.stretching,
+    then
+        nume = (3 alpha - 2) (S_test - S_dom) . Gamma,
+        deno = (E_test - E_dom) . Gamma / (zeta0 / sigma^3),   C = nume / deno
+    with rlxf = 1 (no Lagrangian memory), no clamping, no clipping.  A wrong
+    filter direction (sigma / alpha), a missing factor, a row or scheme mix-up
+    in the oracle shows here."""
+    rng = np.random.default_rng(17)
+    n = 60
+    P = vpm.fields.make_particles(rng.uniform(0, 1, (n, 3)), rng.normal(0, 1e-2, (n, 3)),
+                                  rng.uniform(0.12, 0.2, n))
+    P[7, 42] = 1.0                      # one static particle: skipped throughout
+    alpha = 0.9
+    zeta0 = float(orc.zeta(kernel, 0.0))
+    # the oracle's procedure
+    Q = P.copy()
+    assert orc.sfs_before(Q, kernel, transposed, alpha, 0.0, 1e30) == 0
+    orc.uj_direct(Q, kernel, reset=True, reset_sfs=True, sfs=True, transposed=transposed)
+    assert orc.sfs_after(Q, kernel, transposed, alpha, 1.0, 0.0, 1e30, force_positive=False, clip=False) == 0
+    # the two filter widths, evaluated independently
+    live = P[:, 42] == 0
+    T = P.copy()
+    T[live, 6] *= alpha
+    orc.uj_direct(T, kernel, reset=True, reset_sfs=True, sfs=True, transposed=transposed)
+    D = P.copy()
+    orc.uj_direct(D, kernel, reset=True, reset_sfs=True, sfs=True, transposed=transposed)
+    G = P[:, 3:6]
+    St = vpm.metrics.stretching(T[:, 15:24], G, transposed)
+    Sd = vpm.metrics.stretching(D[:, 15:24], G, transposed)
+    nume = (3 * alpha - 2) * np.sum((St - Sd) * G, axis=1)
+    deno = np.sum((T[:, 39:42] - D[:, 39:42]) * G, axis=1) / (zeta0 / P[:, 6] ** 3)
+    np.testing.assert_allclose(Q[live, 37], nume[live], rtol=1e-9, atol=1e-14 * np.abs(nume).max())
+    np.testing.assert_allclose(Q[live, 38], deno[live], rtol=1e-9, atol=1e-14 * np.abs(deno).max())
+    np.testing.assert_allclose(Q[live, 36], nume[live] / deno[live], rtol=1e-8)
+    assert np.all(Q[live, 27:36] == 0.0)                  # M flushed
+    assert np.array_equal(Q[~live][:, 36:39], P[~live][:, 36:39])     # statics keep their C
+    # the check has teeth: the filter taken the other way round (sigma / alpha) differs
+    W = P.copy()
+    W[live, 6] /= alpha
+    orc.uj_direct(W, kernel, reset=True, reset_sfs=True, sfs=True, transposed=transposed)
+    Sw = vpm.metrics.stretching(W[:, 15:24], G, transposed)
+    wrong = (3 * alpha - 2) * np.sum((Sw - Sd) * G, axis=1)
+    assert np.max(np.abs(wrong[live] - Q[live, 37])) > 1e-3 * np.abs(Q[live, 37]).max()
