@@ -18,11 +18,20 @@
  *   vpmo_direct_ranges equals vpmo_direct2 over the union of the ranges
  *   vpmo_estr_direct   mpmath brute force; zero when all J are equal
  *   vpmo_update_particle_states / vpmo_relax / vpmo_core_spreading_rk3 /
- *   vpmo_sfs_before / vpmo_sfs_after
- *                      exact-arithmetic (fractions) restatements on small
- *                      inputs, plus the single-ring translation speed test
- *                      of test/runtests_singlevortexring.jl:22-23,143
- *   vpmo_uj_adjoint    central finite differences of vpmo_loss_local
+ *   vpmo_sfs_before / vpmo_sfs_after / vpmo_rk3_step
+ *                      PARITY PARTLY UNPINNED: line-by-line restatements,
+ *                      checked by the properties the mathematics fixes --
+ *                      exact free-stream advection, third-order RK3
+ *                      convergence, Pedrizzetti alignment and corrected-norm
+ *                      preservation, sigma^2 growing by 2 nu dt, the dynamic
+ *                      coefficient's bounds and clipping rules, its numerator
+ *                      and denominator rebuilt from independent evaluations
+ *                      of the two filter widths -- and by the single-ring
+ *                      translation speed test of
+ *                      test/runtests_singlevortexring.jl:22-23,143
+ *   vpmo_uj_adjoint    central finite differences of vpmo_loss_local;
+ *                      equals vpmo_uj_adjoint_rrules (the reference's
+ *                      UJS_pullback restated, a cross-check) to FP64 rounding
  *   vpmo_estr_ranges   equals vpmo_estr_direct over the pair list of the whole
  *                      field minus its static rows, and the sum of its
  *                      sub-range lists
