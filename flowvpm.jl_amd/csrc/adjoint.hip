@@ -195,7 +195,7 @@ __global__ __launch_bounds__(kBlock) void adj_source_kernel(const double* __rest
 }
 
 int launch_uj_adjoint_f32c(vpm_field f, int kernel, const double* dU, const double* dJ, double* dX, double* dG,
-                           double* dS, float4* tcot);
+                           double* dS, float4* tcot, bool jbar);
 
 // Device staging of the U/J adjoint: Ubar 3n, Jbar 9n, Xbar 3n, Gbar 3n,
 // Sbar n (doubles), then the sorted FP32 cotangents of the F32C path (4n
@@ -220,7 +220,7 @@ int launch_uj_adjoint_dev(vpm_field f, int kernel, int precision, const AdjBufs&
 {
     const int64_t np = f->np;
     if (np == 0) return VPM_OK;
-    if (precision != VPM_PREC_F64) return launch_uj_adjoint_f32c(f, kernel, b.U, b.J, b.X, b.G, b.S, b.cot);
+    if (precision != VPM_PREC_F64) return launch_uj_adjoint_f32c(f, kernel, b.U, b.J, b.X, b.G, b.S, b.cot, b.jbar);
     const dim3 grid((unsigned)((np + kBlock - 1) / kBlock));
     timing_begin(f);
     switch (kernel) {
@@ -250,6 +250,9 @@ int launch_uj_adjoint(vpm_field f, int kernel, int precision, const double* Ubar
     const size_t n = (size_t)np;
     AdjBufs b;
     if (int rc = adj_buffers(f, b)) return rc;
+    // a loss on U alone (every Jbar zero) takes the kernel without Jbar terms
+    b.jbar = false;
+    for (size_t k = 0; k < 9 * n && !b.jbar; ++k) b.jbar = !(Jbar[k] == 0.0);
     hipError_t e;
     if ((e = hipMemcpyAsync(b.U, Ubar, sizeof(double) * 3 * n, hipMemcpyHostToDevice, f->stream)) != hipSuccess ||
         (e = hipMemcpyAsync(b.J, Jbar, sizeof(double) * 9 * n, hipMemcpyHostToDevice, f->stream)) != hipSuccess)

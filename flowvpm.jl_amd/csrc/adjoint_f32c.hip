@@ -155,8 +155,10 @@ struct AdjTarget {
 
 // One pair: target T (the lane's), source (a = s'_hi, b = (Gamma', 1/sigma),
 // l = s'_lo).  X gets +dl/dd; the rotating source sums R get -dl/dd,
-// dl/dGamma', dl/dsigma.
-template <int K, bool NEAR>
+// dl/dGamma', dl/dsigma.  JB = false is the loss on U alone (Jbar = 0, so
+// tau = 0, w = 0, B = 0): the terms dropped are exact zeros, and every kept
+// term is rounded as in the general form.
+template <int K, bool NEAR, bool JB>
 __device__ __forceinline__ void adj_pair(const AdjTarget& T, const float4 a, const float4 b, const float4 l,
                                          float (&X)[3], float (&R)[kAdjAcc])
 {
@@ -201,6 +203,26 @@ __device__ __forceinline__ void adj_pair(const AdjTarget& T, const float4 a, con
     const float vx = fmaf(dy, b.z, -dz * b.y);
     const float vy = fmaf(dz, b.x, -dx * b.z);
     const float vz = fmaf(dx, b.y, -dy * b.x);
+    if constexpr (!JB) {
+        const float AC = fmaf(T.u2, vz, fmaf(T.u1, vy, T.u0 * vx));
+        const float s = H * AC;
+        const float zx = F * T.u0, zy = F * T.u1, zz = F * T.u2;
+        const float gx = fmaf(s, dx, fmaf(b.y, zz, -b.z * zy));
+        const float gy = fmaf(s, dy, fmaf(b.z, zx, -b.x * zz));
+        const float gz = fmaf(s, dz, fmaf(b.x, zy, -b.y * zx));
+        X[0] += gx;
+        X[1] += gy;
+        X[2] += gz;
+        R[0] = rot1(R[0]) - gx;
+        R[1] = rot1(R[1]) - gy;
+        R[2] = rot1(R[2]) - gz;
+        R[3] = rot1(R[3]) + fmaf(zy, dz, -zz * dy);
+        R[4] = rot1(R[4]) + fmaf(zz, dx, -zx * dz);
+        R[5] = rot1(R[5]) + fmaf(zx, dy, -zy * dx);
+        // the product rounded on its own, as inside fmaf(Hs, B, Fs * AC) below
+        if constexpr (kSigma) R[6] = rot1(R[6]) + __fmul_rn(Fs, AC);
+        return;
+    }
     // w = Jbar d
     const float wx = fmaf(T.c0, dz, fmaf(T.b0, dy, T.a0 * dx));
     const float wy = fmaf(T.c1, dz, fmaf(T.b1, dy, T.a1 * dx));
@@ -235,21 +257,22 @@ __device__ __forceinline__ void adj_pair(const AdjTarget& T, const float4 a, con
 
 // 64 steps over one group.  pa/pg/pl point at the lane's own row of the
 // doubled group arrays; step k reads row lane + k.
-template <int K, bool NEAR>
+template <int K, bool NEAR, bool JB>
 __device__ __forceinline__ void adj_group(const AdjTarget& T, const float4* pa, const float4* pg, const float4* pl,
                                           float (&X)[3], float (&R)[kAdjAcc])
 {
 #pragma unroll 8
     for (int k = 0; k < 64; ++k) {
         const float4 l = NEAR ? pl[k] : make_float4(0.f, 0.f, 0.f, 0.f);
-        adj_pair<K, NEAR>(T, pa[k], pg[k], l, X, R);
+        adj_pair<K, NEAR, JB>(T, pa[k], pg[k], l, X, R);
     }
 }
 
 // partT [nss][3][np]: target-side sums per source super-tile.
 // partS [ntt][7][np]: source-side sums per target super-tile.
 // tcot [4][np] float4: (Ubar, tau_x), (Jbar col 0, tau_y), (col 1, tau_z), (col 2, 0).
-template <int K>
+// JB = false: every Jbar is zero, and only tcot[0] is read.
+template <int K, bool JB>
 __global__ __launch_bounds__(kBlock) void adj_f32c_kernel(
     const double* __restrict__ pos, int64_t ld, const float4* __restrict__ sgs, const float* __restrict__ scut,
     const float4* __restrict__ tcot, int64_t np, int64_t tsz, int64_t ssz, int ntt, int nss, int remap,
@@ -318,12 +341,18 @@ __global__ __launch_bounds__(kBlock) void adj_f32c_kernel(
             split_rel(pos[ld + it], oy, T.y, T.ly);
             split_rel(pos[2 * ld + it], oz, T.z, T.lz);
             {
-                const float4 c0 = tcot[it], c1 = tcot[np + it], c2 = tcot[2 * np + it], c3 = tcot[3 * np + it];
                 const float m = tvalid ? 1.f : 0.f;   // padding targets carry zero cotangents
+                const float4 c0 = tcot[it];
                 T.u0 = m * c0.x; T.u1 = m * c0.y; T.u2 = m * c0.z; T.t0 = m * c0.w;
-                T.a0 = m * c1.x; T.a1 = m * c1.y; T.a2 = m * c1.z; T.t1 = m * c1.w;
-                T.b0 = m * c2.x; T.b1 = m * c2.y; T.b2 = m * c2.z; T.t2 = m * c2.w;
-                T.c0 = m * c3.x; T.c1 = m * c3.y; T.c2 = m * c3.z;
+                if constexpr (JB) {
+                    const float4 c1 = tcot[np + it], c2 = tcot[2 * np + it], c3 = tcot[3 * np + it];
+                    T.a0 = m * c1.x; T.a1 = m * c1.y; T.a2 = m * c1.z; T.t1 = m * c1.w;
+                    T.b0 = m * c2.x; T.b1 = m * c2.y; T.b2 = m * c2.z; T.t2 = m * c2.w;
+                    T.c0 = m * c3.x; T.c1 = m * c3.y; T.c2 = m * c3.z;
+                } else {
+                    T.a0 = T.a1 = T.a2 = T.b0 = T.b1 = T.b2 = T.c0 = T.c1 = T.c2 = 0.f;
+                    T.t1 = T.t2 = 0.f;
+                }
             }
             double prev[3] = {0.0, 0.0, 0.0};
             if (tvalid && !first) {
@@ -346,9 +375,9 @@ __global__ __launch_bounds__(kBlock) void adj_f32c_kernel(
 #pragma unroll
                     for (int c = 0; c < kAdjAcc; ++c) R[c] = 0.f;
                     if (near)
-                        adj_group<K, true>(T, &lpos[g][lane], &lgs[g][lane], &llo[g][lane], X, R);
+                        adj_group<K, true, JB>(T, &lpos[g][lane], &lgs[g][lane], &llo[g][lane], X, R);
                     else
-                        adj_group<K, false>(T, &lpos[g][lane], &lgs[g][lane], &llo[g][lane], X, R);
+                        adj_group<K, false, JB>(T, &lpos[g][lane], &lgs[g][lane], &llo[g][lane], X, R);
                     // after 64 rotations lane l holds the sums of row (l - 1) mod 64
                     const int slot = g * 64 + ((lane + 63) & 63);
 #pragma unroll
@@ -440,7 +469,7 @@ static void adj_tiles(int64_t np, int64_t& tsz, int& ntt, int64_t& ssz, int& nss
 }
 
 int launch_uj_adjoint_f32c(vpm_field f, int kernel, const double* dU, const double* dJ, double* dX, double* dG,
-                           double* dS, float4* tcot)
+                           double* dS, float4* tcot, bool jbar)
 {
     const int64_t np = f->np;
     int rc;
@@ -463,8 +492,14 @@ int launch_uj_adjoint_f32c(vpm_field f, int kernel, const double* dU, const doub
     switch (kernel) {
 #define VPM_ADJ32(KID)                                                                                        \
     case KID:                                                                                                 \
-        hipLaunchKernelGGL(adj_f32c_kernel<KID>, dim3(grid), dim3(kBlock), 0, f->stream, S.pos, S.ld, S.gs,   \
-                           S.cut, (const float4*)tcot, np, tsz, ssz, ntt, nss, remap, partT, partS);          \
+        if (jbar)                                                                                             \
+            hipLaunchKernelGGL((adj_f32c_kernel<KID, true>), dim3(grid), dim3(kBlock), 0, f->stream, S.pos,     \
+                               S.ld, S.gs, S.cut, (const float4*)tcot, np, tsz, ssz, ntt, nss, remap, partT,  \
+                               partS);                                                                        \
+        else                                                                                                  \
+            hipLaunchKernelGGL((adj_f32c_kernel<KID, false>), dim3(grid), dim3(kBlock), 0, f->stream, S.pos,    \
+                               S.ld, S.gs, S.cut, (const float4*)tcot, np, tsz, ssz, ntt, nss, remap, partT,  \
+                               partS);                                                                        \
         break;
         VPM_ADJ32(VPM_KERNEL_SINGULAR)
         VPM_ADJ32(VPM_KERNEL_GAUSSIAN)

@@ -126,6 +126,7 @@ int launch_uj_adjoint(vpm_field f, int kernel, int precision, const double* Ubar
 struct AdjBufs {
     double *U, *J, *X, *G, *S;
     float4* cot;
+    bool jbar = true;   // some Jbar may be nonzero (false: the loss is on U alone)
 };
 int adj_buffers(vpm_field f, AdjBufs& b);
 int launch_uj_adjoint_dev(vpm_field f, int kernel, int precision, const AdjBufs& b);

@@ -155,3 +155,25 @@ def test_adjoint_cfg3(vpm, orc, kernel):
         worst = max(worst, e)
         assert e < (ADJ_SCALED_TOL_SIGMA if col == 6 else ADJ_SCALED_TOL), (k, col, fd, an, mag)
     _say("cfg3 finite differences, worst error / summation scale", kernel, worst)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_adjoint_f32c_loss_on_u_only(vpm, orc, kernel):
+    """A loss on U alone (every Jbar zero) takes the kernel without the Jbar
+    terms.  Against the oracle with the same cotangents, to the bounds of the
+    general kernel; and against the general kernel run with Jbar = 1e-30 R,
+    whose Jbar terms fall below FP32 rounding of every kept term: the same
+    gradients bit for bit, since the terms dropped are exact zeros and the
+    kept ones are rounded in the same order."""
+    P = vpm.workloads.random_cloud(1500, seed=23)
+    w, W = vpm.workloads.adjoint_weights(1500)
+    Z = np.zeros_like(W)
+    ref = orc.uj_adjoint(P, kernel, w, Z)
+    got = vpm.uj_adjoint(P, w, Z, kernel=kernel, precision="f32c")
+    _check(vpm, "u-only " + kernel, got, ref, orc.uj_adjoint_scale(P, kernel, w, Z))
+    gen = vpm.uj_adjoint(P, w, 1e-30 * W, kernel=kernel, precision="f32c")
+    for name, a, b in zip(("X", "Gamma", "sigma"), got, gen):
+        d = np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300)
+        _say("u-only vs general", kernel, name, "%.1e" % d, "identical entries %.4f"
+             % np.mean(np.asarray(a) == np.asarray(b)))
+        assert np.array_equal(np.asarray(a), np.asarray(b)), (kernel, name, d)
