@@ -257,9 +257,14 @@ end
 
 function ChainRulesCore.rrule(::typeof(uj_hip), pfield::vpm.ParticleField; precision=0)
     U, J = uj_hip(pfield; precision)
+    # a cotangent that is structurally zero (a loss on U alone leaves Jbar as
+    # ZeroTangent or nothing) becomes zeros; all-zero Jbar takes the library's
+    # kernel without the Jbar terms
+    dense(x, rows, n) = (x === nothing || x isa ChainRulesCore.AbstractZero) ? zeros(rows, n) :
+                        Matrix{Float64}(ChainRulesCore.unthunk(x))
     function uj_hip_pullback((Ubar, Jbar))
         n = pfield.np
-        Ub = Matrix{Float64}(ChainRulesCore.unthunk(Ubar)); Jb = Matrix{Float64}(ChainRulesCore.unthunk(Jbar))
+        Ub = dense(Ubar, 3, n); Jb = dense(Jbar, 9, n)
         Xb = zeros(3, n); Gb = zeros(3, n); Sb = zeros(n)
         cfg = Ref(config(pfield; precision))
         check(ccall((:vpm_uj_adjoint, libvpm), Cint,
