@@ -422,8 +422,11 @@ int vpm_p2p_ranges(vpm_field f, const vpm_config* cfg, const int64_t* tb, const 
     if (npairs == 0) return VPM_OK;
     if (!tb || !te || !sb || !se) return fail(VPM_EINVAL, "null range array");
     // each pair is checked against [0, np) while its plan is built
-    if (int rc = launch_p2p_ranges(f, cfg->kernel, cfg->precision, tb, te, sb, se, npairs)) return rc;
-    return sync(f);
+    // synchronized on every path: the plan's pinned staging buffer may be the
+    // source of a copy still in flight
+    const int rc = launch_p2p_ranges(f, cfg->kernel, cfg->precision, tb, te, sb, se, npairs);
+    const int rs = sync(f);
+    return rc ? rc : rs;
 }
 
 int vpm_estr_ranges(vpm_field f, const vpm_config* cfg, const int64_t* tb, const int64_t* te, const int64_t* sb,
@@ -435,9 +438,9 @@ int vpm_estr_ranges(vpm_field f, const vpm_config* cfg, const int64_t* tb, const
     if (npairs == 0) return VPM_OK;
     if (!tb || !te || !sb || !se) return fail(VPM_EINVAL, "null range array");
     // each pair is checked against [0, np) while its plan is built
-    if (int rc = launch_estr_ranges(f, cfg->kernel, cfg->precision, cfg->transposed, tb, te, sb, se, npairs))
-        return rc;
-    return sync(f);
+    const int rc = launch_estr_ranges(f, cfg->kernel, cfg->precision, cfg->transposed, tb, te, sb, se, npairs);
+    const int rs = sync(f);
+    return rc ? rc : rs;
 }
 
 int vpm_zeta_ranges(vpm_field f, const vpm_config* cfg, const int64_t* tb, const int64_t* te, const int64_t* sb,
@@ -449,8 +452,9 @@ int vpm_zeta_ranges(vpm_field f, const vpm_config* cfg, const int64_t* tb, const
     if (npairs == 0) return VPM_OK;
     if (!tb || !te || !sb || !se) return fail(VPM_EINVAL, "null range array");
     // each pair is checked against [0, np) while its plan is built
-    if (int rc = launch_zeta_ranges(f, cfg->kernel, cfg->precision, tb, te, sb, se, npairs)) return rc;
-    return sync(f);
+    const int rc = launch_zeta_ranges(f, cfg->kernel, cfg->precision, tb, te, sb, se, npairs);
+    const int rs = sync(f);
+    return rc ? rc : rs;
 }
 
 int vpm_estr_direct(vpm_field f, const vpm_config* cfg)
