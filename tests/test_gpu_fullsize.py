@@ -61,14 +61,17 @@ def test_jet_forward_f32c_vs_f64(vpm, jet):
     assert np.all(np.abs(tr) <= 1e-12 * np.maximum(jn, 1e-300)), np.abs(tr / np.maximum(jn, 1e-300)).max()
 
 
-def test_jet_adjoint_identities(vpm, jet):
+@pytest.mark.parametrize("with_j", [True, False], ids=["loss_U_and_J", "loss_U"])
+def test_jet_adjoint_identities(vpm, jet, with_j):
+    """The identities below, for L = sum U.U + sum J:J (the general kernel)
+    and for L = sum U.U alone (Jbar = 0: the kernel without Jbar terms)."""
     P, f = jet
     n = P.shape[0]
     f.uj_direct("gaussianerf", "f32c")
     A = np.zeros_like(P)
     f.get_particles(A, rows=(9, 24))
     U, J = A[:, 9:12], A[:, 15:24]
-    Ubar, Jbar = U.copy(), J.copy()
+    Ubar, Jbar = U.copy(), (J.copy() if with_j else np.zeros_like(J))
     tu, tj = np.sum(Ubar * U, axis=1), np.sum(Jbar * J, axis=1)
     L = float(np.sum(tu) + np.sum(tj))
     Xb, Gb, Sb = f.uj_adjoint(Ubar, Jbar, "gaussianerf", "f32c")
@@ -87,7 +90,8 @@ def test_jet_adjoint_identities(vpm, jet):
     rhs = -2.0 * tu - 3.0 * tj
     scale_h = np.sum(np.abs(hx)) + np.sum(np.abs(hs)) + np.sum(np.abs(rhs))
     e_hom = abs(float(np.sum(hx) + np.sum(hs) - np.sum(rhs))) / scale_h
-    _say("jet adjoint identities: linearity %.2e translation %s homogeneity %.2e (L %.4e)" % (e_lin, e_tr, e_hom, L))
+    _say("jet adjoint identities (%s): linearity %.2e translation %s homogeneity %.2e (L %.4e)"
+         % ("U and J" if with_j else "U", e_lin, e_tr, e_hom, L))
     assert e_lin < ADJ_IDENTITY_TOL, e_lin
     assert np.all(e_tr < ADJ_IDENTITY_TOL), e_tr
     assert e_hom < ADJ_IDENTITY_TOL, e_hom
