@@ -215,7 +215,9 @@ int vpm_rbf_conjugategradient(vpm_field f, const vpm_config* cfg, int32_t itmax,
  * dL/dGamma (3 np) and dL/dsigma (np), particle-major.  Host arrays;
  * Ubar is 3 np, Jbar 9 np in the J storage order.  cfg->precision selects
  * the fused FP32 pair kernel with FP64 partial sums (VPM_PREC_F32C) or FP64
- * pair arithmetic (VPM_PREC_F64). */
+ * pair arithmetic (VPM_PREC_F64).  With VPM_PREC_F32C and every Jbar zero (a
+ * loss on U alone) a kernel without the Jbar terms runs, with the same
+ * results bit for bit. */
 int vpm_uj_adjoint(vpm_field f, const vpm_config* cfg, const double* Ubar, const double* Jbar,
                    double* Xbar, double* Gbar, double* Sbar);
 
