@@ -82,13 +82,21 @@ __global__ __launch_bounds__(kBlock) void ranges_f32c_kernel(const RangeItem* __
     }
 }
 
-// Wave-granular variant for small target intervals (FMM leaves): each wave
-// owns one item of <= 64 targets and streams that item's source ranges
-// through its own 64-row LDS slot, with no block-wide barrier, so a leaf of
-// 32-64 particles keeps half to all of the wave's lanes busy.  Within a wave
-// LDS accesses complete in issue order; the scheduling barriers keep the
-// compiler from moving the next chunk's stores above this chunk's loads.
-template <int K>
+// F32C over short target intervals (FMM leaves): each wave owns one item of
+// at most 64 / R targets and stages sources through its own 64-row LDS slot,
+// with no block-wide barrier (within a wave LDS accesses complete in issue
+// order; the scheduling barriers keep the compiler from moving the next
+// slot's stores above this slot's loads).  R = 1: one lane per target, each
+// source range in turn through the all-pairs tile code.  R > 1 (leaves of
+// <= 32 particles): each target is replicated over R lane groups, and group
+// g takes every R-th source (g, g + R, ...) of the item's source stream --
+// its ranges in list order, concatenated and staged 64 at a time -- so small
+// leaves still keep all 64 lanes busy.
+// A step of R sources is far when all R are beyond the wave's target box by
+// the far radius (then every pair has r > 0), near otherwise.  Per-lane FP32
+// sums fold into TwoSum pairs every ~64 pairs; at the end the R groups'
+// FP64 values are added in a fixed butterfly and group 0 writes.
+template <int K, int R>
 __global__ __launch_bounds__(kBlock) void ranges_wave_f32c_kernel(const RangeItem* __restrict__ items,
                                                                   int64_t nitems,
                                                                   const int2* __restrict__ lsrc,
@@ -97,6 +105,8 @@ __global__ __launch_bounds__(kBlock) void ranges_wave_f32c_kernel(const RangeIte
                                                                   const float* __restrict__ cut,
                                                                   double* __restrict__ P)
 {
+    static_assert(R == 1 || R == 2 || R == 4 || R == 8, "replication");
+    constexpr int TW = 64 / R;                  // targets per wave
     __shared__ float4 lpos_all[kBlock / 64][64];
     __shared__ float4 lgs_all[kBlock / 64][64];
     __shared__ float4 llo_all[kBlock / 64][64];
@@ -110,7 +120,8 @@ __global__ __launch_bounds__(kBlock) void ranges_wave_f32c_kernel(const RangeIte
     const RangeItem it = items[w];
     const int64_t ib = it.t_begin;
     const double ox = pos[ib], oy = pos[ld + ib], oz = pos[2 * ld + ib];
-    const int64_t t = it.t_begin + lane;
+    const int g = lane / TW;
+    const int64_t t = ib + (lane % TW);
     const bool active = t < it.t_end;
     const int64_t tt = active ? t : ib;
     float tx, ty, tz, tlx, tly, tlz;
@@ -118,43 +129,123 @@ __global__ __launch_bounds__(kBlock) void ranges_wave_f32c_kernel(const RangeIte
     split_rel(pos[ld + tt], oy, ty, tly);
     split_rel(pos[2 * ld + tt], oz, tz, tlz);
     const WaveBox box = wave_box(tx, ty, tz);
-    float sum[kAccF32], err[kAccF32];
+    if constexpr (R == 1) {
+        // one lane per target: each source range in turn, 64 sources at a
+        // time, through the all-pairs tile code; a fold per chunk
+        float sum[kAccF32], err[kAccF32];
 #pragma unroll
-    for (int c = 0; c < kAccF32; ++c) { sum[c] = 0.f; err[c] = 0.f; }
-    for (int64_t l = it.l_begin; l < it.l_end; ++l) {
-        const int2 rs = lsrc[l];
-        const int64_t sb = rs.x, se = rs.y;
-        for (int64_t base = sb; base < se; base += 64) {
-            const int n = (int)((se - base) < 64 ? (se - base) : 64);
-            __builtin_amdgcn_wave_barrier();
-            if (lane < n) {
-                const int64_t j = base + lane;
-                float4 p, q;
-                split_rel(pos[j], ox, p.x, q.x);
-                split_rel(pos[ld + j], oy, p.y, q.y);
-                split_rel(pos[2 * ld + j], oz, p.z, q.z);
-                p.w = cut[j];
-                q.w = 0.f;
-                lpos[lane] = p;
-                llo[lane] = q;
-                lgs[lane] = gs[j];
+        for (int c = 0; c < kAccF32; ++c) { sum[c] = 0.f; err[c] = 0.f; }
+        for (int64_t l = it.l_begin; l < it.l_end; ++l) {
+            const int2 rs = lsrc[l];
+            const int64_t sb = rs.x, se = rs.y;
+            for (int64_t base = sb; base < se; base += 64) {
+                const int n = (int)((se - base) < 64 ? (se - base) : 64);
+                __builtin_amdgcn_wave_barrier();
+                if (lane < n) {
+                    const int64_t j = base + lane;
+                    float4 p, q;
+                    split_rel(pos[j], ox, p.x, q.x);
+                    split_rel(pos[ld + j], oy, p.y, q.y);
+                    split_rel(pos[2 * ld + j], oz, p.z, q.z);
+                    p.w = cut[j];
+                    q.w = 0.f;
+                    lpos[lane] = p;
+                    llo[lane] = q;
+                    lgs[lane] = gs[j];
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                float acc[kAccF32];
+#pragma unroll
+                for (int c = 0; c < kAccF32; ++c) acc[c] = 0.f;
+                tile_f32c<K, 1>(n, lpos, lgs, llo, tx, ty, tz, tlx, tly, tlz, box, acc);
+                fold_twosum(acc, sum, err);
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            float acc[kAccF32];
+        }
+        if (active) {
+            double a[kAccF32];
+#pragma unroll
+            for (int c = 0; c < kAccF32; ++c) a[c] = (double)sum[c] + (double)err[c];
+            add_f32c_to_record(P + t * NF, a);
+        }
+        return;
+    }
+    float sum[kAccF32], err[kAccF32], acc[kAccF32];
+#pragma unroll
+    for (int c = 0; c < kAccF32; ++c) { sum[c] = 0.f; err[c] = 0.f; acc[c] = 0.f; }
+    int since = 0;                              // pairs per lane since the last fold (wave-uniform)
+    int64_t l = it.l_begin, off = 0;            // stream position: range l, offset in it
+    while (l < it.l_end) {
+        // the next <= 64 sources of the stream; lane m takes stream slot m
+        int filled = 0;
+        int64_t j = 0;
+        while (filled < 64 && l < it.l_end) {
+            const int2 rs = lsrc[l];
+            const int64_t len = (int64_t)rs.y - rs.x;
+            const int take = (int)min<int64_t>(len - off, (int64_t)(64 - filled));
+            if (lane >= filled && lane < filled + take) j = rs.x + off + (lane - filled);
+            filled += take;
+            off += take;
+            if (off == len) { ++l; off = 0; }
+        }
+        if (filled == 0) break;                 // only empty ranges were left
+        __builtin_amdgcn_wave_barrier();
+        if (lane < filled) {
+            float4 p, q;
+            split_rel(pos[j], ox, p.x, q.x);
+            split_rel(pos[ld + j], oy, p.y, q.y);
+            split_rel(pos[2 * ld + j], oz, p.z, q.z);
+            p.w = cut[j];
+            q.w = 0.f;
+            lpos[lane] = p;
+            llo[lane] = q;
+            lgs[lane] = gs[j];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        {
+            bool nr = false;
+            if (lane < filled) {
+                const float4 a = lpos[lane];
+                const float ex = fmaxf(fmaxf(box.lo[0] - a.x, a.x - box.hi[0]), 0.f);
+                const float ey = fmaxf(fmaxf(box.lo[1] - a.y, a.y - box.hi[1]), 0.f);
+                const float ez = fmaxf(fmaxf(box.lo[2] - a.z, a.z - box.hi[2]), 0.f);
+                // 1e-4 margin covers the FP32 rounding of the box test
+                nr = fmaf(ez, ez, fmaf(ey, ey, ex * ex)) < a.w * 1.0001f;
+            }
+            const uint64_t nearm = __ballot(nr);
+            for (int k = 0; k * R < filled; ++k) {
+                const int sidx = k * R + g;
+                const bool ok = sidx < filled;
+                if ((nearm >> (k * R)) & ((1ull << R) - 1)) {
+                    if (ok) near_pair<K>(tx, ty, tz, tlx, tly, tlz, lpos[sidx], lgs[sidx], llo[sidx], acc);
+                } else {
+                    if (ok) far_pair<K>(tx, ty, tz, lpos[sidx], lgs[sidx], acc);
+                }
+            }
+        }
+        since += (filled + R - 1) / R;
+        if (since >= 64) {
+            fold_twosum(acc, sum, err);
 #pragma unroll
             for (int c = 0; c < kAccF32; ++c) acc[c] = 0.f;
-            tile_f32c<K, 1>(n, lpos, lgs, llo, tx, ty, tz, tlx, tly, tlz, box, acc);
-            fold_twosum(acc, sum, err);
+            since = 0;
         }
     }
-    if (active) {
-        double a[kAccF32];
+    fold_twosum(acc, sum, err);
+    double a[kAccF32];
 #pragma unroll
-        for (int c = 0; c < kAccF32; ++c) a[c] = (double)sum[c] + (double)err[c];
-        add_f32c_to_record(P + t * NF, a);
+    for (int c = 0; c < kAccF32; ++c) a[c] = (double)sum[c] + (double)err[c];
+    if constexpr (R > 1) {
+#pragma unroll
+        for (int m = TW; m < 64; m <<= 1) {
+#pragma unroll
+            for (int c = 0; c < kAccF32; ++c) a[c] += __shfl_xor(a[c], m);
+        }
     }
+    if (g == 0 && active) add_f32c_to_record(P + t * NF, a);
 }
 
 #pragma clang fp contract(off)
@@ -211,7 +302,8 @@ struct RangePlan {
     RangeItem* items = nullptr;   // in f->hstage
     int2* src = nullptr;          // source ranges (begin, end); positions fit 32 bits
     int64_t nitems = 0, nsrc = 0;
-    bool wave = false;            // F32C over short intervals: one wave per <= 64 targets
+    bool wave = false;            // F32C over short intervals: one wave per <= 64 / R targets
+    int R = 1;                    // lane groups per target of the wave kernel
 };
 
 static int stage_plan(vpm_field f, int64_t nitems, int64_t nsrc, RangePlan& plan)
@@ -235,7 +327,7 @@ static int range_error(int64_t k)
 // (0 <= begin <= end <= np) in the passes that read it, before any device
 // work.
 static int plan_ranges(vpm_field f, const int64_t* tb, const int64_t* te, const int64_t* sb, const int64_t* se,
-                       int64_t npairs, bool allow_wave, RangePlan& plan)
+                       int64_t npairs, bool allow_wave, RangePlan& plan, int rmax = 1)
 {
     plan = RangePlan();
     const int64_t np = f->np;
@@ -252,7 +344,11 @@ static int plan_ranges(vpm_field f, const int64_t* tb, const int64_t* te, const 
         k = m;
     }
     plan.wave = allow_wave && cnt > 0 && tot <= 128 * cnt;
-    const int64_t chunk = plan.wave ? 64 : kBlock;
+    // leaves of <= 8, 16 or 32 targets on average share a wave's lanes 8, 4
+    // or 2 ways
+    plan.R = !plan.wave ? 1 : tot <= 8 * cnt ? 8 : tot <= 16 * cnt ? 4 : tot <= 32 * cnt ? 2 : 1;
+    plan.R = std::min(plan.R, rmax);
+    const int64_t chunk = plan.wave ? 64 / plan.R : kBlock;
     auto pieces = [chunk](int64_t a, int64_t b) { return b > a ? (b - a + chunk - 1) / chunk : 0; };
 
     // The usual FMM direct list: every target range is a leaf, so ranges are
@@ -346,7 +442,7 @@ int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb,
     // one wave per <= 64 targets; otherwise one block per <= 256 targets
     RangePlan plan;
     int rc;
-    if ((rc = plan_ranges(f, tb, te, sb, se, npairs, precision != VPM_PREC_F64, plan))) return rc;
+    if ((rc = plan_ranges(f, tb, te, sb, se, npairs, precision != VPM_PREC_F64, plan, 8))) return rc;
     if (plan.nitems == 0) return VPM_OK;
     const bool wave = plan.wave;
     Staged st;
@@ -368,9 +464,19 @@ int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb,
         const int64_t ni = plan.nitems;
         const dim3 wgrid((unsigned)((ni + kBlock / 64 - 1) / (kBlock / 64)));
         switch (kernel) {
-#define VPM_RW(KID) case KID: hipLaunchKernelGGL(ranges_wave_f32c_kernel<KID>, wgrid, dim3(kBlock), 0, f->stream, d_items, ni, d_src, st.pos, st.ld, st.gs, st.cut, f->P); break;
+#define VPM_RWR(KID, R) hipLaunchKernelGGL((ranges_wave_f32c_kernel<KID, R>), wgrid, dim3(kBlock), 0, f->stream, d_items, ni, d_src, st.pos, st.ld, st.gs, st.cut, f->P)
+#define VPM_RW(KID)                                                                   \
+    case KID:                                                                         \
+        switch (plan.R) {                                                             \
+        case 8: VPM_RWR(KID, 8); break;                                               \
+        case 4: VPM_RWR(KID, 4); break;                                               \
+        case 2: VPM_RWR(KID, 2); break;                                               \
+        default: VPM_RWR(KID, 1); break;                                              \
+        }                                                                             \
+        break;
             VPM_RW(VPM_KERNEL_SINGULAR) VPM_RW(VPM_KERNEL_GAUSSIAN) VPM_RW(VPM_KERNEL_GAUSSIANERF) VPM_RW(VPM_KERNEL_WINCKELMANS)
 #undef VPM_RW
+#undef VPM_RWR
         default: return fail(VPM_EINVAL, "unknown kernel id");
         }
     } else {

@@ -267,14 +267,15 @@ def test_cfg1_full_size_sample(vpm, orc):
             assert np.all(np.isfinite(A[:, 9:24]))
 
 
-@pytest.mark.parametrize("leaf", [16, 64, 200])
+@pytest.mark.parametrize("leaf", [5, 8, 16, 32, 64, 200])
 @pytest.mark.parametrize("precision", ["f32c", "f64"])
 def test_p2p_ranges_leaf_lists(vpm, orc, leaf, precision):
     """FMM-style direct lists: target ranges are leaves (identical or
     disjoint), each target leaf's pairs listed together (the linear-time
     plan), then the same pairs with one target leaf split into two runs (the
-    general interval plan).  Leaves of 16 and 64 take the wave-per-leaf
-    kernel, leaves of 200 the block kernel."""
+    general interval plan).  Leaves of up to 64 take the wave kernel (leaves
+    of <= 8, 16, 32 share each wave's lanes 8, 4, 2 ways), leaves of 200 the
+    block kernel."""
     n = 1500
     P = vpm.workloads.random_cloud(n, seed=26)
     P[:, 6] *= 3.0
@@ -294,3 +295,35 @@ def test_p2p_ranges_leaf_lists(vpm, orc, leaf, precision):
         eu, ej = _uj_err(vpm, A, ref)
         tol = F64_TOL if precision == "f64" else F32_TOL
         assert eu["max"] < tol and ej["max"] < tol, (leaf, eu, ej)
+
+
+@pytest.mark.parametrize("precision", ["f32c", "f64"])
+def test_p2p_ranges_ragged_leaves(vpm, orc, precision):
+    """Leaves of uneven size (3 to 40 particles), lists with empty source
+    ranges and a source range listed twice for one leaf: the wave kernel's
+    source stream crosses range ends inside a 64-source slot, skips empty
+    ranges, and counts a repeated range twice, as the oracle does."""
+    n = 1200
+    P = vpm.workloads.random_cloud(n, seed=31)
+    P[:, 6] *= 3.0
+    rng = np.random.default_rng(32)
+    cuts = [0]
+    while cuts[-1] < n:
+        cuts.append(min(n, cuts[-1] + int(rng.integers(3, 41))))
+    starts, ends = np.array(cuts[:-1]), np.array(cuts[1:])
+    C = np.array([P[a:b, 0:3].mean(0) for a, b in zip(starts, ends)])
+    tb, te, sb, se = [], [], [], []
+    for i in range(len(starts)):
+        for j in range(len(starts)):
+            if np.linalg.norm(C[i] - C[j]) < 0.3:
+                tb.append(starts[i]); te.append(ends[i]); sb.append(starts[j]); se.append(ends[j])
+        tb.append(starts[i]); te.append(ends[i]); sb.append(starts[i]); se.append(starts[i])   # empty range
+        if i % 7 == 0:                                                                           # listed twice
+            tb.append(starts[i]); te.append(ends[i]); sb.append(starts[i]); se.append(ends[i])
+    ref = P.copy()
+    orc.direct_ranges(ref, "gaussianerf", tb, te, sb, se)
+    A = P.copy()
+    vpm.p2p_ranges(A, tb, te, sb, se, kernel="gaussianerf", precision=precision)
+    eu, ej = _uj_err(vpm, A, ref)
+    tol = F64_TOL if precision == "f64" else F32_TOL
+    assert eu["max"] < tol and ej["max"] < tol, (eu, ej)
