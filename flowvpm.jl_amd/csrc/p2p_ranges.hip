@@ -635,6 +635,176 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_f32c_kernel(const RangeIte
     }
 }
 
+// E_str over short target intervals (FMM leaves), as ranges_wave_f32c_kernel:
+// one wave per item of <= 64 / R targets, each target replicated over R lane
+// groups that split the item's source stream (its ranges in list order,
+// staged 64 at a time through the wave's own LDS slot).  A step of R sources
+// is evaluated when some lane's pair is inside its source's cut; the lanes
+// whose own pair is outside add a zeta below FP32 resolution, as the block
+// kernel does for such lanes.  The per-lane (A, B) sums are combined over the
+// R groups in a fixed butterfly before J'^T A - B is formed (it is linear in
+// A and B).  Each lane adds its FP32 sums into FP64 accumulators after
+// kEstrFold of its own sources (a conversion and an FP64 add per component).
+// Chains of 4 keep the error against the summation scale sum |zeta (J_i -
+// J_j)^T Gamma_j| at the block kernel's on 5-particle leaves and below it on
+// longer ones; chains of 16 doubled it on 5-particle leaves for 4-8 % less time.
+constexpr int kEstrFold = 4;
+
+template <int K, int R>
+__global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const RangeItem* __restrict__ items,
+                                                                       int64_t nitems,
+                                                                       const int2* __restrict__ lsrc,
+                                                                       double* __restrict__ P, int transposed)
+{
+    static_assert(R == 1 || R == 2 || R == 4 || R == 8, "replication");
+    constexpr int TW = 64 / R;
+    __shared__ float4 lp_all[kBlock / 64][64];   // s'_hi, a
+    __shared__ float4 lg_all[kBlock / 64][64];   // Gamma, b
+    __shared__ float4 lq_all[kBlock / 64][64];   // J'^T Gamma (or J' Gamma), cut
+    __shared__ float4 ll_all[kBlock / 64][64];   // s'_lo
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t w = (int64_t)blockIdx.x * (kBlock / 64) + wave;
+    if (w >= nitems) return;   // no block barrier below
+    float4* lp = lp_all[wave];
+    float4* lg = lg_all[wave];
+    float4* lq = lq_all[wave];
+    float4* ll = ll_all[wave];
+    const RangeItem it = items[w];
+    const double* r0 = P + it.t_begin * NF;
+    const double ox = r0[RX], oy = r0[RX + 1], oz = r0[RX + 2];
+    double Jr[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) Jr[c] = r0[RJ + c];
+    const int g = lane / TW;
+    const int64_t t = it.t_begin + (lane % TW);
+    const bool active = t < it.t_end;
+    const double* tr = P + (active ? t : it.t_begin) * NF;
+    float tx, ty, tz, tlx, tly, tlz;
+    split_rel(tr[RX], ox, tx, tlx);
+    split_rel(tr[RX + 1], oy, ty, tly);
+    split_rel(tr[RX + 2], oz, tz, tlz);
+    float acc[6];
+    double A[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) { acc[c] = 0.f; A[c] = 0.0; }
+    const double l2e = 1.4426950408889634;
+    int since = 0;
+    int64_t l = it.l_begin, off = 0;
+    while (l < it.l_end) {
+        int filled = 0;
+        int64_t j = 0;
+        while (filled < 64 && l < it.l_end) {
+            const int2 rs = lsrc[l];
+            const int64_t len = (int64_t)rs.y - rs.x;
+            const int take = (int)min<int64_t>(len - off, (int64_t)(64 - filled));
+            if (lane >= filled && lane < filled + take) j = rs.x + off + (lane - filled);
+            filled += take;
+            off += take;
+            if (off == len) { ++l; off = 0; }
+        }
+        if (filled == 0) break;
+        __builtin_amdgcn_wave_barrier();
+        if (lane < filled) {
+            const double* sr = P + j * NF;
+            float4 p, lo;
+            split_rel(sr[RX], ox, p.x, lo.x);
+            split_rel(sr[RX + 1], oy, p.y, lo.y);
+            split_rel(sr[RX + 2], oz, p.z, lo.z);
+            const double sg = sr[RSIG];
+            const double si = 1.0 / sg, s2 = si * si, s3 = s2 * si;
+            double a, b, cut;
+            if (K == VPM_KERNEL_GAUSSIANERF) { a = -0.5 * l2e * s2; b = kConst1 * s3; cut = 49.0 * sg * sg; }
+            else if (K == VPM_KERNEL_GAUSSIAN) { a = -l2e * s3; b = kConst3 * s3; cut = 9.0 * sg * sg; }
+            else { a = s2; b = kConst4 * 7.5 * s3; cut = (double)FLT_MAX; }
+            p.w = (float)a;
+            lo.w = 0.f;
+            const double G0 = sr[RG], G1 = sr[RG + 1], G2 = sr[RG + 2];
+            double Jp[9];
+#pragma unroll
+            for (int c = 0; c < 9; ++c) Jp[c] = sr[RJ + c] - Jr[c];
+            float4 q;
+            if (transposed) {
+                q = make_float4((float)(Jp[0] * G0 + Jp[1] * G1 + Jp[2] * G2),
+                                (float)(Jp[3] * G0 + Jp[4] * G1 + Jp[5] * G2),
+                                (float)(Jp[6] * G0 + Jp[7] * G1 + Jp[8] * G2), (float)cut);
+            } else {
+                q = make_float4((float)(Jp[0] * G0 + Jp[3] * G1 + Jp[6] * G2),
+                                (float)(Jp[1] * G0 + Jp[4] * G1 + Jp[7] * G2),
+                                (float)(Jp[2] * G0 + Jp[5] * G1 + Jp[8] * G2), (float)cut);
+            }
+            lp[lane] = p;
+            ll[lane] = lo;
+            lq[lane] = q;
+            lg[lane] = make_float4((float)G0, (float)G1, (float)G2, (float)b);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int k = 0; k * R < filled; ++k) {
+            const int sidx = k * R + g;
+            const bool ok = sidx < filled;
+            const int sl = ok ? sidx : 0;
+            const float4 a = lp[sl];
+            const float4 c = lq[sl];
+            float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
+            float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            if (__any(ok && r2 < c.w)) {
+                if (ok) {
+                    const float4 lo = ll[sl];
+                    const float4 gg = lg[sl];
+                    dx += tlx - lo.x;
+                    dy += tly - lo.y;
+                    dz += tlz - lo.z;
+                    r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    float z;
+                    if constexpr (K == VPM_KERNEL_GAUSSIANERF) {
+                        z = gg.w * ex2(a.w * r2);
+                    } else if constexpr (K == VPM_KERNEL_GAUSSIAN) {
+                        const float r = r2 > 0.f ? r2 * rsq(r2) : 0.f;
+                        z = gg.w * ex2(a.w * r2 * r);
+                    } else {
+                        const float ww = rsq(fmaf(a.w, r2, 1.f));
+                        const float w2 = ww * ww;
+                        z = gg.w * (w2 * w2) * (w2 * ww);
+                    }
+                    acc[0] = fmaf(z, gg.x, acc[0]);
+                    acc[1] = fmaf(z, gg.y, acc[1]);
+                    acc[2] = fmaf(z, gg.z, acc[2]);
+                    acc[3] = fmaf(z, c.x, acc[3]);
+                    acc[4] = fmaf(z, c.y, acc[4]);
+                    acc[5] = fmaf(z, c.z, acc[5]);
+                }
+            }
+            if (++since == kEstrFold) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) { A[c] += (double)acc[c]; acc[c] = 0.f; }
+                since = 0;
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) A[c] += (double)acc[c];
+    if constexpr (R > 1) {
+#pragma unroll
+        for (int m = TW; m < 64; m <<= 1) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) A[c] += __shfl_xor(A[c], m);
+        }
+    }
+    if (g == 0 && active) {
+        double* rec = P + t * NF;
+        double Jp[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) Jp[c] = rec[RJ + c] - Jr[c];
+        for (int q = 0; q < 3; ++q) {
+            const double S = transposed ? Jp[3 * q] * A[0] + Jp[3 * q + 1] * A[1] + Jp[3 * q + 2] * A[2] - A[3 + q]
+                                        : Jp[q] * A[0] + Jp[3 + q] * A[1] + Jp[6 + q] * A[2] - A[3 + q];
+            rec[RSFS + q] += S;
+        }
+    }
+}
+
 #pragma clang fp contract(off)
 template <int K>
 __device__ __forceinline__ double zeta_ref_r(double r)
@@ -717,16 +887,36 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_f64_kernel(const RangeItem
 int launch_estr_ranges(vpm_field f, int kernel, int precision, int transposed, const int64_t* tb,
                        const int64_t* te, const int64_t* sb, const int64_t* se, int64_t npairs)
 {
+    const bool f64 = precision == VPM_PREC_F64 || kernel == VPM_KERNEL_SINGULAR;
     RangePlan plan;
     int rc;
-    if ((rc = plan_ranges(f, tb, te, sb, se, npairs, false, plan))) return rc;
+    if ((rc = plan_ranges(f, tb, te, sb, se, npairs, !f64, plan, 8))) return rc;
     if (plan.nitems == 0) return VPM_OK;
     RangeItem* d_items;
     int2* d_src;
     if ((rc = upload_plan(f, plan, d_items, d_src))) return rc;
     const dim3 grid((unsigned)plan.nitems);
     timing_begin(f);
-    if (precision == VPM_PREC_F64 || kernel == VPM_KERNEL_SINGULAR) {
+    if (!f64 && plan.wave) {
+        const int64_t ni = plan.nitems;
+        const dim3 wgrid((unsigned)((ni + kBlock / 64 - 1) / (kBlock / 64)));
+        switch (kernel) {
+#define VPM_EWR(KID, R) hipLaunchKernelGGL((estr_ranges_wave_f32c_kernel<KID, R>), wgrid, dim3(kBlock), 0, f->stream, d_items, ni, d_src, f->P, transposed)
+#define VPM_EW(KID)                                                                   \
+    case KID:                                                                         \
+        switch (plan.R) {                                                             \
+        case 8: VPM_EWR(KID, 8); break;                                               \
+        case 4: VPM_EWR(KID, 4); break;                                               \
+        case 2: VPM_EWR(KID, 2); break;                                               \
+        default: VPM_EWR(KID, 1); break;                                              \
+        }                                                                             \
+        break;
+            VPM_EW(VPM_KERNEL_GAUSSIAN) VPM_EW(VPM_KERNEL_GAUSSIANERF) VPM_EW(VPM_KERNEL_WINCKELMANS)
+#undef VPM_EW
+#undef VPM_EWR
+        default: return fail(VPM_EINVAL, "unknown kernel id");
+        }
+    } else if (f64) {
         switch (kernel) {
 #define VPM_E64(KID) case KID: hipLaunchKernelGGL(estr_ranges_f64_kernel<KID>, grid, dim3(kBlock), 0, f->stream, d_items, d_src, f->P, transposed); break;
             VPM_E64(VPM_KERNEL_SINGULAR) VPM_E64(VPM_KERNEL_GAUSSIAN) VPM_E64(VPM_KERNEL_GAUSSIANERF) VPM_E64(VPM_KERNEL_WINCKELMANS)

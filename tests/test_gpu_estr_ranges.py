@@ -62,3 +62,39 @@ def test_estr_ranges_whole_field_equals_estr_direct(vpm, orc):
     A = P.copy()
     vpm.api.estr_ranges(A, [0], [n], [0], [n], kernel="gaussianerf", precision="f64")
     np.testing.assert_allclose(A[:, 39:42], ref[:, 39:42], rtol=1e-11, atol=1e-12 * np.abs(ref[:, 39:42]).max())
+
+
+@pytest.mark.parametrize("leaf", [5, 12, 24, 48, "ragged"])
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans"])
+def test_estr_ranges_small_leaves(vpm, orc, leaf, kernel):
+    """FMM-style lists over leaves of 5, 12, 24 and 48 particles and of uneven
+    size (3 to 40), with an empty source range per leaf: the wave kernel, its
+    lanes shared 8, 4, 2 or 1 ways, against the oracle."""
+    n = 1000
+    P = vpm.workloads.random_cloud(n, seed=41)
+    P[:, 6] *= 3.0
+    rng = np.random.default_rng(42)
+    P[:, 15:24] = rng.normal(0, 1, (n, 9))
+    if leaf == "ragged":
+        cuts = [0]
+        while cuts[-1] < n:
+            cuts.append(min(n, cuts[-1] + int(rng.integers(3, 41))))
+        starts, ends = np.array(cuts[:-1]), np.array(cuts[1:])
+    else:
+        starts = np.arange(0, n, leaf)
+        ends = np.minimum(starts + leaf, n)
+    C = np.array([P[a:b, 0:3].mean(0) for a, b in zip(starts, ends)])
+    tb, te, sb, se = [], [], [], []
+    for i in range(len(starts)):
+        for j in range(len(starts)):
+            if np.linalg.norm(C[i] - C[j]) < 0.3:
+                tb.append(starts[i]); te.append(ends[i]); sb.append(starts[j]); se.append(ends[j])
+        tb.append(starts[i]); te.append(ends[i]); sb.append(starts[i]); se.append(starts[i])
+    ref = P.copy()
+    orc.estr_ranges(ref, kernel, tb, te, sb, se)
+    A = P.copy()
+    vpm.api.estr_ranges(A, tb, te, sb, se, kernel=kernel, precision="f32c")
+    assert np.array_equal(A[:, 0:39], P[:, 0:39])
+    st = vpm.metrics.err_stats(A[:, 39:42], ref[:, 39:42])
+    print("estr_ranges leaves", leaf, kernel, st, flush=True)
+    assert st["max"] < ESTR_MAX_TOL and st["rms"] < ESTR_RMS_TOL, st
