@@ -11,6 +11,7 @@
 // sum runs in pair order.  The pair arithmetic is p2p_pair.h's.
 #include <algorithm>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "p2p_pair.h"
@@ -322,6 +323,44 @@ static int range_error(int64_t k)
     return fail(VPM_EINVAL, "range pair " + std::to_string(k) + " outside [0, np)");
 }
 
+// Runs fn(k0, k1, t) over [0, n) cut into T <= kPlanThreads contiguous chunks
+// of at least 2^20 pairs, chunk t on thread t (the caller's thread takes
+// chunk 0; a chunk whose thread cannot start runs on the caller's).  Returns T.
+constexpr int kPlanThreads = 8;
+template <class Fn>
+static int plan_parallel(int64_t n, Fn&& fn)
+{
+    int T = (int)std::min<int64_t>((n >> 20) + 1, kPlanThreads);
+    const unsigned hc = std::thread::hardware_concurrency();
+    if (hc > 0) T = std::min(T, (int)hc);
+    if (T <= 1) {
+        fn((int64_t)0, n, 0);
+        return 1;
+    }
+    std::vector<std::thread> th;
+    std::vector<int> inline_chunks;
+    for (int t = 1; t < T; ++t) {
+        try {
+            th.emplace_back(fn, n * t / T, n * (t + 1) / T, t);
+        } catch (...) {
+            inline_chunks.push_back(t);
+        }
+    }
+    fn((int64_t)0, n / T, 0);
+    for (int t : inline_chunks) fn(n * t / T, n * (t + 1) / T, t);
+    for (std::thread& x : th) x.join();
+    return T;
+}
+
+// Lowest failing pair index over the chunks' first failures (-1 if none).
+static int64_t first_bad(const int64_t* bad, int T)
+{
+    int64_t k = -1;
+    for (int t = 0; t < T; ++t)
+        if (bad[t] >= 0 && (k < 0 || bad[t] < k)) k = bad[t];
+    return k;
+}
+
 // allow_wave: the caller has a wave kernel, taken when the mean target range
 // of the list is at most 128 targets (FMM leaves).  Every pair is checked
 // (0 <= begin <= end <= np) in the passes that read it, before any device
@@ -331,17 +370,39 @@ static int plan_ranges(vpm_field f, const int64_t* tb, const int64_t* te, const 
 {
     plan = RangePlan();
     const int64_t np = f->np;
-    // runs of consecutive pairs with one target range
+    // runs of consecutive pairs with one target range, found per chunk of the
+    // list (a run belongs to the chunk it starts in) and joined in list order
     struct Run { int64_t tb, te, l0, l1; };
+    struct Part { std::vector<Run> runs; int64_t tot = 0, cnt = 0; };
+    Part parts[kPlanThreads];
+    int64_t bad[kPlanThreads];
+    for (int t = 0; t < kPlanThreads; ++t) bad[t] = -1;
+    auto same = [tb, te](int64_t a, int64_t b) { return tb[a] == tb[b] && te[a] == te[b]; };
+    const int T = plan_parallel(npairs, [&](int64_t k0, int64_t k1, int t) {
+        Part& pt = parts[t];
+        int64_t k = k0;
+        while (k < k1 && k > 0 && same(k, k - 1)) ++k;
+        while (k < k1) {
+            if (tb[k] < 0 || te[k] > np || tb[k] > te[k]) { bad[t] = k; return; }
+            int64_t m = k + 1;
+            while (m < npairs && same(m, k)) ++m;
+            pt.runs.push_back({tb[k], te[k], k, m});
+            if (te[k] > tb[k]) { pt.tot += te[k] - tb[k]; ++pt.cnt; }
+            k = m;
+        }
+    });
+    if (int64_t k = first_bad(bad, T); k >= 0) return range_error(k);
     std::vector<Run> runs;
     int64_t tot = 0, cnt = 0;
-    for (int64_t k = 0; k < npairs;) {
-        if (tb[k] < 0 || te[k] > np || tb[k] > te[k]) return range_error(k);
-        int64_t m = k + 1;
-        while (m < npairs && tb[m] == tb[k] && te[m] == te[k]) ++m;
-        runs.push_back({tb[k], te[k], k, m});
-        if (te[k] > tb[k]) { tot += te[k] - tb[k]; ++cnt; }
-        k = m;
+    {
+        size_t nr = 0;
+        for (int t = 0; t < T; ++t) nr += parts[t].runs.size();
+        runs.reserve(nr);
+        for (int t = 0; t < T; ++t) {
+            runs.insert(runs.end(), parts[t].runs.begin(), parts[t].runs.end());
+            tot += parts[t].tot;
+            cnt += parts[t].cnt;
+        }
     }
     plan.wave = allow_wave && cnt > 0 && tot <= 128 * cnt;
     // leaves of <= 8, 16 or 32 targets on average share a wave's lanes 8, 4
@@ -354,25 +415,40 @@ static int plan_ranges(vpm_field f, const int64_t* tb, const int64_t* te, const 
     // The usual FMM direct list: every target range is a leaf, so ranges are
     // identical or disjoint, and the pairs of one target range are listed
     // together.  Then the list itself is the per-interval source list.
+    // Empty target ranges hold no targets and are left out of the check.  Lists
+    // whose runs come in target order, as FMM lists do, need no sort.
     bool leaves = true;
     {
-        std::vector<size_t> order(runs.size());
-        for (size_t r = 0; r < runs.size(); ++r) order[r] = r;
-        std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return runs[a].tb < runs[b].tb; });
-        for (size_t r = 1; r < order.size() && leaves; ++r) {
-            const Run& p = runs[order[r - 1]];
-            const Run& q = runs[order[r]];
-            if (q.tb < p.te || (q.tb == p.tb && q.te > q.tb && p.te > p.tb)) leaves = false;   // overlap or split run
+        const Run* prev = nullptr;
+        bool sorted = true;
+        for (const Run& r : runs) {
+            if (r.te == r.tb) continue;
+            if (prev && r.tb < prev->tb) { sorted = false; break; }
+            if (prev && r.tb < prev->te) { leaves = false; break; }   // overlap, or one range in two runs
+            prev = &r;
+        }
+        if (!sorted) {
+            std::vector<const Run*> order;
+            for (const Run& r : runs)
+                if (r.te > r.tb) order.push_back(&r);
+            std::sort(order.begin(), order.end(), [](const Run* a, const Run* b) { return a->tb < b->tb; });
+            for (size_t r = 1; r < order.size() && leaves; ++r)
+                if (order[r]->tb < order[r - 1]->te) leaves = false;
         }
     }
     if (leaves) {
         int64_t nitems = 0;
         for (const Run& r : runs) nitems += pieces(r.tb, r.te);
         if (int rc = stage_plan(f, nitems, npairs, plan)) return rc;
-        for (int64_t k = 0; k < npairs; ++k) {
-            if (sb[k] < 0 || se[k] > np || sb[k] > se[k]) return range_error(k);
-            plan.src[k] = make_int2((int)sb[k], (int)se[k]);
-        }
+        for (int t = 0; t < kPlanThreads; ++t) bad[t] = -1;
+        int2* src = plan.src;
+        const int T2 = plan_parallel(npairs, [&](int64_t k0, int64_t k1, int t) {
+            for (int64_t k = k0; k < k1; ++k) {
+                if (sb[k] < 0 || se[k] > np || sb[k] > se[k]) { bad[t] = k; return; }
+                src[k] = make_int2((int)sb[k], (int)se[k]);
+            }
+        });
+        if (int64_t k = first_bad(bad, T2); k >= 0) return range_error(k);
         int64_t m = 0;
         for (const Run& r : runs)
             for (int64_t t = r.tb; t < r.te; t += chunk)

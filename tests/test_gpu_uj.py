@@ -327,3 +327,68 @@ def test_p2p_ranges_ragged_leaves(vpm, orc, precision):
     eu, ej = _uj_err(vpm, A, ref)
     tol = F64_TOL if precision == "f64" else F32_TOL
     assert eu["max"] < tol and ej["max"] < tol, (eu, ej)
+
+
+def test_p2p_ranges_long_list_plan_chunks(vpm, orc):
+    """A leaf list of 2.6M pairs (every 4-particle leaf against every leaf, 40
+    times over), long enough that the host plan splits it over threads with
+    runs of one target leaf crossing the chunk ends: results equal the
+    oracle's, and a bad pair is named by its lowest index whichever chunk
+    finds it (target ranges are checked before source ranges)."""
+    n, leaf, rep = 1024, 4, 40
+    P = vpm.workloads.random_cloud(n, seed=33)
+    P[:, 6] *= 3.0
+    starts = np.arange(0, n, leaf, dtype=np.int64)
+    L = len(starts)
+    tb = np.repeat(starts, rep * L)
+    te = tb + leaf
+    sb = np.tile(starts, rep * L)
+    se = sb + leaf
+    assert len(tb) >= (2 << 20)             # three plan chunks
+    ref = P.copy()
+    orc.direct_ranges(ref, "gaussianerf", tb, te, sb, se)
+    A = P.copy()
+    vpm.p2p_ranges(A, tb, te, sb, se, kernel="gaussianerf", precision="f32c")
+    eu, ej = _uj_err(vpm, A, ref)
+    assert eu["max"] < F32_TOL and ej["max"] < F32_TOL, (eu, ej)
+    cases = ((((3, 2_300_000, -1), (3, 1_100_000, n + 1)), 1_100_000),
+             (((1, 2_000_000, n + 3), (0, 1_500_000, -2), (2, 500_000, -1)), 1_500_000))
+    for bads, first in cases:
+        lists = [tb.copy(), te.copy(), sb.copy(), se.copy()]
+        for col, k, v in bads:
+            lists[col][k] = v
+        A = P.copy()
+        with pytest.raises(vpm.VPMError) as ei:
+            vpm.p2p_ranges(A, *lists, kernel="gaussianerf", precision="f32c")
+        assert ei.value.status == -1 and ("range pair %d " % first) in str(ei.value), str(ei.value)
+        assert np.array_equal(A, P)
+
+
+@pytest.mark.parametrize("precision", ["f32c", "f64"])
+def test_p2p_ranges_leaves_out_of_order(vpm, orc, precision):
+    """Leaf lists whose target leaves come in shuffled order, with an empty
+    target range at the start of some leaves: still one item per leaf, and
+    the results equal the oracle's."""
+    n = 900
+    P = vpm.workloads.random_cloud(n, seed=34)
+    P[:, 6] *= 3.0
+    rng = np.random.default_rng(35)
+    cuts = [0]
+    while cuts[-1] < n:
+        cuts.append(min(n, cuts[-1] + int(rng.integers(4, 30))))
+    starts, ends = np.array(cuts[:-1]), np.array(cuts[1:])
+    C = np.array([P[a:b, 0:3].mean(0) for a, b in zip(starts, ends)])
+    tb, te, sb, se = [], [], [], []
+    for i in rng.permutation(len(starts)):
+        for j in range(len(starts)):
+            if np.linalg.norm(C[i] - C[j]) < 0.3:
+                tb.append(starts[i]); te.append(ends[i]); sb.append(starts[j]); se.append(ends[j])
+        if i % 3 == 0:                                  # empty target range, same begin
+            tb.append(starts[i]); te.append(starts[i]); sb.append(0); se.append(n)
+    ref = P.copy()
+    orc.direct_ranges(ref, "gaussianerf", tb, te, sb, se)
+    A = P.copy()
+    vpm.p2p_ranges(A, tb, te, sb, se, kernel="gaussianerf", precision=precision)
+    eu, ej = _uj_err(vpm, A, ref)
+    tol = F64_TOL if precision == "f64" else F32_TOL
+    assert eu["max"] < tol and ej["max"] < tol, (eu, ej)
