@@ -115,7 +115,9 @@ def leaf_list(vpm, P, kernel, allpairs_rate, leaf=64, reps=3):
     """The near-field service of an FMM (FastMultipole's direct list through
     fmm.direct!, src/FLOWVPM_fmm.jl:102-168) on the jet: leaves of 64
     Morton-consecutive particles, direct pairs within 3 median leaf radii,
-    one vpm_p2p_ranges call per evaluation."""
+    one vpm_p2p_ranges call per evaluation.  Then Estr_fmm! over the same
+    list (src/FLOWVPM_subfilterscale_models.jl:157-188), one vpm_estr_ranges
+    call per evaluation, with the J the list evaluation left."""
     import ctypes as C
     api = vpm.api
     Q = vpm.workloads.morton_order(P)
@@ -135,10 +137,18 @@ def leaf_list(vpm, P, kernel, allpairs_rate, leaf=64, reps=3):
         vpm._lib.check(L.vpm_p2p_ranges(f.handle, C.byref(cfg), *ptr, len(tb)))
     wall = (time.perf_counter() - t0) / reps
     ms, cnt = f.timing(0)
+    vpm._lib.check(L.vpm_estr_ranges(f.handle, C.byref(cfg), *ptr, len(tb)))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        vpm._lib.check(L.vpm_estr_ranges(f.handle, C.byref(cfg), *ptr, len(tb)))
+    ewall = (time.perf_counter() - t0) / reps
+    ems, ecnt = f.timing(2)
     f.close()
     out = {"N": n, "leaf": leaf, "list_pairs": len(tb), "particle_pairs": pairs, "sources_per_target": pairs / n,
            "call_ms": 1e3 * wall, "kernel_ms": ms / cnt, "pairs_per_s_kernel": pairs / (ms / cnt / 1e3),
-           "pairs_per_s_call": pairs / wall, "fraction_of_allpairs_rate": pairs / (ms / cnt / 1e3) / allpairs_rate}
+           "pairs_per_s_call": pairs / wall, "fraction_of_allpairs_rate": pairs / (ms / cnt / 1e3) / allpairs_rate,
+           "estr_call_ms": 1e3 * ewall, "estr_kernel_ms": ems / ecnt,
+           "estr_pairs_per_s_kernel": pairs / (ems / ecnt / 1e3)}
     log("bench: leaf list %s" % out)
     return out
 
@@ -376,6 +386,7 @@ def main():
 
     if not args.no_leaf:
         res["fmm_leaf_list"] = leaf_list(vpm, P, kernel, value)
+        res["fmm_leaf_list_16"] = leaf_list(vpm, P, kernel, value, leaf=16)
 
     if not args.no_reset:
         res["core_spreading_reset"] = core_spreading_reset(vpm, P, kernel)
