@@ -117,7 +117,8 @@ def leaf_list(vpm, P, kernel, allpairs_rate, leaf=64, reps=3):
     Morton-consecutive particles, direct pairs within 3 median leaf radii,
     one vpm_p2p_ranges call per evaluation.  Then Estr_fmm! over the same
     list (src/FLOWVPM_subfilterscale_models.jl:157-188), one vpm_estr_ranges
-    call per evaluation, with the J the list evaluation left."""
+    call per evaluation, with the J the list evaluation left, and zeta_fmm's
+    pair loop (src/FLOWVPM_viscous.jl:523-558), one vpm_zeta_ranges call."""
     import ctypes as C
     api = vpm.api
     Q = vpm.workloads.morton_order(P)
@@ -143,12 +144,20 @@ def leaf_list(vpm, P, kernel, allpairs_rate, leaf=64, reps=3):
         vpm._lib.check(L.vpm_estr_ranges(f.handle, C.byref(cfg), *ptr, len(tb)))
     ewall = (time.perf_counter() - t0) / reps
     ems, ecnt = f.timing(2)
+    vpm._lib.check(L.vpm_zeta_ranges(f.handle, C.byref(cfg), *ptr, len(tb)))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        vpm._lib.check(L.vpm_zeta_ranges(f.handle, C.byref(cfg), *ptr, len(tb)))
+    zwall = (time.perf_counter() - t0) / reps
+    zms, zcnt = f.timing(4)
     f.close()
     out = {"N": n, "leaf": leaf, "list_pairs": len(tb), "particle_pairs": pairs, "sources_per_target": pairs / n,
            "call_ms": 1e3 * wall, "kernel_ms": ms / cnt, "pairs_per_s_kernel": pairs / (ms / cnt / 1e3),
            "pairs_per_s_call": pairs / wall, "fraction_of_allpairs_rate": pairs / (ms / cnt / 1e3) / allpairs_rate,
            "estr_call_ms": 1e3 * ewall, "estr_kernel_ms": ems / ecnt,
-           "estr_pairs_per_s_kernel": pairs / (ems / ecnt / 1e3)}
+           "estr_pairs_per_s_kernel": pairs / (ems / ecnt / 1e3),
+           "zeta_call_ms": 1e3 * zwall, "zeta_kernel_ms": zms / zcnt,
+           "zeta_pairs_per_s_kernel": pairs / (zms / zcnt / 1e3)}
     log("bench: leaf list %s" % out)
     return out
 

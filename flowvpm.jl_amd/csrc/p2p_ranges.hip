@@ -1024,6 +1024,7 @@ int launch_estr_ranges(vpm_field f, int kernel, int precision, int transposed, c
 // source as zeta_params, sources beyond zeta_cut skipped per wave, sums
 // folded into TwoSum every 32 sources and added onto the FP64 record.  BS
 // is the block size: 64 (one wave per FMM leaf) for short items, else 256.
+// Leaves of <= 32 targets on average take zeta_ranges_wave_f32c_kernel.
 template <int K, int BS>
 __global__ __launch_bounds__(BS) void zeta_ranges_f32c_kernel(const RangeItem* __restrict__ items,
                                                               const int2* __restrict__ lsrc,
@@ -1103,6 +1104,124 @@ __global__ __launch_bounds__(BS) void zeta_ranges_f32c_kernel(const RangeItem* _
     }
 }
 
+// zeta over FMM leaves shared R = 2, 4 or 8 ways per wave, as
+// estr_ranges_wave_f32c_kernel: one wave per item of <= 64 / R targets, each
+// target replicated over R lane groups that split the item's source stream
+// (its ranges in list order, staged 64 at a time through the wave's own LDS
+// slot).  FP32 chains of kZetaFold lane sources are added into FP64 sums,
+// which the R groups combine in a fixed butterfly.  The zeta sum has no
+// cancelling J'^T A - B form, so its chains are longer than E_str's.  Leaves
+// of a whole wave (R = 1) take zeta_ranges_f32c_kernel<K, 64>, which was 22 %
+// faster there (6.35 against 8.13 ms on the 2^20 jet's 64-particle leaves).
+constexpr int kZetaFold = 16;
+template <int K, int R>
+__global__ __launch_bounds__(kBlock) void zeta_ranges_wave_f32c_kernel(const RangeItem* __restrict__ items,
+                                                                       int64_t nitems,
+                                                                       const int2* __restrict__ lsrc,
+                                                                       double* __restrict__ P)
+{
+    static_assert(R == 2 || R == 4 || R == 8, "replication");
+    constexpr int TW = 64 / R;
+    __shared__ float4 lp_all[kBlock / 64][64];   // s'_hi, cut
+    __shared__ float4 lg_all[kBlock / 64][64];   // Gamma, b
+    __shared__ float4 ll_all[kBlock / 64][64];   // s'_lo, a
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t w = (int64_t)blockIdx.x * (kBlock / 64) + wave;
+    if (w >= nitems) return;   // no block barrier below
+    float4* lp = lp_all[wave];
+    float4* lg = lg_all[wave];
+    float4* ll = ll_all[wave];
+    const RangeItem it = items[w];
+    const double* r0 = P + it.t_begin * NF;
+    const double ox = r0[RX], oy = r0[RX + 1], oz = r0[RX + 2];
+    const int g = lane / TW;
+    const int64_t t = it.t_begin + (lane % TW);
+    const bool active = t < it.t_end;
+    const double* tr = P + (active ? t : it.t_begin) * NF;
+    float tx, ty, tz, tlx, tly, tlz;
+    split_rel(tr[RX], ox, tx, tlx);
+    split_rel(tr[RX + 1], oy, ty, tly);
+    split_rel(tr[RX + 2], oz, tz, tlz);
+    float acc[3] = {0.f, 0.f, 0.f};
+    double A[3] = {0.0, 0.0, 0.0};
+    int since = 0;
+    int64_t l = it.l_begin, off = 0;
+    while (l < it.l_end) {
+        int filled = 0;
+        int64_t j = 0;
+        while (filled < 64 && l < it.l_end) {
+            const int2 rs = lsrc[l];
+            const int64_t len = (int64_t)rs.y - rs.x;
+            const int take = (int)min<int64_t>(len - off, (int64_t)(64 - filled));
+            if (lane >= filled && lane < filled + take) j = rs.x + off + (lane - filled);
+            filled += take;
+            off += take;
+            if (off == len) { ++l; off = 0; }
+        }
+        if (filled == 0) break;
+        __builtin_amdgcn_wave_barrier();
+        if (lane < filled) {
+            const double* sr = P + j * NF;
+            float4 p, lo;
+            split_rel(sr[RX], ox, p.x, lo.x);
+            split_rel(sr[RX + 1], oy, p.y, lo.y);
+            split_rel(sr[RX + 2], oz, p.z, lo.z);
+            double a, b;
+            zeta_params<K>(sr[RSIG], a, b);
+            p.w = zeta_cut<K>(sr[RSIG]);
+            lo.w = (float)a;
+            lp[lane] = p;
+            ll[lane] = lo;
+            lg[lane] = make_float4((float)sr[RG], (float)sr[RG + 1], (float)sr[RG + 2], (float)b);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int k = 0; k * R < filled; ++k) {
+            const int sidx = k * R + g;
+            const bool ok = sidx < filled;
+            const int sl = ok ? sidx : 0;
+            const float4 a = lp[sl];
+            float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
+            float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            if (__any(ok && r2 < a.w)) {
+                if (ok) {
+                    const float4 lo = ll[sl];
+                    const float4 gg = lg[sl];
+                    dx += tlx - lo.x;
+                    dy += tly - lo.y;
+                    dz += tlz - lo.z;
+                    r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    const float z = zeta_ab<K>(r2, lo.w, gg.w);
+                    acc[0] = fmaf(gg.x, z, acc[0]);
+                    acc[1] = fmaf(gg.y, z, acc[1]);
+                    acc[2] = fmaf(gg.z, z, acc[2]);
+                }
+            }
+            if (++since == kZetaFold) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) { A[c] += (double)acc[c]; acc[c] = 0.f; }
+                since = 0;
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) A[c] += (double)acc[c];
+    if constexpr (R > 1) {
+#pragma unroll
+        for (int m = TW; m < 64; m <<= 1) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) A[c] += __shfl_xor(A[c], m);
+        }
+    }
+    if (g == 0 && active) {
+        double* rec = P + t * NF;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) rec[RJ + c] += A[c];
+    }
+}
+
 #pragma clang fp contract(off)
 // FP64: viscous.jl:545-554 per pair, list order, accumulating onto J[1:3]
 // as the reference does.
@@ -1156,7 +1275,7 @@ int launch_zeta_ranges(vpm_field f, int kernel, int precision, const int64_t* tb
     const bool f64 = precision == VPM_PREC_F64 || kernel == VPM_KERNEL_SINGULAR;
     RangePlan plan;
     int rc;
-    if ((rc = plan_ranges(f, tb, te, sb, se, npairs, !f64, plan))) return rc;
+    if ((rc = plan_ranges(f, tb, te, sb, se, npairs, !f64, plan, 8))) return rc;
     if (plan.nitems == 0) return VPM_OK;
     const bool wave = plan.wave;
     RangeItem* d_items;
@@ -1172,10 +1291,22 @@ int launch_zeta_ranges(vpm_field f, int kernel, int precision, const int64_t* tb
         default: return fail(VPM_EINVAL, "unknown kernel id");
         }
     } else if (wave) {
+        const int64_t ni = plan.nitems;
+        const dim3 wgrid((unsigned)((ni + kBlock / 64 - 1) / (kBlock / 64)));
         switch (kernel) {
-#define VPM_ZW(KID) case KID: hipLaunchKernelGGL((zeta_ranges_f32c_kernel<KID, 64>), grid, dim3(64), 0, f->stream, d_items, d_src, f->P); break;
+#define VPM_ZWR(KID, R) hipLaunchKernelGGL((zeta_ranges_wave_f32c_kernel<KID, R>), wgrid, dim3(kBlock), 0, f->stream, d_items, ni, d_src, f->P)
+#define VPM_ZW(KID)                                                                                                   \
+    case KID:                                                                                                         \
+        switch (plan.R) {                                                                                             \
+        case 8: VPM_ZWR(KID, 8); break;                                                                               \
+        case 4: VPM_ZWR(KID, 4); break;                                                                               \
+        case 2: VPM_ZWR(KID, 2); break;                                                                               \
+        default: hipLaunchKernelGGL((zeta_ranges_f32c_kernel<KID, 64>), grid, dim3(64), 0, f->stream, d_items, d_src, f->P); break; \
+        }                                                                                                             \
+        break;
             VPM_ZW(VPM_KERNEL_GAUSSIAN) VPM_ZW(VPM_KERNEL_GAUSSIANERF) VPM_ZW(VPM_KERNEL_WINCKELMANS)
 #undef VPM_ZW
+#undef VPM_ZWR
         default: return fail(VPM_EINVAL, "unknown kernel id");
         }
     } else {

@@ -80,10 +80,12 @@ def _leaf_lists(n, nleaves, seed, density=0.4):
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
-@pytest.mark.parametrize("nleaves,extra", [(21, False), (21, True), (4, False)])
+@pytest.mark.parametrize("nleaves,extra", [(21, False), (21, True), (4, False), (180, False), (75, False),
+                                           (37, False), (75, True)])
 def test_zeta_ranges_vs_oracle(vpm, orc, kernel, nleaves, extra):
     """zeta_fmm's pair loop over a direct list (vpm_zeta_ranges): leaf lists
-    of ~43 targets (one wave per leaf) or ~225 (256-lane blocks), with
+    of ~5, ~12, ~24 (a wave shared by 8, 4, 2 leaves) or ~43 targets (one
+    wave per leaf), or ~225 (256-lane blocks), with
     overlapping and repeated target ranges when `extra`; J[1:3] accumulates
     onto a nonzero start and static particles are not skipped.  F64 within
     FP64 rounding of the oracle; F32C within ZETA_SCALED_TOL of the
