@@ -10,6 +10,7 @@
 // targets' records: no two blocks write the same particle, and each target's
 // sum runs in pair order.  The pair arithmetic is p2p_pair.h's.
 #include <algorithm>
+#include <new>
 #include <string>
 #include <thread>
 #include <vector>
@@ -339,6 +340,8 @@ static int plan_parallel(int64_t n, Fn&& fn)
     }
     std::vector<std::thread> th;
     std::vector<int> inline_chunks;
+    th.reserve(T);
+    inline_chunks.reserve(T);
     for (int t = 1; t < T; ++t) {
         try {
             th.emplace_back(fn, n * t / T, n * (t + 1) / T, t);
@@ -365,15 +368,15 @@ static int64_t first_bad(const int64_t* bad, int T)
 // of the list is at most 128 targets (FMM leaves).  Every pair is checked
 // (0 <= begin <= end <= np) in the passes that read it, before any device
 // work.
-static int plan_ranges(vpm_field f, const int64_t* tb, const int64_t* te, const int64_t* sb, const int64_t* se,
-                       int64_t npairs, bool allow_wave, RangePlan& plan, int rmax = 1)
+static int plan_ranges_impl(vpm_field f, const int64_t* tb, const int64_t* te, const int64_t* sb,
+                            const int64_t* se, int64_t npairs, bool allow_wave, RangePlan& plan, int rmax)
 {
     plan = RangePlan();
     const int64_t np = f->np;
     // runs of consecutive pairs with one target range, found per chunk of the
     // list (a run belongs to the chunk it starts in) and joined in list order
     struct Run { int64_t tb, te, l0, l1; };
-    struct Part { std::vector<Run> runs; int64_t tot = 0, cnt = 0; };
+    struct Part { std::vector<Run> runs; int64_t tot = 0, cnt = 0; bool oom = false; };
     Part parts[kPlanThreads];
     int64_t bad[kPlanThreads];
     for (int t = 0; t < kPlanThreads; ++t) bad[t] = -1;
@@ -386,11 +389,18 @@ static int plan_ranges(vpm_field f, const int64_t* tb, const int64_t* te, const 
             if (tb[k] < 0 || te[k] > np || tb[k] > te[k]) { bad[t] = k; return; }
             int64_t m = k + 1;
             while (m < npairs && same(m, k)) ++m;
-            pt.runs.push_back({tb[k], te[k], k, m});
+            try {
+                pt.runs.push_back({tb[k], te[k], k, m});
+            } catch (...) {   // no exception leaves a worker thread
+                pt.oom = true;
+                return;
+            }
             if (te[k] > tb[k]) { pt.tot += te[k] - tb[k]; ++pt.cnt; }
             k = m;
         }
     });
+    for (int t = 0; t < T; ++t)
+        if (parts[t].oom) throw std::bad_alloc();
     if (int64_t k = first_bad(bad, T); k >= 0) return range_error(k);
     std::vector<Run> runs;
     int64_t tot = 0, cnt = 0;
@@ -493,6 +503,19 @@ static int plan_ranges(vpm_field f, const int64_t* tb, const int64_t* te, const 
             plan.items[q++] = {t, std::min<int64_t>(t + chunk, cuts[m + 1]), count[m], count[m + 1]};
     }
     return VPM_OK;
+}
+
+// Host vectors of the plan that cannot be allocated end the call with
+// VPM_ENOMEM; no exception crosses the C ABI.
+static int plan_ranges(vpm_field f, const int64_t* tb, const int64_t* te, const int64_t* sb, const int64_t* se,
+                       int64_t npairs, bool allow_wave, RangePlan& plan, int rmax = 1)
+{
+    try {
+        return plan_ranges_impl(f, tb, te, sb, se, npairs, allow_wave, plan, rmax);
+    } catch (const std::exception&) {
+        plan = RangePlan();
+        return fail(VPM_ENOMEM, "host allocation failed while planning a range list");
+    }
 }
 
 // One copy of the staged plan to the device.  The pinned source stays in use

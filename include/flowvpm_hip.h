@@ -30,7 +30,7 @@ typedef enum {
     VPM_OK = 0,
     VPM_EINVAL = -1,    /* bad argument (null pointer, bad id, bad range)      */
     VPM_ERANGE = -2,    /* particle count above the field's capacity           */
-    VPM_ENOMEM = -3,    /* device allocation failed                            */
+    VPM_ENOMEM = -3,    /* device or host allocation failed                    */
     VPM_EDEVICE = -4,   /* no such device or a HIP runtime error               */
     VPM_ENAN = -5,      /* NaN in the dynamic SFS coefficient (subfilterscale.jl:645-652) */
     VPM_ERESET = -6,    /* core-spreading reset required (viscous.jl:202) and rbf_itmax == 0 */
