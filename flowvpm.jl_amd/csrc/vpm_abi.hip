@@ -356,7 +356,12 @@ int vpm_get_particles(vpm_field f, double* particles, int64_t np, int32_t row0, 
     const size_t n = (size_t)np * (size_t)nr;
     if (int rc = ensure(f->xfer, sizeof(double) * n)) return rc;
     if (int rc = launch_gather_rows(f, row0, nr, (double*)f->xfer.p)) return rc;
-    std::vector<double> host(n);
+    std::vector<double> host;
+    try {
+        host.resize(n);
+    } catch (const std::exception&) {
+        return fail(VPM_ENOMEM, "host buffer of vpm_get_particles");
+    }
     e = hipMemcpyAsync(host.data(), f->xfer.p, sizeof(double) * n, hipMemcpyDeviceToHost, f->stream);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(D2H)");
     if (int rc = sync(f)) return rc;
@@ -667,9 +672,12 @@ int vpm_tape_create(vpm_field f, vpm_tape* out)
     vpm_tape t = new (std::nothrow) vpm_tape_s;
     if (!t) return fail(VPM_ENOMEM, "vpm_tape_create");
     t->f = f;
-    {
+    try {
         std::lock_guard<std::mutex> lock(f->mu);
         f->tapes.push_back(t);
+    } catch (const std::exception&) {
+        delete t;
+        return fail(VPM_ENOMEM, "vpm_tape_create");
     }
     *out = t;
     return VPM_OK;
