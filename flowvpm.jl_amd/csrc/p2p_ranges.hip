@@ -304,6 +304,7 @@ struct RangePlan {
     RangeItem* items = nullptr;   // in f->hstage
     int2* src = nullptr;          // source ranges (begin, end); positions fit 32 bits
     int64_t nitems = 0, nsrc = 0;
+    bool src_on_device = false;   // src already copied to the device plan buffer
     bool wave = false;            // F32C over short intervals: one wave per <= 64 / R targets
     int R = 1;                    // lane groups per target of the wave kernel
 };
@@ -325,13 +326,14 @@ static int range_error(int64_t k)
 }
 
 // Runs fn(k0, k1, t) over [0, n) cut into T <= kPlanThreads contiguous chunks
-// of at least 2^20 pairs, chunk t on thread t (the caller's thread takes
+// of at least 2^18 pairs, chunk t on thread t (the caller's thread takes
 // chunk 0; a chunk whose thread cannot start runs on the caller's).  Returns T.
 constexpr int kPlanThreads = 8;
+constexpr int64_t kPlanPiece = int64_t(1) << 18;   // pairs per thread at least, and per early copy
 template <class Fn>
 static int plan_parallel(int64_t n, Fn&& fn)
 {
-    int T = (int)std::min<int64_t>((n >> 20) + 1, kPlanThreads);
+    int T = (int)std::min<int64_t>(n / kPlanPiece + 1, kPlanThreads);
     const unsigned hc = std::thread::hardware_concurrency();
     if (hc > 0) T = std::min(T, (int)hc);
     if (T <= 1) {
@@ -450,15 +452,34 @@ static int plan_ranges_impl(vpm_field f, const int64_t* tb, const int64_t* te, c
         int64_t nitems = 0;
         for (const Run& r : runs) nitems += pieces(r.tb, r.te);
         if (int rc = stage_plan(f, nitems, npairs, plan)) return rc;
-        for (int t = 0; t < kPlanThreads; ++t) bad[t] = -1;
+        // The device plan buffer has the host layout (items, then source
+        // ranges), so each piece of packed source ranges is copied as soon as
+        // it is written, behind the packing of the next.  Copies of a list
+        // that then fails its checks land in the plan buffer only; the entry
+        // points synchronize before they return.
+        const size_t src_off = sizeof(RangeItem) * (size_t)nitems;
+        if (int rc = ensure(f->rng, src_off + sizeof(int2) * (size_t)npairs + 64)) return rc;
+        char* dsrc = (char*)f->rng.p + src_off;
+        hipError_t copy_err[kPlanThreads];
+        for (int t = 0; t < kPlanThreads; ++t) { bad[t] = -1; copy_err[t] = hipSuccess; }
         int2* src = plan.src;
+        hipStream_t stream = f->stream;
         const int T2 = plan_parallel(npairs, [&](int64_t k0, int64_t k1, int t) {
-            for (int64_t k = k0; k < k1; ++k) {
-                if (sb[k] < 0 || se[k] > np || sb[k] > se[k]) { bad[t] = k; return; }
-                src[k] = make_int2((int)sb[k], (int)se[k]);
+            for (int64_t a = k0; a < k1; a += kPlanPiece) {
+                const int64_t b = std::min(a + kPlanPiece, k1);
+                for (int64_t k = a; k < b; ++k) {
+                    if (sb[k] < 0 || se[k] > np || sb[k] > se[k]) { bad[t] = k; return; }
+                    src[k] = make_int2((int)sb[k], (int)se[k]);
+                }
+                const hipError_t e = hipMemcpyAsync(dsrc + sizeof(int2) * (size_t)a, src + a,
+                                                    sizeof(int2) * (size_t)(b - a), hipMemcpyHostToDevice, stream);
+                if (e != hipSuccess) { copy_err[t] = e; return; }
             }
         });
         if (int64_t k = first_bad(bad, T2); k >= 0) return range_error(k);
+        for (int t = 0; t < T2; ++t)
+            if (copy_err[t] != hipSuccess) return hip_fail(copy_err[t], "hipMemcpyAsync(range lists)");
+        plan.src_on_device = true;
         int64_t m = 0;
         for (const Run& r : runs)
             for (int64_t t = r.tb; t < r.te; t += chunk)
@@ -525,11 +546,12 @@ static int upload_plan(vpm_field f, const RangePlan& plan, RangeItem*& d_items, 
 {
     const size_t bytes_items = sizeof(RangeItem) * (size_t)plan.nitems;
     const size_t bytes = bytes_items + sizeof(int2) * (size_t)plan.nsrc;
-    if (int rc = ensure(f->rng, bytes + 64)) return rc;
+    if (int rc = ensure(f->rng, bytes + 64)) return rc;   // the size the plan already ensured, if it did
     char* base = (char*)f->rng.p;
     d_items = (RangeItem*)base;
     d_src = (int2*)(base + bytes_items);
-    hipError_t e = hipMemcpyAsync(base, f->hstage.p, bytes, hipMemcpyHostToDevice, f->stream);
+    hipError_t e = hipMemcpyAsync(base, f->hstage.p, plan.src_on_device ? bytes_items : bytes,
+                                  hipMemcpyHostToDevice, f->stream);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(range lists)");
     return VPM_OK;
 }

@@ -344,7 +344,7 @@ def test_p2p_ranges_long_list_plan_chunks(vpm, orc):
     te = tb + leaf
     sb = np.tile(starts, rep * L)
     se = sb + leaf
-    assert len(tb) >= (2 << 20)             # three plan chunks
+    assert len(tb) >= (2 << 20)             # eight plan chunks
     ref = P.copy()
     orc.direct_ranges(ref, "gaussianerf", tb, te, sb, se)
     A = P.copy()
