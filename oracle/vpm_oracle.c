@@ -17,18 +17,25 @@
  *                      tr J = 0, rotation/scaling/translation invariance
  *   vpmo_direct_ranges equals vpmo_direct2 over the union of the ranges
  *   vpmo_estr_direct   mpmath brute force; zero when all J are equal
- *   vpmo_update_particle_states / vpmo_relax / vpmo_core_spreading_rk3 /
- *   vpmo_sfs_before / vpmo_sfs_after / vpmo_rk3_step
- *                      PARITY PARTLY UNPINNED: line-by-line restatements,
- *                      checked by the properties the mathematics fixes --
- *                      exact free-stream advection, third-order RK3
- *                      convergence, Pedrizzetti alignment and corrected-norm
- *                      preservation, sigma^2 growing by 2 nu dt, the dynamic
- *                      coefficient's bounds and clipping rules, its numerator
- *                      and denominator rebuilt from independent evaluations
- *                      of the two filter widths -- and by the single-ring
- *                      translation speed test of
+ *   vpmo_update_particle_states / vpmo_rk3_step
+ *                      invariants of the update's ODE, with planted mistakes
+ *                      shown to fail them: |Gamma| sigma^((1-3g)/(f+g))
+ *                      per particle (third-order drift over whole steps,
+ *                      second-order over one Euler-form call with random J),
+ *                      |Gamma|^f sigma with SFS at g = 0, sum Gamma conserved
+ *                      by transposed stretching under cVPM with uniform
+ *                      sigma (not by the classic scheme); exact free-stream
+ *                      advection; third-order RK3 convergence; the
+ *                      single-ring translation speed test of
  *                      test/runtests_singlevortexring.jl:22-23,143
+ *   vpmo_relax         Pedrizzetti alignment at rlxf = 1, identity at 0,
+ *                      corrected-norm preservation
+ *   vpmo_core_spreading_rk3  sigma^2 grows by exactly 2 nu dt
+ *   vpmo_sfs_before / vpmo_sfs_after
+ *                      numerator and denominator rebuilt from independent
+ *                      evaluations of the two filter widths; the Lagrangian
+ *                      memory C2_k = nume (1 - (1 - rlxf)^k) under a frozen
+ *                      M; the coefficient's bounds and clipping rules
  *   vpmo_uj_adjoint    central finite differences of vpmo_loss_local;
  *                      equals vpmo_uj_adjoint_rrules (the reference's
  *                      UJS_pullback restated, a cross-check) to FP64 rounding

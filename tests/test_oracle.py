@@ -789,3 +789,249 @@ def test_dynamic_sfs_coefficient_from_independent_filters(vpm, orc, transposed, 
     Sw = vpm.metrics.stretching(W[:, 15:24], G, transposed)
     wrong = (3 * alpha - 2) * np.sum((Sw - Sd) * G, axis=1)
     assert np.max(np.abs(wrong[live] - Q[live, 37])) > 1e-3 * np.abs(Q[live, 37]).max()
+
+
+# ---- the stretching term and the rVPM update: invariants of the ODE ------
+# (timeintegration.jl:479-526).  Per particle the update integrates
+#   dX/dt = U + Uinf,
+#   dGamma/dt = S - 3 Z Gamma - E,   dsigma/dt = -sigma Z,
+#   Z = [(f+g)/(1+3f) S.Gamma - f/(1+3f) E.Gamma] / |Gamma|^2,
+#   E = C SFS sigma^3 / zeta0,
+# with S = J^T Gamma (transposed) or J Gamma (classic).  From these:
+#  * d ln|Gamma| / dt = [(1-3g) S.Gamma - E.Gamma] / ((1+3f) |Gamma|^2) and
+#    d ln sigma / dt = -Z, so without SFS |Gamma| sigma^((1-3g)/(f+g)) is
+#    constant for any S, and with g = 0 |Gamma|^f sigma is constant for any
+#    S and E;
+#  * with uniform sigma the transposed stretching of a pair (p, q) is
+#    H(d).(Gamma_q x Gamma_p) d + F (Gamma_q x Gamma_p) on p and the negative
+#    of it on q (H, F even in d), so sum_p S_p = 0 and, with f = g = 0
+#    (Z = 0) and no SFS, sum_p Gamma_p is conserved by any Runge-Kutta
+#    scheme (a linear invariant); the classic J Gamma has no such symmetry.
+# The mutants below restate the update with one plausible mistake each and
+# show that the checks catch it.
+
+def _update_np(P, a, b, dt, Uinf, f, g, zeta0, transposed, mutant=None):
+    """The rVPM update restated in numpy (timeintegration.jl:479-526), with
+    an optional planted mistake: "2Z" (2 Z Gamma for 3 Z Gamma), "+sZ" (the
+    sigma register gains +dt sigma Z), "coefZ" ((f+g)/(1+3g) in Z), "sfsZ"
+    (the SFS part of Z with the sign flipped), "classic" (J Gamma under the
+    transposed flag)."""
+    live = P[:, 42] == 0
+    Q = P[live]
+    M, G, J, U = Q[:, 27:36], Q[:, 3:6], Q[:, 15:24], Q[:, 9:12]
+    C, sfs, sg = Q[:, 36], Q[:, 39:42], Q[:, 6].copy()
+    M[:, 0:3] = a * M[:, 0:3] + dt * (U + np.asarray(Uinf))
+    Q[:, 0:3] += b * M[:, 0:3]
+    if transposed and mutant != "classic":
+        S = np.stack([np.sum(J[:, 3 * k:3 * k + 3] * G, axis=1) for k in range(3)], 1)
+    else:
+        S = np.stack([np.sum(J[:, k::3] * G, axis=1) for k in range(3)], 1)
+    Gn2 = np.sum(G * G, axis=1)
+    cz = (f + g) / (1 + 3 * g) if mutant == "coefZ" else (f + g) / (1 + 3 * f)
+    cs = -f / (1 + 3 * f) if mutant == "sfsZ" else f / (1 + 3 * f)
+    E = C[:, None] * sfs * (sg ** 3 / zeta0)[:, None]
+    Z = np.where(Gn2 > 0, (cz * np.sum(S * G, axis=1) - cs * np.sum(E * G, axis=1)) / np.where(Gn2 > 0, Gn2, 1), 0)
+    k3 = 2.0 if mutant == "2Z" else 3.0
+    M[:, 3:6] = a * M[:, 3:6] + dt * (S - k3 * Z[:, None] * G - E)
+    M[:, 7] = a * M[:, 7] + (dt * sg * Z if mutant == "+sZ" else -dt * sg * Z)
+    Q[:, 27:36] = M
+    Q[:, 3:6] = G + b * M[:, 3:6]
+    Q[:, 6] = sg + b * M[:, 7]
+    P[live] = Q
+
+
+_RK3 = ((0.0, 1.0 / 3.0), (-5.0 / 9.0, 15.0 / 16.0), (-153.0 / 128.0, 8.0 / 15.0))
+
+
+def _rk3_np(orc, P, kernel, dt, f, g, transposed, mutant):
+    """rungekutta3 without SFS, viscosity or relaxation
+    (timeintegration.jl:410-434) over the oracle's U/J and _update_np."""
+    P[:, 27:36] = 0.0
+    for a, b in _RK3:
+        orc.uj_direct(P, kernel, reset=True, transposed=transposed)
+        _update_np(P, a, b, dt, (0.0, 0.0, 0.0), f, g, 1.0, transposed, mutant)
+
+
+def test_update_np_restatement_equals_oracle(orc, vpm):
+    """The numpy restatement used for the mutants is the oracle's update
+    (both schemes, SFS term on, a static particle), so a mutant differs from
+    the oracle in its planted mistake only."""
+    rng = np.random.default_rng(60)
+    n = 40
+    P = vpm.workloads.random_cloud(n, seed=61)
+    P[:, 9:12], P[:, 15:24] = rng.normal(size=(n, 3)), rng.normal(size=(n, 9))
+    P[:, 27:36], P[:, 39:42] = rng.normal(size=(n, 9)), rng.normal(size=(n, 3))
+    P[:, 36] = rng.uniform(0, 1, n)
+    P[3, 42] = 1.0
+    P[4, 3:6] = 0.0
+    for transposed in (True, False):
+        A, B = P.copy(), P.copy()
+        orc.update_particle_states(A, -5 / 9, 15 / 16, 0.01, [0.1, 0.2, 0.3], 0.1, 0.2, 0.06, transposed)
+        _update_np(B, -5 / 9, 15 / 16, 0.01, [0.1, 0.2, 0.3], 0.1, 0.2, 0.06, transposed)
+        np.testing.assert_allclose(B, A, rtol=1e-13, atol=1e-16)
+    # and the RK3 driver over it is the oracle's rk3_step
+    A = vpm.fields.make_particles(P[:, 0:3], P[:, 3:6] * 1e-2, 0.15)
+    B = A.copy()
+    orc.rk3_step(A, orc.make_settings(kernel="gaussianerf", f=0.0, g=0.2, relax=0), 0.05, relax_now=False)
+    _rk3_np(orc, B, "gaussianerf", 0.05, 0.0, 0.2, True, None)
+    np.testing.assert_allclose(B[:, 0:7], A[:, 0:7], rtol=1e-13, atol=1e-16)
+
+
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans", "gaussian", "singular"])
+def test_transposed_stretching_conserves_total_circulation(orc, vpm, kernel):
+    """cVPM (f = g = 0), uniform sigma, inviscid, no SFS or relaxation: 10 RK3
+    steps of 60 interacting particles keep sum Gamma to FP64 rounding of the
+    step sums (below 1e-13 of sum_p |Gamma_p(T) - Gamma_p(0)|; measured 4e-17
+    to 4e-14 against per-particle changes of 9e-3 to 50), while the classic
+    scheme (J Gamma) moves it by 1e-2 or more."""
+    rng = np.random.default_rng(21)
+    n = 60
+    P = vpm.fields.make_particles(rng.uniform(0, 1, (n, 3)), rng.normal(0, 2e-2, (n, 3)), 0.15)
+    drift, change, scale = {}, {}, {}
+    for transposed in (True, False):
+        s = orc.make_settings(kernel=kernel, transposed=transposed, f=0.0, g=0.0, relax=0)
+        R = P.copy()
+        for _ in range(10):
+            assert orc.rk3_step(R, s, 0.05, relax_now=False) == 0
+        drift[transposed] = np.abs(R[:, 3:6].sum(0) - P[:, 3:6].sum(0)).max()
+        change[transposed] = np.abs(R[:, 3:6] - P[:, 3:6]).max()
+        scale[transposed] = np.abs(R[:, 3:6] - P[:, 3:6]).sum()
+        assert np.all(R[:, 6] == P[:, 6])                 # Z = 0: sigma untouched
+    assert change[True] > 5e-3 and change[False] > 5e-3, change
+    assert drift[True] < 1e-13 * scale[True], (drift, scale)
+    assert drift[False] > 1e-2, drift
+    assert drift[False] > 1e10 * drift[True], drift
+
+
+def _rvpm_invariant(R, f, g):
+    return np.linalg.norm(R[:, 3:6], axis=1) * R[:, 6] ** ((1 - 3 * g) / (f + g))
+
+
+@pytest.mark.parametrize("fg", [(0.0, 0.2), (0.25, 0.25), (0.5, 0.0)])
+def test_rvpm_invariant_third_order(orc, vpm, fg):
+    """rVPM without SFS keeps |Gamma| sigma^((1-3g)/(f+g)) per particle for any
+    stretching, so over a whole rk3_step run its drift is the scheme's
+    truncation error: it falls 8x per halving of dt (measured 7.9-8.1) while
+    |Gamma| itself changes by 4-12 %.  Planted mistakes in the update (2 Z Gamma,
+    +sigma Z, (f+g)/(1+3g)) break the invariant: their drift is 7e-2 to 0.74
+    and does not fall with dt.  ((f+g)/(1+3g) equals the right coefficient
+    when f = g, so that mutant is checked at the other two (f, g).)"""
+    f, g = fg
+    rng = np.random.default_rng(21)
+    n = 60
+    P = vpm.fields.make_particles(rng.uniform(0, 1, (n, 3)), rng.normal(0, 2e-2, (n, 3)),
+                                  rng.uniform(0.12, 0.2, n))
+    I0 = _rvpm_invariant(P, f, g)
+    s = orc.make_settings(kernel="gaussianerf", transposed=True, f=f, g=g, relax=0)
+    errs = []
+    for nst in (4, 8, 16, 32):
+        R = P.copy()
+        for _ in range(nst):
+            assert orc.rk3_step(R, s, 1.0 / nst, relax_now=False) == 0
+        errs.append(np.abs(_rvpm_invariant(R, f, g) / I0 - 1).max())
+    gchange = np.abs(np.linalg.norm(R[:, 3:6], axis=1) / np.linalg.norm(P[:, 3:6], axis=1) - 1).max()
+    ratios = [errs[i] / errs[i + 1] for i in range(3)]
+    assert gchange > 0.03, gchange
+    assert all(7.0 < r < 9.0 for r in ratios), (errs, ratios)
+    assert errs[-1] < 1e-6, errs
+    mutants = ["2Z", "+sZ"] + (["coefZ"] if f != g else [])
+    for mutant in mutants:
+        me = []
+        for nst in (8, 16):
+            R = P.copy()
+            for _ in range(nst):
+                _rk3_np(orc, R, "gaussianerf", 1.0 / nst, f, g, True, mutant)
+            me.append(np.abs(_rvpm_invariant(R, f, g) / I0 - 1).max())
+        assert me[1] > 1e3 * errs[2] and me[0] / me[1] < 1.5, (mutant, me, errs)
+
+
+@pytest.mark.parametrize("case", ["nosfs_rvpm", "nosfs_tube", "sfs_g0"])
+def test_update_local_invariant_second_order(orc, vpm, case):
+    """One update call (a = 0, b = 1: an Euler step of the ODE) on random
+    records with random J, SFS and C changes the invariant by O(dt^2) only:
+    4x less per halving of dt.  Without SFS the invariant is
+    |Gamma| sigma^((1-3g)/(f+g)); with SFS and g = 0 it is |Gamma|^f sigma,
+    which also fixes the SFS part of Z against the SFS part of dGamma/dt.
+    Every planted mistake changes it at O(dt) (2x per halving)."""
+    rng = np.random.default_rng(62)
+    n = 200
+    P = vpm.workloads.random_cloud(n, seed=63)
+    P[:, 3:6] = rng.normal(size=(n, 3))
+    P[:, 15:24] = rng.normal(size=(n, 9))
+    P[:, 6] = rng.uniform(0.5, 1.5, n)
+    if case == "sfs_g0":
+        f, g = 0.5, 0.0
+        P[:, 39:42] = rng.normal(size=(n, 3))
+        P[:, 36] = rng.uniform(0.1, 1.0, n)
+        inv = lambda R: np.linalg.norm(R[:, 3:6], axis=1) ** f * R[:, 6]
+    else:
+        f, g = (0.0, 0.2) if case == "nosfs_rvpm" else (0.5, 0.0)
+        inv = lambda R: _rvpm_invariant(R, f, g)
+    zeta0 = 0.7
+    I0 = inv(P)
+
+    def change(dt, mutant):
+        R = P.copy()
+        if mutant is None:
+            orc.update_particle_states(R, 0.0, 1.0, dt, [0.0, 0.0, 0.0], f, g, zeta0, True)
+        else:
+            _update_np(R, 0.0, 1.0, dt, (0.0, 0.0, 0.0), f, g, zeta0, True, mutant)
+        return np.abs(np.log(inv(R) / I0)).max()
+
+    e = [change(dt, None) for dt in (2e-3, 1e-3, 5e-4)]
+    assert all(3.6 < e[i] / e[i + 1] < 4.4 for i in range(2)), e
+    mutants = ["2Z", "+sZ", "coefZ"] + (["sfsZ"] if case == "sfs_g0" else [])
+    for mutant in mutants:
+        m = [change(dt, mutant) for dt in (1e-3, 5e-4)]
+        assert 1.8 < m[0] / m[1] < 2.2 and m[1] > 10 * e[2], (mutant, m, e)
+
+
+def test_transposed_circulation_mutant_caught(orc, vpm):
+    """The classic product J Gamma put where the transposed J^T Gamma belongs
+    (the one mistake the rVPM invariant cannot see: S.Gamma is the same for
+    both) moves sum Gamma at once."""
+    rng = np.random.default_rng(21)
+    n = 60
+    P = vpm.fields.make_particles(rng.uniform(0, 1, (n, 3)), rng.normal(0, 2e-2, (n, 3)), 0.15)
+    R, M = P.copy(), P.copy()
+    for _ in range(3):
+        _rk3_np(orc, R, "gaussianerf", 0.05, 0.0, 0.0, True, None)
+        _rk3_np(orc, M, "gaussianerf", 0.05, 0.0, 0.0, True, "classic")
+    good = np.abs(R[:, 3:6].sum(0) - P[:, 3:6].sum(0)).max()
+    bad = np.abs(M[:, 3:6].sum(0) - P[:, 3:6].sum(0)).max()
+    assert good < 1e-15 and bad > 1e-3, (good, bad)
+
+
+def test_sfs_lagrangian_memory(orc, vpm):
+    """dynamicprocedure_pseudo3level_afterUJ (subfilterscale.jl:571-617) with
+    the same M before each of three calls, no clamping (minC = 0, maxC huge)
+    and no clipping: the denominator memory starts at the first deno and
+    stays there, and the numerator follows the Lagrangian average
+    C2_k = nume (1 - (1 - rlxf)^k) with nume = (3 alpha - 2) M[1:3].Gamma and
+    deno = M[4:6].Gamma sigma^3 / zeta0; C = C2 / C3 (|C2 / C3| with
+    force_positive); M is flushed after every call."""
+    rng = np.random.default_rng(64)
+    n = 50
+    P = vpm.workloads.random_cloud(n, seed=65)
+    P[:, 3:6] = rng.normal(size=(n, 3))
+    M = rng.normal(size=(n, 9))
+    P[:, 36:39] = 0.0
+    P[4, 42] = 1.0
+    alpha, rlxf, kernel = 0.999, 0.3, "gaussianerf"
+    zeta0 = float(orc.zeta(kernel, 0.0))
+    G, sg = P[:, 3:6], P[:, 6]
+    nume = (3 * alpha - 2) * np.sum(M[:, 0:3] * G, axis=1)
+    deno = np.sum(M[:, 3:6] * G, axis=1) * sg ** 3 / zeta0
+    live = P[:, 42] == 0
+    for fp in (False, True):
+        R = P.copy()
+        for k in (1, 2, 3):
+            R[live, 27:36] = M[live]
+            assert orc.sfs_after(R, kernel, True, alpha, rlxf, 0.0, 1e300, force_positive=fp, clip=False) == 0
+            c2 = nume * (1 - (1 - rlxf) ** k)
+            np.testing.assert_allclose(R[live, 37], c2[live], rtol=1e-13)
+            np.testing.assert_allclose(R[live, 38], deno[live], rtol=1e-13)
+            c1 = c2 / deno
+            np.testing.assert_allclose(R[live, 36], (np.abs(c1) if fp else c1)[live], rtol=1e-12)
+            assert np.all(R[live, 27:36] == 0.0)
+        assert np.array_equal(R[~live], P[~live])
