@@ -57,6 +57,7 @@ _SIGS = {
     "vpm_get_np": (C.c_int, [C.c_void_p, _i64p]),
     "vpm_synchronize": (C.c_int, [C.c_void_p]),
     "vpm_uj_direct": (C.c_int, [C.c_void_p, C.POINTER(Config), C.c_uint32]),
+    "vpm_uj_direct_noalloc": (C.c_int, [C.c_void_p, C.POINTER(Config), C.c_uint32]),
     "vpm_uj_direct_fields": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Config)]),
     "vpm_p2p_ranges": (C.c_int, [C.c_void_p, C.POINTER(Config), _i64p, _i64p, _i64p, _i64p, C.c_int64]),
     "vpm_estr_direct": (C.c_int, [C.c_void_p, C.POINTER(Config)]),

@@ -305,75 +305,107 @@ static inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) /
         if (e_ != hipSuccess) return hip_fail(e_, what);                       \
     } while (0)
 
-// Morton (Z-order) permutation of the field's particles: perm[k] is the
-// particle at sorted position k.  Reused until positions change.
-int sort_field(vpm_field f, const uint32_t*& perm)
+// A grow-only field buffer, or the next piece of the arena when there is one.
+static int take_buffer(vpm_field, DevBuf& b, size_t bytes, Arena* ar, void*& p, const char* what)
 {
-    const int64_t np = f->np;
-    const int64_t ld = f->cap;
-    int rc;
-    perm = nullptr;
-    if (np <= 1) return VPM_OK;
-    if (f->perm_valid) {
-        perm = (const uint32_t*)f->perm.p;
+    if (!ar) {
+        if (int rc = ensure(b, bytes)) return rc;
+        p = b.p;
         return VPM_OK;
     }
-    const int nbb = 256;
-    if ((rc = ensure(f->bbox, sizeof(double) * (6 + 6 * nbb)))) return rc;
-    if ((rc = ensure(f->keys, sizeof(uint64_t) * (size_t)ld))) return rc;
-    if ((rc = ensure(f->keys2, sizeof(uint64_t) * (size_t)ld))) return rc;
-    if ((rc = ensure(f->perm, sizeof(uint32_t) * (size_t)ld))) return rc;
-    if ((rc = ensure(f->iota, sizeof(uint32_t) * (size_t)ld))) return rc;
-    double* bb = (double*)f->bbox.p;
-    hipLaunchKernelGGL(bbox_partial_kernel, dim3(nbb), dim3(kBlock), 0, f->stream, f->P, np, bb);
-    VPM_LAUNCH_CHECK("bbox_partial");
-    hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(64), 0, f->stream, bb, nbb);
-    VPM_LAUNCH_CHECK("bbox_final");
-    hipLaunchKernelGGL(morton_kernel, dim3(nblocks(np)), dim3(kBlock), 0, f->stream, f->P, np, bb,
-                       (uint64_t*)f->keys.p, (uint32_t*)f->iota.p);
-    VPM_LAUNCH_CHECK("morton");
-    size_t tmp = 0;
-    hipError_t e = rocprim::radix_sort_pairs(nullptr, tmp, (const uint64_t*)f->keys.p, (uint64_t*)f->keys2.p,
-                                             (const uint32_t*)f->iota.p, (uint32_t*)f->perm.p, (unsigned)np, 0,
-                                             63, f->stream);
-    if (e != hipSuccess) return hip_fail(e, "radix_sort_pairs (size query)");
-    if ((rc = ensure(f->sorttmp, tmp))) return rc;
-    e = rocprim::radix_sort_pairs(f->sorttmp.p, tmp, (const uint64_t*)f->keys.p, (uint64_t*)f->keys2.p,
-                                  (const uint32_t*)f->iota.p, (uint32_t*)f->perm.p, (unsigned)np, 0, 63,
-                                  f->stream);
-    if (e != hipSuccess) return hip_fail(e, "radix_sort_pairs");
-    f->perm_valid = true;
-    perm = (const uint32_t*)f->perm.p;
+    p = ar->take(bytes);
+    if (!p) return fail(VPM_ERANGE, std::string("no-alloc scratch exhausted at ") + what);
     return VPM_OK;
 }
 
-int stage_field(vpm_field f, int kernel, int precision, bool sort, Staged& out)
+static constexpr int kSortBoxes = 256;   // blocks of the bounding-box pass
+
+int sort_scratch_bytes(vpm_field f, size_t& bytes)
+{
+    const size_t np = (size_t)f->np;
+    size_t tmp = 0;
+    hipError_t e = rocprim::radix_sort_pairs(nullptr, tmp, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                             (const uint32_t*)nullptr, (uint32_t*)nullptr, (unsigned)np, 0, 63,
+                                             f->stream);
+    if (e != hipSuccess) return hip_fail(e, "radix_sort_pairs (size query)");
+    const auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    bytes = al(sizeof(double) * (6 + 6 * kSortBoxes)) + 2 * al(sizeof(uint64_t) * np) +
+            2 * al(sizeof(uint32_t) * np) + al(tmp);
+    return VPM_OK;
+}
+
+// Morton (Z-order) permutation of the field's particles: perm[k] is the
+// particle at sorted position k.  Reused until positions change (not when
+// it lives in an arena).
+int sort_field(vpm_field f, const uint32_t*& perm, Arena* ar)
 {
     const int64_t np = f->np;
-    const int64_t ld = f->cap;
+    const int64_t ld = ar ? np : f->cap;
     int rc;
-    if ((rc = ensure(f->pos, sizeof(double) * 3 * (size_t)ld))) return rc;
-    out.pos = (const double*)f->pos.p;
+    perm = nullptr;
+    if (np <= 1) return VPM_OK;
+    if (f->perm_valid && !ar) {
+        perm = (const uint32_t*)f->perm.p;
+        return VPM_OK;
+    }
+    void *bbp, *kp, *k2p, *pp, *ip;
+    if ((rc = take_buffer(f, f->bbox, sizeof(double) * (6 + 6 * kSortBoxes), ar, bbp, "bbox"))) return rc;
+    if ((rc = take_buffer(f, f->keys, sizeof(uint64_t) * (size_t)ld, ar, kp, "keys"))) return rc;
+    if ((rc = take_buffer(f, f->keys2, sizeof(uint64_t) * (size_t)ld, ar, k2p, "keys"))) return rc;
+    if ((rc = take_buffer(f, f->perm, sizeof(uint32_t) * (size_t)ld, ar, pp, "perm"))) return rc;
+    if ((rc = take_buffer(f, f->iota, sizeof(uint32_t) * (size_t)ld, ar, ip, "perm"))) return rc;
+    double* bb = (double*)bbp;
+    hipLaunchKernelGGL(bbox_partial_kernel, dim3(kSortBoxes), dim3(kBlock), 0, f->stream, f->P, np, bb);
+    VPM_LAUNCH_CHECK("bbox_partial");
+    hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(64), 0, f->stream, bb, kSortBoxes);
+    VPM_LAUNCH_CHECK("bbox_final");
+    hipLaunchKernelGGL(morton_kernel, dim3(nblocks(np)), dim3(kBlock), 0, f->stream, f->P, np, bb,
+                       (uint64_t*)kp, (uint32_t*)ip);
+    VPM_LAUNCH_CHECK("morton");
+    size_t tmp = 0;
+    hipError_t e = rocprim::radix_sort_pairs(nullptr, tmp, (const uint64_t*)kp, (uint64_t*)k2p,
+                                             (const uint32_t*)ip, (uint32_t*)pp, (unsigned)np, 0,
+                                             63, f->stream);
+    if (e != hipSuccess) return hip_fail(e, "radix_sort_pairs (size query)");
+    void* tp;
+    if ((rc = take_buffer(f, f->sorttmp, tmp, ar, tp, "sort"))) return rc;
+    e = rocprim::radix_sort_pairs(tp, tmp, (const uint64_t*)kp, (uint64_t*)k2p,
+                                  (const uint32_t*)ip, (uint32_t*)pp, (unsigned)np, 0, 63,
+                                  f->stream);
+    if (e != hipSuccess) return hip_fail(e, "radix_sort_pairs");
+    if (!ar) f->perm_valid = true;
+    perm = (const uint32_t*)pp;
+    return VPM_OK;
+}
+
+int stage_field(vpm_field f, int kernel, int precision, bool sort, Staged& out, Arena* ar)
+{
+    const int64_t np = f->np;
+    const int64_t ld = ar ? np : f->cap;
+    int rc;
+    void *posp, *gp, *cp;
+    if ((rc = take_buffer(f, f->pos, sizeof(double) * 3 * (size_t)ld, ar, posp, "positions"))) return rc;
+    out.pos = (const double*)posp;
     out.ld = ld;
     out.gs = nullptr; out.cut = nullptr; out.g64 = nullptr; out.perm = nullptr;
     if (np == 0) return VPM_OK;
     if (precision == VPM_PREC_F64) {
-        if ((rc = ensure(f->g64, sizeof(double4) * (size_t)ld))) return rc;
+        if ((rc = take_buffer(f, f->g64, sizeof(double4) * (size_t)ld, ar, gp, "sources"))) return rc;
         hipLaunchKernelGGL(pack_f64_kernel, dim3(nblocks(np)), dim3(kBlock), 0, f->stream,
-                           f->P, np, (double*)f->pos.p, ld, (double4*)f->g64.p);
+                           f->P, np, (double*)posp, ld, (double4*)gp);
         VPM_LAUNCH_CHECK("pack_f64");
-        out.g64 = (const double4*)f->g64.p;
+        out.g64 = (const double4*)gp;
         return VPM_OK;
     }
-    if ((rc = ensure(f->gs, sizeof(float4) * (size_t)ld))) return rc;
-    if ((rc = ensure(f->cut, sizeof(float) * (size_t)ld))) return rc;
+    if ((rc = take_buffer(f, f->gs, sizeof(float4) * (size_t)ld, ar, gp, "sources"))) return rc;
+    if ((rc = take_buffer(f, f->cut, sizeof(float) * (size_t)ld, ar, cp, "sources"))) return rc;
     const uint32_t* perm = nullptr;
-    if (sort && np > 1 && (rc = sort_field(f, perm))) return rc;
+    if (sort && np > 1 && (rc = sort_field(f, perm, ar))) return rc;
     switch (kernel) {
 #define VPM_PACK(KID)                                                                                  \
     case KID:                                                                                          \
         hipLaunchKernelGGL(pack_f32c_kernel<KID>, dim3(nblocks(np)), dim3(kBlock), 0, f->stream, f->P,  \
-                           np, perm, (double*)f->pos.p, ld, (float4*)f->gs.p, (float*)f->cut.p);       \
+                           np, perm, (double*)posp, ld, (float4*)gp, (float*)cp);                      \
         break;
         VPM_PACK(VPM_KERNEL_SINGULAR)
         VPM_PACK(VPM_KERNEL_GAUSSIAN)
@@ -384,8 +416,8 @@ int stage_field(vpm_field f, int kernel, int precision, bool sort, Staged& out)
         return fail(VPM_EINVAL, "unknown kernel id");
     }
     VPM_LAUNCH_CHECK("pack_f32c");
-    out.gs = (const float4*)f->gs.p;
-    out.cut = (const float*)f->cut.p;
+    out.gs = (const float4*)gp;
+    out.cut = (const float*)cp;
     out.perm = perm;
     return VPM_OK;
 }
@@ -396,11 +428,12 @@ int stage_field(vpm_field f, int kernel, int precision, bool sort, Staged& out)
 // 11-25 % faster from N = 32,768 to 262,144, and at N = 2^20 (K = 8 instead
 // of 1) 3-5 % faster; beyond that the gain is flat.  The FP64 partial sums
 // then take at most ~1 GB (K x 14 x 8 B per target).
-static void choose_split(int64_t nt, int64_t ns, int64_t& K, int64_t& chunk)
+void choose_split(int64_t nt, int64_t ns, int64_t kmax, int64_t& K, int64_t& chunk)
 {
     const int64_t nblk = (nt + kBlock - 1) / kBlock;
     const int64_t want = 32768;
     K = std::max<int64_t>(1, (want + nblk - 1) / nblk);
+    if (kmax > 0) K = std::min(K, kmax);
     const int64_t tiles = std::max<int64_t>(1, (ns + kTile - 1) / kTile);
     K = std::min(K, tiles);
     const int64_t tiles_per = (tiles + K - 1) / K;
@@ -410,16 +443,18 @@ static void choose_split(int64_t nt, int64_t ns, int64_t& K, int64_t& chunk)
 }
 
 int launch_uj(vpm_field ft, const Staged& T, int64_t t0, int64_t nt, const Staged& S, int64_t s0,
-              int64_t s1, int kernel, int precision)
+              int64_t s1, int kernel, int precision, Arena* ar, int64_t kmax)
 {
     if (nt <= 0 || s1 <= s0) return VPM_OK;
     int64_t K, chunk;
-    choose_split(nt, s1 - s0, K, chunk);
+    choose_split(nt, s1 - s0, kmax, K, chunk);
     const int acc = precision == VPM_PREC_F64 ? kAccF64 : kAccF32;
     const int64_t pld = nt;
     int rc;
-    if ((rc = ensure(ft->part, sizeof(double) * (size_t)acc * (size_t)K * (size_t)pld))) return rc;
-    double* part = (double*)ft->part.p;
+    void* pp;
+    if ((rc = take_buffer(ft, ft->part, sizeof(double) * (size_t)acc * (size_t)K * (size_t)pld, ar, pp, "partial sums")))
+        return rc;
+    double* part = (double*)pp;
     const dim3 grid(nblocks(nt), (unsigned)K);
     timing_begin(ft);
     if (precision == VPM_PREC_F64) {

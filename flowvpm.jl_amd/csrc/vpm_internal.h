@@ -41,6 +41,22 @@ struct Timing {
     int64_t count = 0;
 };
 
+// Device memory handed out in order from one caller-provided block (the
+// no-alloc entry point's static array), 256-byte aligned.  take() returns
+// nullptr once the block is exhausted.
+struct Arena {
+    char* base = nullptr;
+    size_t size = 0;
+    size_t used = 0;
+    void* take(size_t bytes)
+    {
+        const size_t at = (used + 255) & ~(size_t)255;
+        if (at > size || bytes > size - at) return nullptr;
+        used = at + bytes;
+        return base + at;
+    }
+};
+
 }  // namespace vpm
 
 struct vpm_field_s {
@@ -103,10 +119,20 @@ struct Staged {
     const uint32_t* perm;  // sorted index -> particle index, or nullptr
 };
 
-int sort_field(vpm_field f, const uint32_t*& perm);
-int stage_field(vpm_field f, int kernel, int precision, bool sort, Staged& out);
+// With an Arena, the scratch of these calls (sort keys and permutation,
+// staged sources, partial sums) is taken from it instead of the field's
+// grow-only buffers, nothing is allocated, and the permutation is not kept
+// for later calls.  launch_uj's kmax > 0 caps the number of source splits.
+int sort_field(vpm_field f, const uint32_t*& perm, Arena* ar = nullptr);
+int stage_field(vpm_field f, int kernel, int precision, bool sort, Staged& out, Arena* ar = nullptr);
 int launch_uj(vpm_field ft, const Staged& T, int64_t t0, int64_t nt,
-              const Staged& S, int64_t s0, int64_t s1, int kernel, int precision);
+              const Staged& S, int64_t s0, int64_t s1, int kernel, int precision,
+              Arena* ar = nullptr, int64_t kmax = 0);
+// Source splits of an all-pairs launch over nt targets and ns sources (at
+// most kmax when kmax > 0): K chunks of `chunk` sources, a multiple of the tile.
+void choose_split(int64_t nt, int64_t ns, int64_t kmax, int64_t& K, int64_t& chunk);
+// Scratch bytes of sort_field (temp storage of the radix sort included).
+int sort_scratch_bytes(vpm_field f, size_t& bytes);
 int launch_estr_impl(vpm_field f, int kernel, int transposed, int precision, const uint32_t* perm);
 int launch_reset(vpm_field f, int which);
 int launch_update_states(vpm_field f, double a, double b, double dt, const double Uinf[3],

@@ -116,6 +116,18 @@ int vpm_synchronize(vpm_field f);
  * (:492-507); then all-pairs U/J accumulate (fmm.jl:102-176); sfs then adds
  * E_str (Estr_direct!, subfilterscale_models.jl:43-92). */
 int vpm_uj_direct(vpm_field f, const vpm_config* cfg, uint32_t flags);
+/* vpm_uj_direct without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * same kernels and results as vpm_uj_direct; every scratch buffer (sort
+ * keys, staged sources, partial sums) comes from one 64 MiB device array per
+ * GPU that the library holds until the process ends.  np above what fits
+ * (about 3.4e5 particles for F32C, 4.9e5 for F64, with one source split)
+ * returns VPM_ERANGE; fewer source splits than vpm_uj_direct takes are used
+ * when its own do not fit.  VPM_UJ_SFS is rejected (VPM_EINVAL).  Calls on
+ * one GPU share the array and are serialized by the library (overlapping
+ * ones would race on it). */
+int vpm_uj_direct_noalloc(vpm_field f, const vpm_config* cfg, uint32_t flags);
 /* UJ_direct(source, target) (src/FLOWVPM_UJ.jl:48-50): every particle of
  * `source` acts on every particle of `target`; accumulates, no reset.  Both
  * fields must live on the same device. */
