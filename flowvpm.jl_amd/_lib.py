@@ -60,6 +60,11 @@ _SIGS = {
     "vpm_uj_direct_noalloc": (C.c_int, [C.c_void_p, C.POINTER(Config), C.c_uint32]),
     "vpm_uj_direct_fields": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Config)]),
     "vpm_p2p_ranges": (C.c_int, [C.c_void_p, C.POINTER(Config), _i64p, _i64p, _i64p, _i64p, C.c_int64]),
+    "vpm_plan_ranges": (C.c_int, [C.c_void_p, _i64p, _i64p, _i64p, _i64p, C.c_int64, C.POINTER(C.c_void_p)]),
+    "vpm_plan_destroy": (C.c_int, [C.c_void_p]),
+    "vpm_p2p_plan": (C.c_int, [C.c_void_p, C.POINTER(Config), C.c_void_p]),
+    "vpm_estr_plan": (C.c_int, [C.c_void_p, C.POINTER(Config), C.c_void_p]),
+    "vpm_zeta_plan": (C.c_int, [C.c_void_p, C.POINTER(Config), C.c_void_p]),
     "vpm_estr_direct": (C.c_int, [C.c_void_p, C.POINTER(Config)]),
     "vpm_reset_particles": (C.c_int, [C.c_void_p]),
     "vpm_reset_particles_sfs": (C.c_int, [C.c_void_p]),

@@ -227,6 +227,11 @@ class Field:
                                    Xb.ctypes.data_as(d), Gb.ctypes.data_as(d), Sb.ctypes.data_as(d)))
         return Xb.reshape(n, 3), Gb.reshape(n, 3), Sb
 
+    def plan_ranges(self, tb, te, sb, se):
+        """vpm_plan_ranges: the pair list planned once on this field (see
+        RangePlan)."""
+        return RangePlan(self, tb, te, sb, se)
+
     def profile(self, enable=True):
         check(lib().vpm_profile(self.handle, int(bool(enable))))
 
@@ -234,6 +239,63 @@ class Field:
         ms, n = C.c_double(), C.c_int64()
         check(lib().vpm_timing(self.handle, int(which), C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+def _range_arrays(tb, te, sb, se):
+    arrs = [np.ascontiguousarray(a, dtype=np.int64) for a in (tb, te, sb, se)]
+    if not (len(arrs[0]) == len(arrs[1]) == len(arrs[2]) == len(arrs[3])):
+        raise VPMError(_lib.VPM_EINVAL, "range arrays differ in length")
+    return arrs
+
+
+class RangePlan:
+    """A direct list of 0-based half-open (target, source) range pairs,
+    checked and planned once for a device field (vpm_plan_ranges) and served
+    to the U/J, E_str and zeta list kernels without re-planning or
+    re-uploading it: UJ_fmm hands one direct_list to the near field and then
+    to Estr_fmm! (FLOWVPM_UJ.jl:90-125).  Valid while the field holds the
+    particle count it was made for."""
+
+    def __init__(self, field, tb, te, sb, se):
+        arrs = _range_arrays(tb, te, sb, se)
+        ptr = [a.ctypes.data_as(_lib._i64p) for a in arrs]
+        h = C.c_void_p()
+        check(lib().vpm_plan_ranges(field.handle, ptr[0], ptr[1], ptr[2], ptr[3], len(arrs[0]), C.byref(h)))
+        self._h = h
+        self.field = field
+        self.npairs = len(arrs[0])
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise VPMError(_lib.VPM_EINVAL, "plan is closed")
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib().vpm_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def p2p(self, kernel=kernel_default, precision="f32c"):
+        """vpm_p2p_plan: U and J of every pair accumulated (fmm.direct!)."""
+        cfg = make_config(kernel, precision, True)
+        check(lib().vpm_p2p_plan(self.field.handle, C.byref(cfg), self.handle))
+
+    def estr(self, kernel=kernel_default, precision="f32c", transposed=True):
+        """vpm_estr_plan: E_str of every pair onto SFS (Estr_fmm!)."""
+        cfg = make_config(kernel, precision, transposed)
+        check(lib().vpm_estr_plan(self.field.handle, C.byref(cfg), self.handle))
+
+    def zeta(self, kernel=kernel_default, precision="f32c"):
+        """vpm_zeta_plan: J[1:3] += the zeta field of every pair's sources."""
+        cfg = make_config(kernel, precision, True)
+        check(lib().vpm_zeta_plan(self.field.handle, C.byref(cfg), self.handle))
 
 
 def make_config(kernel=kernel_default, precision="f32c", transposed=True):
@@ -535,6 +597,30 @@ def nextstep(pfield, dt, *, relax=False, resident=False):
     pfield.nt += 1
 
 
+def _resident_callback(pfield, fun, dt):
+    """Calls fun(pfield, t, dt) on a device-resident field as the reference
+    calls it on the live field (FLOWVPM_utils.jl:100,117): the host copy is
+    downloaded first, and whatever the callback changed (particles appended,
+    removed or edited) is brought back to the device afterwards.  Appending
+    only takes vpm_add_particles; any other change re-uploads the field.  The
+    callback's return value is left in pfield._last_callback_result."""
+    np0 = pfield.np
+    if np0:
+        pfield_download(pfield)
+    before = pfield.particles[:np0].copy()
+    pfield._last_callback_result = fun(pfield, pfield.t, dt)
+    n = pfield.np
+    kept = pfield.particles[:min(n, np0)]
+    same_prefix = n >= np0 and np.array_equal(kept, before, equal_nan=True)
+    if same_prefix and n == np0:
+        return
+    f = pfield.device_field()
+    if same_prefix:
+        f.add_particles(pfield.particles[np0:n])
+    else:
+        f.set_particles(pfield.particles[:n])
+
+
 def run_vpm(pfield, dt, nsteps, *, runtime_function=None, static_particles_function=None, resident=True,
             save_path=None, save_pfield=True, create_savepath=True, run_name="pfield", nsteps_save=1,
             prompt=True, save_time=True):
@@ -544,9 +630,12 @@ def run_vpm(pfield, dt, nsteps, *, runtime_function=None, static_particles_funct
     removed after the step unless it returns False (:100-112); then
     runtime_function(pfield, t, dt) ends the run by returning True.  The
     viscous scheme must be compatible with the kernel (:56-64,
-    FLOWVPM.jl:259-271).  State stays
-    on the device between steps when resident=True: appended particles are
-    uploaded with vpm_add_particles and removed with vpm_remove_particle.
+    FLOWVPM.jl:259-271).  State stays on the device between steps when
+    resident=True.  Both callbacks see the live field: it is downloaded
+    before each call, and what the call changed (appended, removed or edited
+    particles) goes back to the device after it.  Particles appended by
+    static_particles_function are removed after the step with
+    vpm_remove_particle.
 
     With save_path, the folder is made (an existing one is replaced when
     create_savepath, :66-68) and the field is saved as <run_name>.<nt>.h5 +
@@ -564,25 +653,30 @@ def run_vpm(pfield, dt, nsteps, *, runtime_function=None, static_particles_funct
         rl = pfield.relaxation
         relax = rl.method != 0 and rl.nsteps_relax >= 1 and i > 0 and i % rl.nsteps_relax == 0
         org_np = pfield.np
+        downloaded = False
         if i != 0:
             remove = None
             if static_particles_function is not None:
-                remove = static_particles_function(pfield, pfield.t, dt)
-                if resident and pfield.np > org_np:
-                    pfield.device_field().add_particles(pfield.particles[org_np:pfield.np])
+                if resident:
+                    # the callback reads and may change the live field
+                    _resident_callback(pfield, static_particles_function, dt)
+                    remove = pfield._last_callback_result
+                else:
+                    remove = static_particles_function(pfield, pfield.t, dt)
             nextstep(pfield, dt, relax=relax, resident=resident)
             if remove is None or remove:
                 for pi in range(pfield.np - 1, org_np - 1, -1):
                     remove_particle(pfield, pi)
                     if resident:
                         pfield.device_field().remove_particle(pi)
-        downloaded = False
         breakflag = False
         if runtime_function is not None:
-            if resident and pfield.np:
-                pfield_download(pfield)
+            if resident:
+                _resident_callback(pfield, runtime_function, dt)
                 downloaded = True
-            breakflag = bool(runtime_function(pfield, pfield.t, dt))
+                breakflag = bool(pfield._last_callback_result)
+            else:
+                breakflag = bool(runtime_function(pfield, pfield.t, dt))
         if save_pfield and save_path is not None and (i % nsteps_save == 0 or i == nsteps or breakflag):
             if resident and pfield.np and not downloaded:
                 pfield_download(pfield)

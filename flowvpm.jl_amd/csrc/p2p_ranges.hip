@@ -2,27 +2,21 @@
 // FastMultipole's leaf-pair P2P (fmm.direct! per direct-list entry,
 // src/FLOWVPM_fmm.jl:102-168), batched into one launch on one device copy.
 //
-// Target ranges of different pairs may overlap, so the host first cuts the
-// target index space at every range end into disjoint elementary intervals
-// and gives each interval the list of source ranges of every pair covering
-// it, in pair order.  A block then owns up to 256 targets of one interval,
-// streams its source ranges through LDS, and adds the result onto its own
-// targets' records: no two blocks write the same particle, and each target's
-// sum runs in pair order.  The pair arithmetic is p2p_pair.h's.
+// The host planner (range_plan.cpp) cuts the target index space into
+// disjoint elementary intervals, each with the source ranges of every pair
+// covering it, in pair order.  A block (or, for FMM leaves, a wave) owns the
+// targets of one item, streams its source ranges through LDS, and adds the
+// result onto its own targets' records: no two blocks write the same
+// particle, and each target's sum runs in pair order.  The pair arithmetic
+// is p2p_pair.h's.  A plan can be made once and served to U/J, E_str and
+// zeta (vpm_plan_ranges).
 #include <algorithm>
-#include <new>
 #include <string>
-#include <thread>
-#include <vector>
 
 #include "p2p_pair.h"
+#include "range_plan.h"
 
 namespace vpm {
-
-struct RangeItem {
-    int64_t t_begin, t_end;    // targets of this block (<= 256, one interval)
-    int64_t l_begin, l_end;    // its source ranges in the list arrays
-};
 
 template <int K>
 __global__ __launch_bounds__(kBlock) void ranges_f32c_kernel(const RangeItem* __restrict__ items,
@@ -295,284 +289,82 @@ __global__ __launch_bounds__(kBlock) void ranges_f64_kernel(const RangeItem* __r
 }
 #pragma clang fp contract(on)
 
-// Host plan of a pair list: elementary target intervals, each with the
-// source ranges of every pair covering it (pair order), cut into items of at
-// most `chunk` targets.  Items and source ranges are written straight into
-// the field's pinned staging buffer (items first) and reach the device in
-// one copy.
-struct RangePlan {
-    RangeItem* items = nullptr;   // in f->hstage
-    int2* src = nullptr;          // source ranges (begin, end); positions fit 32 bits
+// ------------------------------------------------------------- device plans
+// A plan made by the host planner: items written straight into the field's
+// pinned staging buffer (items first, then source ranges), copied into a
+// device buffer with the same layout.  On leaf lists the packed source
+// ranges are copied in pieces as the planner's workers write them, so the
+// upload runs behind the packing.  The pinned source stays in use until the
+// field's stream is synchronized, which every entry point does before it
+// returns (also on failure: copies of a list that then fails its checks land
+// in the device buffer only).
+namespace {
+
+struct FieldSink final : PlanSink {
+    vpm_field f;
+    DevBuf* dst;
     int64_t nitems = 0, nsrc = 0;
-    bool src_on_device = false;   // src already copied to the device plan buffer
-    bool wave = false;            // F32C over short intervals: one wave per <= 64 / R targets
-    int R = 1;                    // lane groups per target of the wave kernel
-};
 
-static int stage_plan(vpm_field f, int64_t nitems, int64_t nsrc, RangePlan& plan)
-{
-    const size_t bytes = sizeof(RangeItem) * (size_t)nitems + sizeof(int2) * (size_t)nsrc;
-    if (int rc = ensure_pinned(f->hstage, bytes)) return rc;
-    plan.items = (RangeItem*)f->hstage.p;
-    plan.src = (int2*)(plan.items + nitems);
-    plan.nitems = nitems;
-    plan.nsrc = nsrc;
-    return VPM_OK;
-}
+    FieldSink(vpm_field f_, DevBuf* d) : f(f_), dst(d) {}
 
-static int range_error(int64_t k)
-{
-    return fail(VPM_EINVAL, "range pair " + std::to_string(k) + " outside [0, np)");
-}
+    size_t items_bytes() const { return sizeof(RangeItem) * (size_t)nitems; }
+    size_t bytes() const { return items_bytes() + sizeof(SrcRange) * (size_t)nsrc; }
 
-// Runs fn(k0, k1, t) over [0, n) cut into T <= kPlanThreads contiguous chunks
-// of at least 2^18 pairs, chunk t on thread t (the caller's thread takes
-// chunk 0; a chunk whose thread cannot start runs on the caller's).  Returns T.
-constexpr int kPlanThreads = 8;
-constexpr int64_t kPlanPiece = int64_t(1) << 18;   // pairs per thread at least, and per early copy
-template <class Fn>
-static int plan_parallel(int64_t n, Fn&& fn)
-{
-    int T = (int)std::min<int64_t>(n / kPlanPiece + 1, kPlanThreads);
-    const unsigned hc = std::thread::hardware_concurrency();
-    if (hc > 0) T = std::min(T, (int)hc);
-    if (T <= 1) {
-        fn((int64_t)0, n, 0);
-        return 1;
-    }
-    std::vector<std::thread> th;
-    std::vector<int> inline_chunks;
-    th.reserve(T);
-    inline_chunks.reserve(T);
-    for (int t = 1; t < T; ++t) {
-        try {
-            th.emplace_back(fn, n * t / T, n * (t + 1) / T, t);
-        } catch (...) {
-            inline_chunks.push_back(t);
-        }
-    }
-    fn((int64_t)0, n / T, 0);
-    for (int t : inline_chunks) fn(n * t / T, n * (t + 1) / T, t);
-    for (std::thread& x : th) x.join();
-    return T;
-}
-
-// Lowest failing pair index over the chunks' first failures (-1 if none).
-static int64_t first_bad(const int64_t* bad, int T)
-{
-    int64_t k = -1;
-    for (int t = 0; t < T; ++t)
-        if (bad[t] >= 0 && (k < 0 || bad[t] < k)) k = bad[t];
-    return k;
-}
-
-// allow_wave: the caller has a wave kernel, taken when the mean target range
-// of the list is at most 128 targets (FMM leaves).  Every pair is checked
-// (0 <= begin <= end <= np) in the passes that read it, before any device
-// work.
-static int plan_ranges_impl(vpm_field f, const int64_t* tb, const int64_t* te, const int64_t* sb,
-                            const int64_t* se, int64_t npairs, bool allow_wave, RangePlan& plan, int rmax)
-{
-    plan = RangePlan();
-    const int64_t np = f->np;
-    // runs of consecutive pairs with one target range, found per chunk of the
-    // list (a run belongs to the chunk it starts in) and joined in list order
-    struct Run { int64_t tb, te, l0, l1; };
-    struct Part { std::vector<Run> runs; int64_t tot = 0, cnt = 0; bool oom = false; };
-    Part parts[kPlanThreads];
-    int64_t bad[kPlanThreads];
-    for (int t = 0; t < kPlanThreads; ++t) bad[t] = -1;
-    auto same = [tb, te](int64_t a, int64_t b) { return tb[a] == tb[b] && te[a] == te[b]; };
-    const int T = plan_parallel(npairs, [&](int64_t k0, int64_t k1, int t) {
-        Part& pt = parts[t];
-        int64_t k = k0;
-        while (k < k1 && k > 0 && same(k, k - 1)) ++k;
-        while (k < k1) {
-            if (tb[k] < 0 || te[k] > np || tb[k] > te[k]) { bad[t] = k; return; }
-            int64_t m = k + 1;
-            while (m < npairs && same(m, k)) ++m;
-            try {
-                pt.runs.push_back({tb[k], te[k], k, m});
-            } catch (...) {   // no exception leaves a worker thread
-                pt.oom = true;
-                return;
-            }
-            if (te[k] > tb[k]) { pt.tot += te[k] - tb[k]; ++pt.cnt; }
-            k = m;
-        }
-    });
-    for (int t = 0; t < T; ++t)
-        if (parts[t].oom) throw std::bad_alloc();
-    if (int64_t k = first_bad(bad, T); k >= 0) return range_error(k);
-    std::vector<Run> runs;
-    int64_t tot = 0, cnt = 0;
+    int stage(int64_t ni, int64_t ns, RangeItem*& items, SrcRange*& src) override
     {
-        size_t nr = 0;
-        for (int t = 0; t < T; ++t) nr += parts[t].runs.size();
-        runs.reserve(nr);
-        for (int t = 0; t < T; ++t) {
-            runs.insert(runs.end(), parts[t].runs.begin(), parts[t].runs.end());
-            tot += parts[t].tot;
-            cnt += parts[t].cnt;
-        }
-    }
-    plan.wave = allow_wave && cnt > 0 && tot <= 128 * cnt;
-    // leaves of <= 8, 16 or 32 targets on average share a wave's lanes 8, 4
-    // or 2 ways
-    plan.R = !plan.wave ? 1 : tot <= 8 * cnt ? 8 : tot <= 16 * cnt ? 4 : tot <= 32 * cnt ? 2 : 1;
-    plan.R = std::min(plan.R, rmax);
-    const int64_t chunk = plan.wave ? 64 / plan.R : kBlock;
-    auto pieces = [chunk](int64_t a, int64_t b) { return b > a ? (b - a + chunk - 1) / chunk : 0; };
-
-    // The usual FMM direct list: every target range is a leaf, so ranges are
-    // identical or disjoint, and the pairs of one target range are listed
-    // together.  Then the list itself is the per-interval source list.
-    // Empty target ranges hold no targets and are left out of the check.  Lists
-    // whose runs come in target order, as FMM lists do, need no sort.
-    bool leaves = true;
-    {
-        const Run* prev = nullptr;
-        bool sorted = true;
-        for (const Run& r : runs) {
-            if (r.te == r.tb) continue;
-            if (prev && r.tb < prev->tb) { sorted = false; break; }
-            if (prev && r.tb < prev->te) { leaves = false; break; }   // overlap, or one range in two runs
-            prev = &r;
-        }
-        if (!sorted) {
-            std::vector<const Run*> order;
-            for (const Run& r : runs)
-                if (r.te > r.tb) order.push_back(&r);
-            std::sort(order.begin(), order.end(), [](const Run* a, const Run* b) { return a->tb < b->tb; });
-            for (size_t r = 1; r < order.size() && leaves; ++r)
-                if (order[r]->tb < order[r - 1]->te) leaves = false;
-        }
-    }
-    if (leaves) {
-        int64_t nitems = 0;
-        for (const Run& r : runs) nitems += pieces(r.tb, r.te);
-        if (int rc = stage_plan(f, nitems, npairs, plan)) return rc;
-        // The device plan buffer has the host layout (items, then source
-        // ranges), so each piece of packed source ranges is copied as soon as
-        // it is written, behind the packing of the next.  Copies of a list
-        // that then fails its checks land in the plan buffer only; the entry
-        // points synchronize before they return.
-        const size_t src_off = sizeof(RangeItem) * (size_t)nitems;
-        if (int rc = ensure(f->rng, src_off + sizeof(int2) * (size_t)npairs + 64)) return rc;
-        char* dsrc = (char*)f->rng.p + src_off;
-        hipError_t copy_err[kPlanThreads];
-        for (int t = 0; t < kPlanThreads; ++t) { bad[t] = -1; copy_err[t] = hipSuccess; }
-        int2* src = plan.src;
-        hipStream_t stream = f->stream;
-        const int T2 = plan_parallel(npairs, [&](int64_t k0, int64_t k1, int t) {
-            for (int64_t a = k0; a < k1; a += kPlanPiece) {
-                const int64_t b = std::min(a + kPlanPiece, k1);
-                for (int64_t k = a; k < b; ++k) {
-                    if (sb[k] < 0 || se[k] > np || sb[k] > se[k]) { bad[t] = k; return; }
-                    src[k] = make_int2((int)sb[k], (int)se[k]);
-                }
-                const hipError_t e = hipMemcpyAsync(dsrc + sizeof(int2) * (size_t)a, src + a,
-                                                    sizeof(int2) * (size_t)(b - a), hipMemcpyHostToDevice, stream);
-                if (e != hipSuccess) { copy_err[t] = e; return; }
-            }
-        });
-        if (int64_t k = first_bad(bad, T2); k >= 0) return range_error(k);
-        for (int t = 0; t < T2; ++t)
-            if (copy_err[t] != hipSuccess) return hip_fail(copy_err[t], "hipMemcpyAsync(range lists)");
-        plan.src_on_device = true;
-        int64_t m = 0;
-        for (const Run& r : runs)
-            for (int64_t t = r.tb; t < r.te; t += chunk)
-                plan.items[m++] = {t, std::min<int64_t>(t + chunk, r.te), r.l0, r.l1};
+        nitems = ni;
+        nsrc = ns;
+        if (int rc = ensure_pinned(f->hstage, bytes())) return rc;
+        items = (RangeItem*)f->hstage.p;
+        src = (SrcRange*)(items + ni);
         return VPM_OK;
     }
+    int prepare_copies(int64_t, int64_t) override { return ensure(*dst, bytes() + 64); }
+    int copy_piece(int64_t a, int64_t b) override
+    {
+        const char* hsrc = (const char*)f->hstage.p + items_bytes();
+        char* dsrc = (char*)dst->p + items_bytes();
+        const size_t off = sizeof(SrcRange) * (size_t)a;
+        const hipError_t e = hipMemcpyAsync(dsrc + off, hsrc + off, sizeof(SrcRange) * (size_t)(b - a),
+                                            hipMemcpyHostToDevice, f->stream);
+        return e == hipSuccess ? 0 : (int)e;
+    }
+    int report(int code) override { return hip_fail((hipError_t)code, "hipMemcpyAsync(range lists)"); }
+};
 
-    // general lists: elementary intervals between all range ends
-    std::vector<int64_t> cuts;
-    cuts.reserve(2 * (size_t)npairs);
-    for (int64_t k = 0; k < npairs; ++k) {
-        if (sb[k] < 0 || se[k] > np || sb[k] > se[k]) return range_error(k);
-        if (tb[k] < te[k] && sb[k] < se[k]) { cuts.push_back(tb[k]); cuts.push_back(te[k]); }
-    }
-    if (cuts.empty()) return VPM_OK;
-    std::sort(cuts.begin(), cuts.end());
-    cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
-    const size_t nint = cuts.size() - 1;
-    std::vector<int64_t> count(nint + 1, 0);
-    for (int64_t k = 0; k < npairs; ++k) {
-        if (!(tb[k] < te[k] && sb[k] < se[k])) continue;
-        const size_t lo = std::lower_bound(cuts.begin(), cuts.end(), tb[k]) - cuts.begin();
-        const size_t hi = std::lower_bound(cuts.begin(), cuts.end(), te[k]) - cuts.begin();
-        for (size_t m = lo; m < hi; ++m) count[m + 1]++;
-    }
-    for (size_t m = 0; m < nint; ++m) count[m + 1] += count[m];
-    int64_t nitems = 0;
-    for (size_t m = 0; m < nint; ++m)
-        if (count[m + 1] != count[m]) nitems += pieces(cuts[m], cuts[m + 1]);
-    if (int rc = stage_plan(f, nitems, count[nint], plan)) return rc;
-    std::vector<int64_t> fill(count.begin(), count.end() - 1);
-    for (int64_t k = 0; k < npairs; ++k) {
-        if (!(tb[k] < te[k] && sb[k] < se[k])) continue;
-        const size_t lo = std::lower_bound(cuts.begin(), cuts.end(), tb[k]) - cuts.begin();
-        const size_t hi = std::lower_bound(cuts.begin(), cuts.end(), te[k]) - cuts.begin();
-        for (size_t m = lo; m < hi; ++m) plan.src[fill[m]++] = make_int2((int)sb[k], (int)se[k]);
-    }
-    int64_t q = 0;
-    for (size_t m = 0; m < nint; ++m) {
-        if (count[m + 1] == count[m]) continue;
-        for (int64_t t = cuts[m]; t < cuts[m + 1]; t += chunk)
-            plan.items[q++] = {t, std::min<int64_t>(t + chunk, cuts[m + 1]), count[m], count[m + 1]};
-    }
-    return VPM_OK;
-}
+}  // namespace
 
-// Host vectors of the plan that cannot be allocated end the call with
-// VPM_ENOMEM; no exception crosses the C ABI.
-static int plan_ranges(vpm_field f, const int64_t* tb, const int64_t* te, const int64_t* sb, const int64_t* se,
-                       int64_t npairs, bool allow_wave, RangePlan& plan, int rmax = 1)
+int make_dev_plan(vpm_field f, const int64_t* tb, const int64_t* te, const int64_t* sb, const int64_t* se,
+                  int64_t npairs, bool allow_wave, int rmax, DevBuf& dst, DevPlan& out)
 {
-    try {
-        return plan_ranges_impl(f, tb, te, sb, se, npairs, allow_wave, plan, rmax);
-    } catch (const std::exception&) {
-        plan = RangePlan();
-        return fail(VPM_ENOMEM, "host allocation failed while planning a range list");
-    }
-}
-
-// One copy of the staged plan to the device.  The pinned source stays in use
-// until the stream is synchronized, which every entry point does before it
-// returns.
-static int upload_plan(vpm_field f, const RangePlan& plan, RangeItem*& d_items, int2*& d_src)
-{
-    const size_t bytes_items = sizeof(RangeItem) * (size_t)plan.nitems;
-    const size_t bytes = bytes_items + sizeof(int2) * (size_t)plan.nsrc;
-    if (int rc = ensure(f->rng, bytes + 64)) return rc;   // the size the plan already ensured, if it did
-    char* base = (char*)f->rng.p;
-    d_items = (RangeItem*)base;
-    d_src = (int2*)(base + bytes_items);
-    hipError_t e = hipMemcpyAsync(base, f->hstage.p, plan.src_on_device ? bytes_items : bytes,
-                                  hipMemcpyHostToDevice, f->stream);
+    out = DevPlan();
+    FieldSink sink(f, &dst);
+    PlanShape shape;
+    if (int rc = plan_ranges(f->np, tb, te, sb, se, npairs, allow_wave, rmax, sink, shape)) return rc;
+    if (shape.nitems == 0) return VPM_OK;
+    if (int rc = ensure(dst, sink.bytes() + 64)) return rc;   // the size the copies already ensured, if any
+    char* base = (char*)dst.p;
+    const hipError_t e = hipMemcpyAsync(base, f->hstage.p, shape.src_copied ? sink.items_bytes() : sink.bytes(),
+                                        hipMemcpyHostToDevice, f->stream);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(range lists)");
+    out.items = (const RangeItem*)base;
+    out.src = (const int2*)(base + sink.items_bytes());
+    out.nitems = shape.nitems;
+    out.wave = shape.wave;
+    out.R = shape.R;
     return VPM_OK;
 }
 
-int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb, const int64_t* te,
-                      const int64_t* sb, const int64_t* se, int64_t npairs)
+// U/J of a planned list (F32C wave kernels for wave plans, the block kernels
+// otherwise; F64 always takes the block kernel, whatever the item size).
+int launch_p2p_plan(vpm_field f, int kernel, int precision, const DevPlan& dp)
 {
-    // F32C over short target intervals (FMM leaves, mean <= 128 targets):
-    // one wave per <= 64 targets; otherwise one block per <= 256 targets
-    RangePlan plan;
-    int rc;
-    if ((rc = plan_ranges(f, tb, te, sb, se, npairs, precision != VPM_PREC_F64, plan, 8))) return rc;
-    if (plan.nitems == 0) return VPM_OK;
-    const bool wave = plan.wave;
+    if (dp.nitems == 0) return VPM_OK;
     Staged st;
-    if ((rc = stage_field(f, kernel, precision, false, st))) return rc;
-    RangeItem* d_items;
-    int2* d_src;
-    if ((rc = upload_plan(f, plan, d_items, d_src))) return rc;
-    const dim3 grid((unsigned)plan.nitems);
-    hipError_t e;
+    if (int rc = stage_field(f, kernel, precision, false, st)) return rc;
+    const RangeItem* d_items = dp.items;
+    const int2* d_src = dp.src;
+    const dim3 grid((unsigned)dp.nitems);
     timing_begin(f);
     if (precision == VPM_PREC_F64) {
         switch (kernel) {
@@ -581,14 +373,14 @@ int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb,
 #undef VPM_R64
         default: return fail(VPM_EINVAL, "unknown kernel id");
         }
-    } else if (wave) {
-        const int64_t ni = plan.nitems;
+    } else if (dp.wave) {
+        const int64_t ni = dp.nitems;
         const dim3 wgrid((unsigned)((ni + kBlock / 64 - 1) / (kBlock / 64)));
         switch (kernel) {
 #define VPM_RWR(KID, R) hipLaunchKernelGGL((ranges_wave_f32c_kernel<KID, R>), wgrid, dim3(kBlock), 0, f->stream, d_items, ni, d_src, st.pos, st.ld, st.gs, st.cut, f->P)
 #define VPM_RW(KID)                                                                   \
     case KID:                                                                         \
-        switch (plan.R) {                                                             \
+        switch (dp.R) {                                                               \
         case 8: VPM_RWR(KID, 8); break;                                               \
         case 4: VPM_RWR(KID, 4); break;                                               \
         case 2: VPM_RWR(KID, 2); break;                                               \
@@ -608,13 +400,20 @@ int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb,
         default: return fail(VPM_EINVAL, "unknown kernel id");
         }
     }
-    e = hipGetLastError();
+    const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "p2p_ranges kernel");
     timing_end(f, precision == VPM_PREC_F64 ? 1 : 0);
-    // the pinned plan must outlive the asynchronous copy
-    e = hipStreamSynchronize(f->stream);
-    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     return VPM_OK;
+}
+
+// F32C over short target intervals (FMM leaves, mean <= 128 targets): one
+// wave per <= 64 targets; otherwise one block per <= 256 targets.
+int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb, const int64_t* te,
+                      const int64_t* sb, const int64_t* se, int64_t npairs)
+{
+    DevPlan dp;
+    if (int rc = make_dev_plan(f, tb, te, sb, se, npairs, precision != VPM_PREC_F64, 8, f->rng, dp)) return rc;
+    return launch_p2p_plan(f, kernel, precision, dp);
 }
 
 // ------------------------------------------------------------ E_str ranges
@@ -1005,27 +804,25 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_f64_kernel(const RangeItem
 }
 #pragma clang fp contract(on)
 
-int launch_estr_ranges(vpm_field f, int kernel, int precision, int transposed, const int64_t* tb,
-                       const int64_t* te, const int64_t* sb, const int64_t* se, int64_t npairs)
+// E_str of a planned list: F32C wave kernels for wave plans, block kernels
+// otherwise; FP64 (and the singular kernel, whose zeta is a delta) always
+// takes the FP64 block kernel, whatever the item size.
+int launch_estr_plan(vpm_field f, int kernel, int precision, int transposed, const DevPlan& dp)
 {
+    if (dp.nitems == 0) return VPM_OK;
     const bool f64 = precision == VPM_PREC_F64 || kernel == VPM_KERNEL_SINGULAR;
-    RangePlan plan;
-    int rc;
-    if ((rc = plan_ranges(f, tb, te, sb, se, npairs, !f64, plan, 8))) return rc;
-    if (plan.nitems == 0) return VPM_OK;
-    RangeItem* d_items;
-    int2* d_src;
-    if ((rc = upload_plan(f, plan, d_items, d_src))) return rc;
-    const dim3 grid((unsigned)plan.nitems);
+    const RangeItem* d_items = dp.items;
+    const int2* d_src = dp.src;
+    const dim3 grid((unsigned)dp.nitems);
     timing_begin(f);
-    if (!f64 && plan.wave) {
-        const int64_t ni = plan.nitems;
+    if (!f64 && dp.wave) {
+        const int64_t ni = dp.nitems;
         const dim3 wgrid((unsigned)((ni + kBlock / 64 - 1) / (kBlock / 64)));
         switch (kernel) {
 #define VPM_EWR(KID, R) hipLaunchKernelGGL((estr_ranges_wave_f32c_kernel<KID, R>), wgrid, dim3(kBlock), 0, f->stream, d_items, ni, d_src, f->P, transposed)
 #define VPM_EW(KID)                                                                   \
     case KID:                                                                         \
-        switch (plan.R) {                                                             \
+        switch (dp.R) {                                                               \
         case 8: VPM_EWR(KID, 8); break;                                               \
         case 4: VPM_EWR(KID, 4); break;                                               \
         case 2: VPM_EWR(KID, 2); break;                                               \
@@ -1052,12 +849,19 @@ int launch_estr_ranges(vpm_field f, int kernel, int precision, int transposed, c
         default: return fail(VPM_EINVAL, "unknown kernel id");
         }
     }
-    hipError_t e = hipGetLastError();
+    const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "estr_ranges kernel");
     timing_end(f, 2);
-    e = hipStreamSynchronize(f->stream);
-    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     return VPM_OK;
+}
+
+int launch_estr_ranges(vpm_field f, int kernel, int precision, int transposed, const int64_t* tb,
+                       const int64_t* te, const int64_t* sb, const int64_t* se, int64_t npairs)
+{
+    const bool f64 = precision == VPM_PREC_F64 || kernel == VPM_KERNEL_SINGULAR;
+    DevPlan dp;
+    if (int rc = make_dev_plan(f, tb, te, sb, se, npairs, !f64, 8, f->rng, dp)) return rc;
+    return launch_estr_plan(f, kernel, precision, transposed, dp);
 }
 
 // ------------------------------------------------------------- zeta ranges
@@ -1314,19 +1118,17 @@ __global__ __launch_bounds__(kBlock) void zeta_ranges_f64_kernel(const RangeItem
 }
 #pragma clang fp contract(on)
 
-int launch_zeta_ranges(vpm_field f, int kernel, int precision, const int64_t* tb, const int64_t* te,
-                       const int64_t* sb, const int64_t* se, int64_t npairs)
+// zeta of a planned list: the F32C wave kernels (R > 1) or the 64-lane block
+// kernel for wave plans, 256-lane blocks otherwise; FP64 (and the singular
+// kernel) takes the FP64 block kernel, whatever the item size.
+int launch_zeta_plan(vpm_field f, int kernel, int precision, const DevPlan& dp)
 {
+    if (dp.nitems == 0) return VPM_OK;
     const bool f64 = precision == VPM_PREC_F64 || kernel == VPM_KERNEL_SINGULAR;
-    RangePlan plan;
-    int rc;
-    if ((rc = plan_ranges(f, tb, te, sb, se, npairs, !f64, plan, 8))) return rc;
-    if (plan.nitems == 0) return VPM_OK;
-    const bool wave = plan.wave;
-    RangeItem* d_items;
-    int2* d_src;
-    if ((rc = upload_plan(f, plan, d_items, d_src))) return rc;
-    const dim3 grid((unsigned)plan.nitems);
+    const bool wave = dp.wave;
+    const RangeItem* d_items = dp.items;
+    const int2* d_src = dp.src;
+    const dim3 grid((unsigned)dp.nitems);
     timing_begin(f);
     if (f64) {
         switch (kernel) {
@@ -1336,13 +1138,13 @@ int launch_zeta_ranges(vpm_field f, int kernel, int precision, const int64_t* tb
         default: return fail(VPM_EINVAL, "unknown kernel id");
         }
     } else if (wave) {
-        const int64_t ni = plan.nitems;
+        const int64_t ni = dp.nitems;
         const dim3 wgrid((unsigned)((ni + kBlock / 64 - 1) / (kBlock / 64)));
         switch (kernel) {
 #define VPM_ZWR(KID, R) hipLaunchKernelGGL((zeta_ranges_wave_f32c_kernel<KID, R>), wgrid, dim3(kBlock), 0, f->stream, d_items, ni, d_src, f->P)
 #define VPM_ZW(KID)                                                                                                   \
     case KID:                                                                                                         \
-        switch (plan.R) {                                                                                             \
+        switch (dp.R) {                                                                                               \
         case 8: VPM_ZWR(KID, 8); break;                                                                               \
         case 4: VPM_ZWR(KID, 4); break;                                                                               \
         case 2: VPM_ZWR(KID, 2); break;                                                                               \
@@ -1362,12 +1164,20 @@ int launch_zeta_ranges(vpm_field f, int kernel, int precision, const int64_t* tb
         default: return fail(VPM_EINVAL, "unknown kernel id");
         }
     }
-    hipError_t e = hipGetLastError();
+    const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "zeta_ranges kernel");
     timing_end(f, 4);
-    e = hipStreamSynchronize(f->stream);
-    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     return VPM_OK;
 }
+
+int launch_zeta_ranges(vpm_field f, int kernel, int precision, const int64_t* tb, const int64_t* te,
+                       const int64_t* sb, const int64_t* se, int64_t npairs)
+{
+    const bool f64 = precision == VPM_PREC_F64 || kernel == VPM_KERNEL_SINGULAR;
+    DevPlan dp;
+    if (int rc = make_dev_plan(f, tb, te, sb, se, npairs, !f64, 8, f->rng, dp)) return rc;
+    return launch_zeta_plan(f, kernel, precision, dp);
+}
+
 
 }  // namespace vpm

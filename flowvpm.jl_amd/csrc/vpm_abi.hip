@@ -260,6 +260,12 @@ int vpm_destroy(vpm_field f)
             t->recorded = false;
         }
         f->tapes.clear();
+        for (vpm_plan pl : f->plans) {   // detach: the plan stays valid only for vpm_plan_destroy
+            release(pl->mem);
+            pl->f = nullptr;
+            pl->dp = DevPlan();
+        }
+        f->plans.clear();
         if (f->P) (void)hipFree(f->P);
         if (f->ev0) (void)hipEventDestroy(f->ev0);
         if (f->ev1) (void)hipEventDestroy(f->ev1);
@@ -458,6 +464,96 @@ int vpm_zeta_ranges(vpm_field f, const vpm_config* cfg, const int64_t* tb, const
     if (!tb || !te || !sb || !se) return fail(VPM_EINVAL, "null range array");
     // each pair is checked against [0, np) while its plan is built
     const int rc = launch_zeta_ranges(f, cfg->kernel, cfg->precision, tb, te, sb, se, npairs);
+    const int rs = sync(f);
+    return rc ? rc : rs;
+}
+
+// ---- planned direct lists ----
+int vpm_plan_ranges(vpm_field f, const int64_t* tb, const int64_t* te, const int64_t* sb, const int64_t* se,
+                    int64_t npairs, vpm_plan* out)
+{
+    if (!out) return fail(VPM_EINVAL, "null output pointer");
+    *out = nullptr;
+    VPM_FIELD_GUARD(f);
+    if (npairs < 0) return fail(VPM_EINVAL, "npairs < 0");
+    if (npairs > 0 && (!tb || !te || !sb || !se)) return fail(VPM_EINVAL, "null range array");
+    vpm_plan p = new (std::nothrow) vpm_plan_s;
+    if (!p) return fail(VPM_ENOMEM, "vpm_plan_ranges");
+    p->f = f;
+    p->np = f->np;
+    // the F32C leaf form (wave items, lane sharing up to 8); the FP64 block
+    // kernels run over the same items
+    int rc = npairs > 0 ? make_dev_plan(f, tb, te, sb, se, npairs, true, 8, p->mem, p->dp) : VPM_OK;
+    const int rs = sync(f);   // the pinned staging buffer is free again
+    if (!rc) rc = rs;
+    if (!rc) {
+        try {
+            f->plans.push_back(p);
+        } catch (const std::exception&) {
+            rc = fail(VPM_ENOMEM, "vpm_plan_ranges");
+        }
+    }
+    if (rc) {
+        release(p->mem);
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return VPM_OK;
+}
+
+int vpm_plan_destroy(vpm_plan p)
+{
+    if (!p) return VPM_OK;
+    if (vpm_field f = p->f) {   // null once the field was destroyed (mem already released)
+        std::lock_guard<std::mutex> lock(f->mu);
+        if (p->mem.p) {
+            if (int rc = set_device(f)) return rc;
+            (void)hipStreamSynchronize(f->stream);
+            release(p->mem);
+        }
+        f->plans.erase(std::remove(f->plans.begin(), f->plans.end(), p), f->plans.end());
+    }
+    delete p;
+    return VPM_OK;
+}
+
+static int check_plan(vpm_field f, vpm_plan p)
+{
+    if (!p) return fail(VPM_EINVAL, "null vpm_plan");
+    if (p->f != f) return fail(VPM_EINVAL, p->f ? "plan belongs to another field" : "the plan's field was destroyed");
+    if (p->np != f->np)
+        return fail(VPM_EINVAL, "plan was made for np = " + std::to_string(p->np) + ", the field holds " +
+                                    std::to_string(f->np));
+    return VPM_OK;
+}
+
+int vpm_p2p_plan(vpm_field f, const vpm_config* cfg, vpm_plan p)
+{
+    VPM_FIELD_GUARD(f);
+    if (int rc = check_cfg(cfg)) return rc;
+    if (int rc = check_plan(f, p)) return rc;
+    const int rc = launch_p2p_plan(f, cfg->kernel, cfg->precision, p->dp);
+    const int rs = sync(f);
+    return rc ? rc : rs;
+}
+
+int vpm_estr_plan(vpm_field f, const vpm_config* cfg, vpm_plan p)
+{
+    VPM_FIELD_GUARD(f);
+    if (int rc = check_cfg(cfg)) return rc;
+    if (int rc = check_plan(f, p)) return rc;
+    const int rc = launch_estr_plan(f, cfg->kernel, cfg->precision, cfg->transposed, p->dp);
+    const int rs = sync(f);
+    return rc ? rc : rs;
+}
+
+int vpm_zeta_plan(vpm_field f, const vpm_config* cfg, vpm_plan p)
+{
+    VPM_FIELD_GUARD(f);
+    if (int rc = check_cfg(cfg)) return rc;
+    if (int rc = check_plan(f, p)) return rc;
+    const int rc = launch_zeta_plan(f, cfg->kernel, cfg->precision, p->dp);
     const int rs = sync(f);
     return rc ? rc : rs;
 }

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/flowvpm_hip.h"
+#include "range_plan.h"
 
 namespace vpm {
 
@@ -85,6 +86,7 @@ struct vpm_field_s {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     vpm::Timing timing[5];      // U/J F32C, U/J F64, E_str, adjoint, zeta
     std::vector<vpm_tape> tapes;     // live tapes of this field (released with it)
+    std::vector<vpm_plan> plans;     // live range plans of this field (released with it)
 };
 
 // Stage records of one rk3 step (state after each stage's UJ evaluation),
@@ -96,6 +98,27 @@ struct vpm_tape_s {
     vpm_step_config s{};
     double dt = 0.0;
     bool recorded = false;
+};
+
+namespace vpm {
+// A range plan on the device: items, then source ranges (int2 = SrcRange).
+struct DevPlan {
+    const RangeItem* items = nullptr;
+    const int2* src = nullptr;
+    int64_t nitems = 0;
+    bool wave = false;   // F32C wave kernels (FMM leaves)
+    int R = 1;           // lane groups per target of the wave kernels
+};
+}  // namespace vpm
+
+// A direct list planned once for a field (vpm_plan_ranges), served to the
+// U/J, E_str and zeta list kernels.  Valid while the field holds the np it
+// was made for.
+struct vpm_plan_s {
+    vpm_field f = nullptr;
+    int64_t np = 0;
+    vpm::DevBuf mem;                 // device items + source ranges
+    vpm::DevPlan dp;
 };
 
 namespace vpm {
@@ -146,6 +169,14 @@ int launch_sfs_coefficient(vpm_field f, int transposed, double alpha, double rlx
 int launch_update_uprev(vpm_field f);
 int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb, const int64_t* te,
                       const int64_t* sb, const int64_t* se, int64_t npairs);
+// Plans a pair list (range_plan.h) through the field's pinned staging buffer
+// into the device buffer dst; the field's stream must be synchronized
+// before the staging buffer is used again.
+int make_dev_plan(vpm_field f, const int64_t* tb, const int64_t* te, const int64_t* sb, const int64_t* se,
+                  int64_t npairs, bool allow_wave, int rmax, DevBuf& dst, DevPlan& out);
+int launch_p2p_plan(vpm_field f, int kernel, int precision, const DevPlan& dp);
+int launch_estr_plan(vpm_field f, int kernel, int precision, int transposed, const DevPlan& dp);
+int launch_zeta_plan(vpm_field f, int kernel, int precision, const DevPlan& dp);
 int launch_uj_adjoint(vpm_field f, int kernel, int precision, const double* Ubar, const double* Jbar,
                       double* Xbar, double* Gbar, double* Sbar);
 // device-side U/J adjoint staging (in f->adj) and launch

@@ -54,19 +54,27 @@ end
 config(pfield; precision=0) = VpmConfig(kernel_id(pfield.kernel), Int32(precision), Int32(pfield.transposed), 0)
 
 # One device field per ParticleField, kept across calls (no re-allocation).
-const FIELDS = IdDict{Any,Ptr{Cvoid}}()
+# The keys are held weakly, so a ParticleField that is no longer referenced
+# is collected, and its finalizer releases the device field (records, stream,
+# scratch buffers).  The finalizer owns the handle it frees; it does not look
+# the field up again.
+const FIELDS = WeakKeyDict{Any,Ptr{Cvoid}}()
+const FIELDS_LOCK = ReentrantLock()
 
 function device_field(pfield)
-    h = get(FIELDS, pfield, C_NULL)
-    if h == C_NULL
-        r = Ref{Ptr{Cvoid}}(C_NULL)
-        check(ccall((:vpm_create, libvpm), Cint, (Cint, Int64, Ref{Ptr{Cvoid}}),
-                    pfield.useGPU > 0 ? pfield.useGPU - 1 : 0, pfield.maxparticles, r))
-        h = r[]
-        FIELDS[pfield] = h
-        finalizer(p -> (ccall((:vpm_destroy, libvpm), Cint, (Ptr{Cvoid},), FIELDS[p]); delete!(FIELDS, p)), pfield)
+    lock(FIELDS_LOCK) do
+        h = get(FIELDS, pfield, C_NULL)
+        if h == C_NULL
+            r = Ref{Ptr{Cvoid}}(C_NULL)
+            check(ccall((:vpm_create, libvpm), Cint, (Cint, Int64, Ref{Ptr{Cvoid}}),
+                        pfield.useGPU > 0 ? pfield.useGPU - 1 : 0, pfield.maxparticles, r))
+            h = r[]
+            FIELDS[pfield] = h
+            handle = h
+            finalizer(_ -> ccall((:vpm_destroy, libvpm), Cint, (Ptr{Cvoid},), handle), pfield)
+        end
+        return h
     end
-    return h
 end
 
 upload(pfield) = GC.@preserve pfield check(ccall((:vpm_set_particles, libvpm), Cint,
@@ -220,15 +228,46 @@ end
 One rungekutta3 step of ReformulatedVPM with UJ = UJ_direct, entirely on the
 device (pfield.particles is uploaded once and downloaded once).
 """
+"""
+    sfs_settings(SFS) -> (sfs, force_positive, clip)
+
+The SFS models the device step runs: NoSFS, and DynamicSFS with the
+pseudo-3-level procedure (src/FLOWVPM_subfilterscale.jl:447-673), whose
+afterUJ half forces C >= 0 only as pseudo3level_positive_afterUJ
+(src/FLOWVPM.jl:202), no controls, and either no clipping or
+clipping_backscatter alone.  Any other model (ConstantSFS such as
+SFS_Cs_nobackscatter, sensor-function procedures, controls) raises.
+"""
+function sfs_settings(SFS)
+    SFS isa vpm.NoSFS && return (false, false, false)
+    SFS isa vpm.DynamicSFS ||
+        error("rungekutta3_hip runs NoSFS and the pseudo-3-level DynamicSFS only; got $(typeof(SFS))")
+    SFS.procedure_beforeUJ === vpm.dynamicprocedure_pseudo3level_beforeUJ ||
+        error("rungekutta3_hip: unsupported SFS procedure_beforeUJ $(SFS.procedure_beforeUJ)")
+    pa = SFS.procedure_afterUJ
+    force_positive = pa === vpm.pseudo3level_positive_afterUJ ? true :
+                     pa === vpm.dynamicprocedure_pseudo3level_afterUJ ? false :
+                     error("rungekutta3_hip: unsupported SFS procedure_afterUJ $pa")
+    isempty(SFS.controls) || error("rungekutta3_hip: SFS controls are not supported")
+    clip = if isempty(SFS.clippings)
+        false
+    elseif length(SFS.clippings) == 1 && SFS.clippings[1] === vpm.clipping_backscatter
+        true
+    else
+        error("rungekutta3_hip: only clippings=(clipping_backscatter,) is supported")
+    end
+    return (true, force_positive, clip)
+end
+
 function rungekutta3_hip(pfield::vpm.ParticleField, dt::Real; relax::Bool=false, precision=0, optargs...)
+    sfs, force_positive, clip = sfs_settings(pfield.SFS)
     upload(pfield)
-    sfs = pfield.SFS isa vpm.DynamicSFS
     visc = pfield.viscous isa vpm.CoreSpreading
     rl = pfield.relaxation
     method = rl.relax === vpm.relax_pedrizzetti ? 1 : rl.relax === vpm.relax_correctedpedrizzetti ? 2 : 0
     s = Ref(VpmStepConfig(config(pfield; precision), pfield.formulation.f, pfield.formulation.g,
             Tuple(pfield.Uinf(pfield.t)),
-            Int32(sfs), Int32(sfs), Int32(sfs && !isempty(pfield.SFS.clippings)), Int32(method),
+            Int32(sfs), Int32(force_positive), Int32(clip), Int32(method),
             sfs ? pfield.SFS.alpha : 0.0, sfs ? pfield.SFS.rlxf : 0.0,
             sfs ? pfield.SFS.minC : 0.0, sfs ? pfield.SFS.maxC : 1.0, rl.rlxf,
             Int32(visc), 0, visc ? pfield.viscous.nu : 0.0, visc ? pfield.viscous.sgm0 : 1.0,
@@ -257,19 +296,25 @@ end
 
 function ChainRulesCore.rrule(::typeof(uj_hip), pfield::vpm.ParticleField; precision=0)
     U, J = uj_hip(pfield; precision)
+    # the forward point: the pullback differentiates at these records even if
+    # pfield (host or device) changes between the forward and the pullback
+    n = pfield.np
+    P0 = pfield.particles[:, 1:n]
     # a cotangent that is structurally zero (a loss on U alone leaves Jbar as
     # ZeroTangent or nothing) becomes zeros; all-zero Jbar takes the library's
     # kernel without the Jbar terms
     dense(x, rows, n) = (x === nothing || x isa ChainRulesCore.AbstractZero) ? zeros(rows, n) :
                         Matrix{Float64}(ChainRulesCore.unthunk(x))
     function uj_hip_pullback((Ubar, Jbar))
-        n = pfield.np
         Ub = dense(Ubar, 3, n); Jb = dense(Jbar, 9, n)
         Xb = zeros(3, n); Gb = zeros(3, n); Sb = zeros(n)
         cfg = Ref(config(pfield; precision))
+        h = device_field(pfield)
+        GC.@preserve P0 check(ccall((:vpm_set_particles, libvpm), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int64),
+                                    h, P0, n))
         check(ccall((:vpm_uj_adjoint, libvpm), Cint,
                     (Ptr{Cvoid}, Ref{VpmConfig}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
-                    device_field(pfield), cfg, Ub, Jb, Xb, Gb, Sb))
+                    h, cfg, Ub, Jb, Xb, Gb, Sb))
         dparticles = zeros(size(pfield.particles))
         dparticles[1:3, 1:n] .= Xb; dparticles[4:6, 1:n] .= Gb; dparticles[7, 1:n] .= Sb
         return ChainRulesCore.NoTangent(), ChainRulesCore.Tangent{typeof(pfield)}(particles=dparticles)

@@ -159,6 +159,27 @@ int vpm_zeta_ranges(vpm_field f, const vpm_config* cfg,
                     const int64_t* tb, const int64_t* te,
                     const int64_t* sb, const int64_t* se, int64_t npairs);
 
+/* A direct list planned once and served to several list evaluations:
+ * UJ_fmm hands one direct_list to the near field (fmm.direct! per pair, with
+ * nearfield_device, src/FLOWVPM_UJ.jl:90-101) and then to Estr_fmm!
+ * (src/FLOWVPM_UJ.jl:125, subfilterscale_models.jl:157-188); zeta_fmm
+ * (viscous.jl:523-558) walks such a list too.  vpm_plan_ranges checks and
+ * plans the pairs (the arguments of vpm_p2p_ranges) for field f and keeps
+ * the plan on the device; vpm_p2p_plan, vpm_estr_plan and vpm_zeta_plan then
+ * run vpm_p2p_ranges, vpm_estr_ranges and vpm_zeta_ranges on it with no host
+ * planning and no list upload, with the same results bit for bit (FP64 runs
+ * its block kernels over the plan's items).  A plan is valid for its field
+ * while the field holds the np it was made for (VPM_EINVAL otherwise).
+ * Destroying the field releases a plan's device memory; the plan must still
+ * be passed to vpm_plan_destroy. */
+typedef struct vpm_plan_s* vpm_plan;
+int vpm_plan_ranges(vpm_field f, const int64_t* tb, const int64_t* te,
+                    const int64_t* sb, const int64_t* se, int64_t npairs, vpm_plan* out);
+int vpm_plan_destroy(vpm_plan p);
+int vpm_p2p_plan(vpm_field f, const vpm_config* cfg, vpm_plan p);
+int vpm_estr_plan(vpm_field f, const vpm_config* cfg, vpm_plan p);
+int vpm_zeta_plan(vpm_field f, const vpm_config* cfg, vpm_plan p);
+
 /* ---- terms built from J ------------------------------------------------- */
 /* Estr_direct! (src/FLOWVPM_subfilterscale_models.jl:16-92): SFS += sum over
  * non-static sources of zeta_sigma(r) (Gamma_s . grad)(U_t - U_s), for

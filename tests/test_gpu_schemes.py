@@ -241,6 +241,63 @@ def test_run_vpm_static_particles_and_relax_cadence_vs_oracle(vpm, orc, resident
     assert np.all(pf.particles[n:n + 3, 9:27] == 0.0)   # removed slots reset
 
 
+@pytest.mark.parametrize("resident", [True, False])
+def test_run_vpm_runtime_function_edits_field_vs_oracle(vpm, orc, resident):
+    """A runtime_function that adds particles, removes one and scales a
+    strength (the round jet's inflow does the first, examples/roundjet/
+    roundjet_simulation.jl:229) acts on the live field, and the next step
+    sees its changes on the device too.  The oracle replays the same
+    edits; F64 state to 1e-10."""
+    api = vpm.api
+    P = vpm.workloads.ring_cfg0()[::10].copy()          # 800 particles
+    n = P.shape[0]
+    rng = np.random.default_rng(41)
+    pf = api.ParticleField(n + 16, kernel="gaussianerf", formulation=api.rVPM, precision="f64")
+    api.add_particles(pf, P)
+    dt = 0.01
+    new = [vpm.fields.make_particles(P[rng.integers(0, n, 2), 0:3] + 0.03, np.full((2, 3), 2e-3 * (k + 1)),
+                                     np.full(2, float(P[0, 6]))) for k in range(3)]
+
+    def edit(F, k):
+        """the edits of step k on a record matrix F of m rows; returns m"""
+        m = F.shape[0]
+        if k == 1:
+            F = np.vstack([F, new[0]])
+        elif k == 2:
+            F = np.vstack([F, new[1]])
+            F[5] = F[-1]              # remove_particle(5): the last record moves in
+            F = F[:-1]
+            F[7, 3:6] *= 1.5
+        elif k == 3:
+            F = np.vstack([F, new[2]])
+            F[1, 3:6] *= 0.5
+        return F
+
+    def runtime(pfield, t, dt_):
+        k = pfield.nt
+        if k == 1:
+            api.add_particles(pfield, new[0])
+        elif k == 2:
+            api.add_particles(pfield, new[1])
+            api.remove_particle(pfield, 5)
+            pfield.particles[7, 3:6] *= 1.5
+        elif k == 3:
+            api.add_particles(pfield, new[2])
+            pfield.particles[1, 3:6] *= 0.5
+        return k == 4
+
+    api.run_vpm(pf, dt, 10, runtime_function=runtime, resident=resident)
+    assert pf.nt == 4 and pf.np == n + 3
+    s = orc.make_settings(kernel="gaussianerf", f=0.0, g=0.2)
+    R = P.copy()
+    for k in range(1, 5):
+        orc.rk3_step(R, s, dt)
+        R = np.ascontiguousarray(edit(R, k))
+    e = _state_err(vpm, pf.particles[:pf.np], R)
+    _say("run_vpm runtime edits resident=%s" % resident, e)
+    assert max(e.values()) < 1e-10, e
+
+
 def test_run_vpm_saves_and_restarts(vpm, tmp_path):
     """run_vpm with save_path on a device-resident field (FLOWVPM_utils.jl:
     124-129): each saved file holds the field as it was at that step, and a
