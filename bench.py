@@ -39,33 +39,66 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(P, kernel, target_s=12.0):
-    """FP64 oracle (reference-algorithm restatement) on a bounded sample of
-    targets against all sources, OpenMP over targets: a short calibration
-    run sizes one timed sample of about target_s seconds of CPU work."""
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     orc.lib()
-    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
-    n = P.shape[0]
-    nt = 64
-    while True:                                   # calibration: at least 0.5 s
-        Q = P.copy()
+    return orc
+
+
+def host_threads():
+    """Threads of the CPU legs: OMP_NUM_THREADS when it is set (the host
+    cores this job owns next to its GPU), else every CPU this process may run
+    on; nproc and os.cpu_count() are recorded beside it."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return (omp or aff), {"nproc": aff, "cpu_count": os.cpu_count(), "OMP_NUM_THREADS": omp or None}
+
+
+def time_cpu(run, n_total, target_s):
+    """Times run(nt) (FP64 oracle over targets [0, nt)) on a sample sized by
+    a short calibration to about target_s seconds; returns (targets/s, nt, s)."""
+    nt = 16
+    while True:                                   # calibration: at least 0.3 s
         t0 = time.perf_counter()
-        orc.direct(Q, kernel, t_range=(0, nt), nthreads=nthreads)
+        run(nt)
         dt = time.perf_counter() - t0
-        if dt > 0.5 or nt * 2 > n:
+        if dt > 0.3 or nt * 2 > n_total:
             break
-        nt = min(n, nt * 4)
-    nt = max(1, min(n, int(nt * target_s / max(dt, 1e-3))))
-    Q = P.copy()
+        nt = min(n_total, nt * 4)
+    nt = max(1, min(n_total, int(nt * target_s / max(dt, 1e-3))))
     t0 = time.perf_counter()
-    orc.direct(Q, kernel, t_range=(0, nt), nthreads=nthreads)
+    Q = run(nt)
     dt = time.perf_counter() - t0
-    rate = nt * n / dt
-    return {"value": rate, "unit": "pair-interactions/s", "cores": nthreads, "kind": "port",
-            "sample": "%d targets x %d sources of the workload, FP64 oracle (oracle/vpm_oracle.c), "
-                      "%.1f s" % (nt, n, dt)}, Q[:nt], nt
+    return nt / dt, nt, dt, Q
+
+
+def cpu_uj_rate(P, kernel, nthreads, target_s):
+    """FP64 oracle U+J (fmm.direct! restated, OpenMP over targets) on a
+    bounded sample of targets against all sources: pairs/s."""
+    orc = _oracle()
+    n = P.shape[0]
+
+    def run(nt):
+        Q = P.copy()
+        orc.direct(Q, kernel, t_range=(0, nt), nthreads=nthreads)
+        return Q
+
+    tps, nt, dt, Q = time_cpu(run, n, target_s)
+    return tps * n, nt, dt, Q
+
+
+def cpu_baseline(P, kernel, target_s=12.0):
+    """FP64 oracle (reference-algorithm restatement) on a bounded sample of
+    targets against all sources, OpenMP over targets on the host threads."""
+    nthreads, info = host_threads()
+    rate, nt, dt, Q = cpu_uj_rate(P, kernel, nthreads, target_s)
+    n = P.shape[0]
+    out = {"value": rate, "unit": "pair-interactions/s", "cores": nthreads, "kind": "port",
+           "sample": "%d targets x %d sources of the workload, FP64 oracle (oracle/vpm_oracle.c), "
+                     "%.1f s" % (nt, n, dt)}
+    out.update(info)
+    return out, Q[:nt], nt
 
 
 def copy_calibration(mib):
@@ -150,6 +183,25 @@ def leaf_list(vpm, P, kernel, allpairs_rate, leaf=64, reps=3):
         vpm._lib.check(L.vpm_zeta_ranges(f.handle, C.byref(cfg), *ptr, len(tb)))
     zwall = (time.perf_counter() - t0) / reps
     zms, zcnt = f.timing(4)
+    # the same list planned once (vpm_plan_ranges) and served to U/J, E_str
+    # and zeta, as UJ_fmm hands one direct_list to the near field and to
+    # Estr_fmm!: call times without the host plan and the list upload
+    t0 = time.perf_counter()
+    plan = f.plan_ranges(*arrs)
+    plan_ms = 1e3 * (time.perf_counter() - t0)
+    planned = {}
+    for name, fn, which in (("uj", lambda: plan.p2p(kernel, "f32c"), 0),
+                            ("estr", lambda: plan.estr(kernel, "f32c", True), 2),
+                            ("zeta", lambda: plan.zeta(kernel, "f32c"), 4)):
+        fn()
+        f.profile(True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        call = 1e3 * (time.perf_counter() - t0) / reps
+        kms, kc = f.timing(which)
+        planned[name] = {"call_ms": call, "kernel_ms": kms / max(kc, 1), "call_over_kernel": call / (kms / max(kc, 1))}
+    plan.close()
     f.close()
     out = {"N": n, "leaf": leaf, "list_pairs": len(tb), "particle_pairs": pairs, "sources_per_target": pairs / n,
            "call_ms": 1e3 * wall, "kernel_ms": ms / cnt, "pairs_per_s_kernel": pairs / (ms / cnt / 1e3),
@@ -157,12 +209,13 @@ def leaf_list(vpm, P, kernel, allpairs_rate, leaf=64, reps=3):
            "estr_call_ms": 1e3 * ewall, "estr_kernel_ms": ems / ecnt,
            "estr_pairs_per_s_kernel": pairs / (ems / ecnt / 1e3),
            "zeta_call_ms": 1e3 * zwall, "zeta_kernel_ms": zms / zcnt,
-           "zeta_pairs_per_s_kernel": pairs / (zms / zcnt / 1e3)}
+           "zeta_pairs_per_s_kernel": pairs / (zms / zcnt / 1e3),
+           "planned": dict(planned, plan_ms=plan_ms)}
     log("bench: leaf list %s" % out)
     return out
 
 
-def cfg1_cloud(vpm, n=65536, reps=5):
+def cfg1_cloud(vpm, n=65536, reps=5, cpu=True):
     """Random cloud N = 65,536 (X ~ U[0,1]^3, Gamma ~ N(0, 1e-3^2), sigma ~
     U[0.02, 0.04], seed 1): one U+J evaluation per kernel in FP32c and FP64 on
     the GPU, throughput and per-output error of FP32c against FP64."""
@@ -188,18 +241,30 @@ def cfg1_cloud(vpm, n=65536, reps=5):
             f.get_particles(Q, rows=(9, 24))
             rows[prec] = Q
         A, R = rows["f32c"], rows["f64"]
+        S32, S64 = vpm.metrics.stretching(A[:, 15:24], P[:, 3:6]), vpm.metrics.stretching(R[:, 15:24], P[:, 3:6])
+        sc = np.linalg.norm(R[:, 15:24], axis=1) * np.linalg.norm(P[:, 3:6], axis=1)
+        es = np.linalg.norm(S32 - S64, axis=1) / sc
         r["error_f32c_vs_f64"] = {
             "U": vpm.metrics.err_stats(A[:, 9:12], R[:, 9:12]),
             "J": vpm.metrics.err_stats(A[:, 15:24], R[:, 15:24]),
-            "dGamma_dt": vpm.metrics.err_stats(vpm.metrics.stretching(A[:, 15:24], P[:, 3:6]),
-                                               vpm.metrics.stretching(R[:, 15:24], P[:, 3:6]))}
+            "dGamma_dt": vpm.metrics.err_stats(S32, S64),
+            "dGamma_dt_vs_J_Gamma_scale": {"max": float(es.max()), "rms": float(np.sqrt(np.mean(es * es)))}}
+        if cpu:
+            nthreads, info = host_threads()
+            rate, nt, dt, Q = cpu_uj_rate(P, kernel, nthreads, 3.0)
+            r["cpu"] = {"pairs_per_s": rate, "threads": nthreads, "kind": "port",
+                        "sample": "%d targets x %d sources, FP64 oracle, %.1f s" % (nt, n, dt)}
+            r["cpu"].update(info)
+            r["gpu_f32c_over_cpu"] = r["f32c_pairs_per_s"] / rate
+            r["error_f64_vs_cpu_oracle_sample"] = {"U": vpm.metrics.err_stats(R[:nt, 9:12], Q[:nt, 9:12]),
+                                                   "J": vpm.metrics.err_stats(R[:nt, 15:24], Q[:nt, 15:24])}
         out[kernel] = r
         log("bench: cfg1 %s %s" % (kernel, r))
     f.close()
     return out
 
 
-def cfg2_rollout(vpm, nsteps=10, dt=0.01):
+def cfg2_rollout(vpm, nsteps=10, dt=0.01, cpu=True):
     """Two leapfrogging rings, N = 262,144: nsteps rk3 steps of rVPM with the
     dynamic SFS model (SFS_Cd_twolevel_nobackscatter), core spreading
     (nu = 1e-4) and Pedrizzetti relaxation every step; FP32c pair passes
@@ -226,7 +291,152 @@ def cfg2_rollout(vpm, nsteps=10, dt=0.01):
                                 for name, sl in (("X", slice(0, 3)), ("Gamma", slice(3, 6)),
                                                  ("sigma", slice(6, 7)))}
     log("bench: cfg2 errors %s" % res["error_f32c_vs_f64"])
+    if cpu:
+        # The reference's step with this model evaluates U+J 5 times (3
+        # stages, the test filter of stage 1, the relaxation's UJ) and E_str
+        # 4 times (rungekutta3, timeintegration.jl:388-461; pseudo3level
+        # before/after UJ, subfilterscale.jl:447-673): 50 U+J and 40 E_str
+        # all-pairs passes over 10 steps.  One pass of each is timed on a
+        # target sample of the initial state and the rollout extrapolated.
+        orc = _oracle()
+        nthreads, info = host_threads()
+        uj_rate, nt, dt, _ = cpu_uj_rate(P, "gaussianerf", nthreads, 4.0)
+        Q = P.copy()
+        g = api.Field(n, 0)
+        g.set_particles(Q)
+        g.uj_direct("gaussianerf", "f64")
+        g.get_particles(Q)
+        g.close()
+
+        def run(m):
+            Z = Q.copy()
+            Z[:, 39:42] = 0.0
+            orc.estr_direct(Z, "gaussianerf", True, nthreads=nthreads, t_range=(0, m))
+            return Z
+
+        etps, ent, edt, _ = time_cpu(run, n, 4.0)
+        e_rate = etps * n
+        pass_pairs = float(n) * n
+        est = nsteps * (5 * pass_pairs / uj_rate + 4 * pass_pairs / e_rate)
+        res["cpu"] = {"uj_pairs_per_s": uj_rate, "estr_pairs_per_s": e_rate, "threads": nthreads, "kind": "port",
+                      "uj_sample": "%d targets x %d sources, %.1f s" % (nt, n, dt),
+                      "estr_sample": "%d targets x %d sources, %.1f s" % (ent, n, edt),
+                      "rollout_s_extrapolated": est,
+                      "extrapolation": "%d steps x (5 U+J + 4 E_str all-pairs passes) at the sampled rates; "
+                                       "the O(N) per-particle work is left out" % nsteps}
+        res["cpu"].update(info)
+        res["gpu_f32c_over_cpu_extrapolated"] = est / res["f32c_s"]
+        log("bench: cfg2 CPU %s" % res["cpu"])
     return res
+
+
+def gpu_uj_time(f, kernel, prec, reps):
+    """Call and kernel time (ms) of vpm_uj_direct on a resident field."""
+    f.uj_direct(kernel, prec)                     # warmup (sort, staging)
+    f.profile(True)
+    f.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        f.uj_direct(kernel, prec)
+    f.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    kms, cnt = f.timing(0 if prec == "f32c" else 1)
+    f.profile(False)
+    return 1e3 * wall, kms / max(cnt, 1)
+
+
+def cfg0_ring(vpm, kernel="gaussianerf"):
+    """Config 0: one vortex ring, N = 8,000 (Nphi = 200 x 40), one UJ_direct
+    U+J evaluation (the reference's CPU-runnable case): GPU F32C and FP64,
+    and the FP64 oracle over every target on the host threads."""
+    api = vpm.api
+    orc = _oracle()
+    nthreads, info = host_threads()
+    P = vpm.workloads.ring_cfg0()
+    n = P.shape[0]
+    pairs = float(n) * n
+    f = api.Field(n, 0)
+    f.set_particles(P)
+    out = {"N": n, "kernel": kernel}
+    rows = {}
+    for prec, reps in (("f32c", 50), ("f64", 10)):
+        call_ms, kern_ms = gpu_uj_time(f, kernel, prec, reps)
+        Q = np.zeros_like(P)
+        f.get_particles(Q, rows=(9, 24))
+        rows[prec] = Q
+        out[prec] = {"call_ms": call_ms, "kernel_ms": kern_ms, "pairs_per_s_call": pairs / (call_ms / 1e3),
+                     "pairs_per_s_kernel": pairs / (kern_ms / 1e3)}
+    f.close()
+    best = None
+    for _ in range(3):
+        R = P.copy()
+        t0 = time.perf_counter()
+        orc.direct(R, kernel, nthreads=nthreads)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    out["cpu"] = {"s": best, "pairs_per_s": pairs / best, "threads": nthreads, "kind": "port",
+                  "sample": "every target, best of 3"}
+    out["cpu"].update(info)
+    for prec in ("f32c", "f64"):
+        A = rows[prec]
+        out[prec]["error_vs_cpu_oracle"] = {"U": vpm.metrics.err_stats(A[:, 9:12], R[:, 9:12]),
+                                            "J": vpm.metrics.err_stats(A[:, 15:24], R[:, 15:24])}
+    out["gpu_f32c_over_cpu"] = out["f32c"]["pairs_per_s_call"] / out["cpu"]["pairs_per_s"]
+    log("bench: cfg0 %s" % out)
+    return out
+
+
+def cfg3_adjoint(vpm, kernel="gaussianerf", cpu=True):
+    """Config 3: random cloud N = 32,768 (seed 33), weights w ~ N(0,1)^3, W ~
+    N(0,1)^(3x3) (seed 34), L = sum w.U + sum W:J.  F32C forward and adjoint
+    kernel times on the GPU and their ratio; the FP64 CPU gradient (the
+    oracle's adjoint, which equals the reference's own UJS_pullback,
+    rrules.jl:183-326, to FP64 rounding) timed on the host threads; the GPU
+    gradient's error against it, per particle, relative to the summation
+    scale of each output."""
+    api = vpm.api
+    orc = _oracle()
+    nthreads, info = host_threads()
+    n = 32768
+    P = vpm.workloads.random_cloud(n, seed=33)
+    w, W = vpm.workloads.adjoint_weights(n, seed=34)
+    f = api.Field(n, 0)
+    f.set_particles(P)
+    fwd_call, fwd_kern = gpu_uj_time(f, kernel, "f32c", 20)
+    f.uj_adjoint(w, W, kernel, "f32c")            # warmup
+    f.profile(True)
+    f.synchronize()
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        got = f.uj_adjoint(w, W, kernel, "f32c")
+    adj_call = 1e3 * (time.perf_counter() - t0) / reps
+    ams, acnt = f.timing(3)
+    f.profile(False)
+    f.close()
+    out = {"N": n, "kernel": kernel, "loss": "sum w.U + sum W:J", "forward_kernel_ms": fwd_kern,
+           "forward_call_ms": fwd_call, "adjoint_kernel_ms": ams / max(acnt, 1), "adjoint_call_ms": adj_call,
+           "ratio_kernel": ams / max(acnt, 1) / fwd_kern, "ratio_call": adj_call / fwd_call}
+    if not cpu:
+        log("bench: cfg3 %s" % out)
+        return out
+    t0 = time.perf_counter()
+    ref = orc.uj_adjoint(P, kernel, w, W, nthreads=nthreads)
+    cpu_s = time.perf_counter() - t0
+    scale = orc.uj_adjoint_scale(P, kernel, w, W, nthreads=nthreads)
+    errs = {}
+    for name, a, r, m in zip(("X", "Gamma", "sigma"), got, ref, scale):
+        a, r, m = (np.asarray(x).reshape(n, -1) for x in (a, r, m))
+        den = np.linalg.norm(m, axis=1)
+        e = np.linalg.norm(a - r, axis=1) / np.maximum(den, 1e-6 * den.max())
+        errs[name] = {"max": float(e.max()), "rms": float(np.sqrt(np.mean(e * e)))}
+    out.update({"cpu_gradient_s": cpu_s, "cpu_threads": nthreads,
+                "cpu_kind": "port (oracle/vpm_oracle.c vpmo_uj_adjoint)",
+                "gpu_adjoint_over_cpu": cpu_s * 1e3 / adj_call,
+                "error_vs_cpu_gradient_per_summation_scale": errs})
+    out.update({"cpu_" + k: v for k, v in info.items()})
+    log("bench: cfg3 %s" % out)
+    return out
 
 
 def core_spreading_reset(vpm, P, kernel, itmax=15, tol=1e-3):
@@ -284,6 +494,9 @@ def main():
                     help="skip the N=65,536 random-cloud accuracy/throughput report")
     ap.add_argument("--no-cfg2", action="store_true",
                     help="skip the 10-step leapfrog-rings rollout, FP32c vs FP64")
+    ap.add_argument("--no-cfg0", action="store_true", help="skip the N=8,000 ring (GPU and CPU)")
+    ap.add_argument("--no-cfg3", action="store_true",
+                    help="skip the N=32,768 adjoint/forward ratio and the CPU gradient")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -400,11 +613,17 @@ def main():
     if not args.no_reset:
         res["core_spreading_reset"] = core_spreading_reset(vpm, P, kernel)
 
+    if not args.no_cfg0:
+        res["cfg0_ring"] = cfg0_ring(vpm)
+
     if not args.no_cfg1:
-        res["cfg1_cloud"] = cfg1_cloud(vpm)
+        res["cfg1_cloud"] = cfg1_cloud(vpm, cpu=not args.no_cpu)
 
     if not args.no_cfg2:
-        res["cfg2_rollout"] = cfg2_rollout(vpm)
+        res["cfg2_rollout"] = cfg2_rollout(vpm, cpu=not args.no_cpu)
+
+    if not args.no_cfg3:
+        res["cfg3_adjoint"] = cfg3_adjoint(vpm, cpu=not args.no_cpu)
 
     if not args.no_cpu:
         log("bench: CPU baseline")

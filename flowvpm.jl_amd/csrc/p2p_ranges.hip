@@ -18,6 +18,13 @@
 
 namespace vpm {
 
+// Component-wise select of two float4 values (kept in registers: a ?: on
+// the vector type itself can go through private memory).
+__device__ __forceinline__ float4 sel4(bool c, float4 a, float4 b)
+{
+    return make_float4(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z, c ? a.w : b.w);
+}
+
 template <int K>
 __global__ __launch_bounds__(kBlock) void ranges_f32c_kernel(const RangeItem* __restrict__ items,
                                                              const int2* __restrict__ lsrc,
@@ -66,7 +73,7 @@ __global__ __launch_bounds__(kBlock) void ranges_f32c_kernel(const RangeItem* __
             float acc[kAccF32];
 #pragma unroll
             for (int c = 0; c < kAccF32; ++c) acc[c] = 0.f;
-            tile_f32c<K>(n, lpos, lgs, llo, tx, ty, tz, tlx, tly, tlz, box, acc);
+            tile_f32c<K>(n, lpos, lgs, llo, tx, ty, tz, tlx, tly, tlz, box, acc, sum, err);
             fold_twosum(acc, sum, err);
         }
     }
@@ -155,7 +162,7 @@ __global__ __launch_bounds__(kBlock) void ranges_wave_f32c_kernel(const RangeIte
                 float acc[kAccF32];
 #pragma unroll
                 for (int c = 0; c < kAccF32; ++c) acc[c] = 0.f;
-                tile_f32c<K, 1>(n, lpos, lgs, llo, tx, ty, tz, tlx, tly, tlz, box, acc);
+                tile_f32c<K, 1>(n, lpos, lgs, llo, tx, ty, tz, tlx, tly, tlz, box, acc, sum, err);
                 fold_twosum(acc, sum, err);
             }
         }
@@ -187,16 +194,24 @@ __global__ __launch_bounds__(kBlock) void ranges_wave_f32c_kernel(const RangeIte
         }
         if (filled == 0) break;                 // only empty ranges were left
         __builtin_amdgcn_wave_barrier();
-        if (lane < filled) {
+        {
+            // slots past `filled` hold a zero-strength source far away, so
+            // every lane evaluates its step's pair unconditionally: it adds
+            // exact zeros (F' = H'' = 0 at 1/sigma = 0, and d x 0 = 0), and
+            // no lane-dependent branch splits the accumulator updates.  Those
+            // lanes load source j = 0 (a valid index) and select the padding.
+            const bool real = lane < filled;
             float4 p, q;
             split_rel(pos[j], ox, p.x, q.x);
             split_rel(pos[ld + j], oy, p.y, q.y);
             split_rel(pos[2 * ld + j], oz, p.z, q.z);
-            p.w = cut[j];
-            q.w = 0.f;
+            const float c = cut[j];
+            const float4 gr = gs[j];
+            p = sel4(real, make_float4(p.x, p.y, p.z, c), make_float4(1e15f, 1e15f, 1e15f, 0.f));
+            q = sel4(real, make_float4(q.x, q.y, q.z, 0.f), make_float4(0.f, 0.f, 0.f, 0.f));
             lpos[lane] = p;
             llo[lane] = q;
-            lgs[lane] = gs[j];
+            lgs[lane] = sel4(real, gr, make_float4(0.f, 0.f, 0.f, 0.f));
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -212,14 +227,13 @@ __global__ __launch_bounds__(kBlock) void ranges_wave_f32c_kernel(const RangeIte
                 nr = fmaf(ez, ez, fmaf(ey, ey, ex * ex)) < a.w * 1.0001f;
             }
             const uint64_t nearm = __ballot(nr);
+            // k R < filled <= 64 and 64 is a multiple of R, so sidx < 64
             for (int k = 0; k * R < filled; ++k) {
                 const int sidx = k * R + g;
-                const bool ok = sidx < filled;
-                if ((nearm >> (k * R)) & ((1ull << R) - 1)) {
-                    if (ok) near_pair<K>(tx, ty, tz, tlx, tly, tlz, lpos[sidx], lgs[sidx], llo[sidx], acc);
-                } else {
-                    if (ok) far_pair<K>(tx, ty, tz, lpos[sidx], lgs[sidx], acc);
-                }
+                if ((nearm >> (k * R)) & ((1ull << R) - 1))
+                    near_pair<K>(tx, ty, tz, tlx, tly, tlz, lpos[sidx], lgs[sidx], llo[sidx], acc);
+                else
+                    far_pair<K>(tx, ty, tz, lpos[sidx], lgs[sidx], acc);
             }
         }
         since += (filled + R - 1) / R;
@@ -625,9 +639,12 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
         }
         if (filled == 0) break;
         __builtin_amdgcn_wave_barrier();
-        if (lane < filled) {
+        // slots past `filled`: a zero-strength source far away with a zero
+        // cut (it never starts a step, and adds exact zeros to one); those
+        // lanes read source j = 0 (a valid index) and select the padding
+        float4 p, lo, q, gq;
+        {
             const double* sr = P + j * NF;
-            float4 p, lo;
             split_rel(sr[RX], ox, p.x, lo.x);
             split_rel(sr[RX + 1], oy, p.y, lo.y);
             split_rel(sr[RX + 2], oz, p.z, lo.z);
@@ -643,7 +660,6 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
             double Jp[9];
 #pragma unroll
             for (int c = 0; c < 9; ++c) Jp[c] = sr[RJ + c] - Jr[c];
-            float4 q;
             if (transposed) {
                 q = make_float4((float)(Jp[0] * G0 + Jp[1] * G1 + Jp[2] * G2),
                                 (float)(Jp[3] * G0 + Jp[4] * G1 + Jp[5] * G2),
@@ -653,26 +669,28 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
                                 (float)(Jp[1] * G0 + Jp[4] * G1 + Jp[7] * G2),
                                 (float)(Jp[2] * G0 + Jp[5] * G1 + Jp[8] * G2), (float)cut);
             }
-            lp[lane] = p;
-            ll[lane] = lo;
-            lq[lane] = q;
-            lg[lane] = make_float4((float)G0, (float)G1, (float)G2, (float)b);
+            gq = make_float4((float)G0, (float)G1, (float)G2, (float)b);
         }
+        const bool real = lane < filled;
+        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        lp[lane] = sel4(real, p, make_float4(1e15f, 1e15f, 1e15f, 0.f));
+        ll[lane] = sel4(real, lo, z4);
+        lq[lane] = sel4(real, q, z4);
+        lg[lane] = sel4(real, gq, z4);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // k R < filled <= 64 and 64 is a multiple of R, so sidx < 64
         for (int k = 0; k * R < filled; ++k) {
             const int sidx = k * R + g;
-            const bool ok = sidx < filled;
-            const int sl = ok ? sidx : 0;
-            const float4 a = lp[sl];
-            const float4 c = lq[sl];
+            const float4 a = lp[sidx];
+            const float4 c = lq[sidx];
             float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
             float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            if (__any(ok && r2 < c.w)) {
-                if (ok) {
-                    const float4 lo = ll[sl];
-                    const float4 gg = lg[sl];
+            if (__any(r2 < c.w)) {
+                {
+                    const float4 lo = ll[sidx];
+                    const float4 gg = lg[sidx];
                     dx += tlx - lo.x;
                     dy += tly - lo.y;
                     dz += tlz - lo.z;
@@ -696,7 +714,8 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
                     acc[5] = fmaf(z, c.z, acc[5]);
                 }
             }
-            if (++since == kEstrFold) {
+            since = __builtin_amdgcn_readfirstlane(since + 1);   // wave-uniform
+            if (since == kEstrFold) {
 #pragma unroll
                 for (int c = 0; c < 6; ++c) { A[c] += (double)acc[c]; acc[c] = 0.f; }
                 since = 0;
@@ -1010,9 +1029,13 @@ __global__ __launch_bounds__(kBlock) void zeta_ranges_wave_f32c_kernel(const Ran
         }
         if (filled == 0) break;
         __builtin_amdgcn_wave_barrier();
-        if (lane < filled) {
-            const double* sr = P + j * NF;
+        {
+            // slots past `filled`: a zero-strength source far away with a
+            // zero cut (it never starts a step, and adds exact zeros to one);
+            // those lanes read source j = 0 (a valid index) and select the
+            // padding
             float4 p, lo;
+            const double* sr = P + j * NF;
             split_rel(sr[RX], ox, p.x, lo.x);
             split_rel(sr[RX + 1], oy, p.y, lo.y);
             split_rel(sr[RX + 2], oz, p.z, lo.z);
@@ -1020,24 +1043,26 @@ __global__ __launch_bounds__(kBlock) void zeta_ranges_wave_f32c_kernel(const Ran
             zeta_params<K>(sr[RSIG], a, b);
             p.w = zeta_cut<K>(sr[RSIG]);
             lo.w = (float)a;
-            lp[lane] = p;
-            ll[lane] = lo;
-            lg[lane] = make_float4((float)sr[RG], (float)sr[RG + 1], (float)sr[RG + 2], (float)b);
+            const float4 gq = make_float4((float)sr[RG], (float)sr[RG + 1], (float)sr[RG + 2], (float)b);
+            const bool real = lane < filled;
+            const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+            lp[lane] = sel4(real, p, make_float4(1e15f, 1e15f, 1e15f, 0.f));
+            ll[lane] = sel4(real, lo, z4);
+            lg[lane] = sel4(real, gq, z4);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // k R < filled <= 64 and 64 is a multiple of R, so sidx < 64
         for (int k = 0; k * R < filled; ++k) {
             const int sidx = k * R + g;
-            const bool ok = sidx < filled;
-            const int sl = ok ? sidx : 0;
-            const float4 a = lp[sl];
+            const float4 a = lp[sidx];
             float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
             float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            if (__any(ok && r2 < a.w)) {
-                if (ok) {
-                    const float4 lo = ll[sl];
-                    const float4 gg = lg[sl];
+            if (__any(r2 < a.w)) {
+                {
+                    const float4 lo = ll[sidx];
+                    const float4 gg = lg[sidx];
                     dx += tlx - lo.x;
                     dy += tly - lo.y;
                     dz += tlz - lo.z;
@@ -1048,7 +1073,8 @@ __global__ __launch_bounds__(kBlock) void zeta_ranges_wave_f32c_kernel(const Ran
                     acc[2] = fmaf(gg.z, z, acc[2]);
                 }
             }
-            if (++since == kZetaFold) {
+            since = __builtin_amdgcn_readfirstlane(since + 1);   // wave-uniform
+            if (since == kZetaFold) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) { A[c] += (double)acc[c]; acc[c] = 0.f; }
                 since = 0;

@@ -95,7 +95,8 @@ __global__ __launch_bounds__(kBlock) void uj_f32c_kernel(
         float acc[kAccF32];
 #pragma unroll
         for (int c = 0; c < kAccF32; ++c) acc[c] = 0.f;
-        tile_f32c<K>((int)min((int64_t)kTile, se - base), lpos, lgs, llo, tx, ty, tz, tlx, tly, tlz, box, acc);
+        tile_f32c<K>((int)min((int64_t)kTile, se - base), lpos, lgs, llo, tx, ty, tz, tlx, tly, tlz, box, acc, sum,
+                     err);
         fold_twosum(acc, sum, err);
     }
     if (active) {

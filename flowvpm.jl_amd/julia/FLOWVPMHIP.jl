@@ -107,7 +107,11 @@ end
 
 FastMultipole leaf-pair P2P: for each k, the particles in source_ranges[k]
 act on the particles in target_ranges[k] (1-based UnitRanges), accumulating
-U and J as fmm.direct! + buffer_to_target_system! do.
+U and J as fmm.direct! + buffer_to_target_system! do.  The ranges are
+positions in `pfield.particles` as it is at the call: a tree's
+`bodies_index` ranges index the tree's sorted order, so on a field that is
+not stored in that order use `plan_direct_list` (which maps them through
+`tree.sort_index_list`, as Estr_fmm! does) instead.
 """
 function direct_hip!(pfield::vpm.ParticleField, target_ranges, source_ranges; precision=0)
     n = length(target_ranges)
@@ -125,9 +129,11 @@ end
 """
     estr_hip!(pfield, target_ranges, source_ranges; precision=0)
 
-Estr_fmm! over a FastMultipole direct list (src/FLOWVPM_subfilterscale_models.jl:157-188):
-E_str of every (target range, source range) pair onto SFS, with no reset and
-no static skip.
+The pair loop of Estr_fmm! (src/FLOWVPM_subfilterscale_models.jl:157-188)
+over explicit ranges: E_str of every (target range, source range) pair onto
+SFS, with no reset and no static skip.  As for `direct_hip!`, the ranges are
+positions in `pfield.particles`; Estr_fmm! itself takes a tree and maps its
+branches through `sort_index_list` -- that is `estr_fmm_hip!`.
 """
 function estr_hip!(pfield::vpm.ParticleField, target_ranges, source_ranges; precision=0)
     n = length(target_ranges)
@@ -139,6 +145,143 @@ function estr_hip!(pfield::vpm.ParticleField, target_ranges, source_ranges; prec
                 (Ptr{Cvoid}, Ref{VpmConfig}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Int64),
                 device_field(pfield), cfg, tb, te, sb, se, n))
     download(pfield, Int32(39), Int32(42))
+    return nothing
+end
+
+"""
+    FastMultipole.nearfield_device!(target, target_indices, switch, source, source_indices)
+
+FastMultipole's device near-field hook: `fmm.fmm!` calls it instead of its
+CPU `direct!` loop when `nearfield_device=true`, which UJ_fmm sets for
+`useGPU > 0` (src/FLOWVPM_UJ.jl:97); the call shape is the reference's own
+(src/FLOWVPM_gpu.jl:643).  `target_indices[k]` and `source_indices[k]` are
+the k-th leaf pair (UnitRanges of particles); all pairs go to the device in
+one `vpm_p2p_ranges` call, which accumulates U and J as `fmm.direct!` +
+`buffer_to_target_system!` do (src/FLOWVPM_fmm.jl:102-176).  The ranges are
+taken as positions in `target.particles`, the order the field is in when
+FastMultipole calls the hook; FastMultipole 2.x's own indexing contract for
+this hook is external to FLOWVPM and not restated here.  Target and source
+must be the same field (UJ_fmm's case).
+"""
+function vpm.fmm.nearfield_device!(target::vpm.ParticleField, target_indices, derivatives_switch,
+                                   source::vpm.ParticleField, source_indices)
+    target === source || error("nearfield_device!: libflowvpm_hip serves a field acting on itself")
+    length(target_indices) == length(source_indices) ||
+        error("nearfield_device!: $(length(target_indices)) target and $(length(source_indices)) source ranges")
+    isempty(target_indices) && return nothing
+    direct_hip!(target, target_indices, source_indices)
+    return nothing
+end
+
+# (target, source) bodies_index ranges of every direct_list entry as 0-based
+# half-open positions in the tree's sorted order
+function list_ranges(target_tree, source_tree, direct_list)
+    m = length(direct_list)
+    tb = Vector{Int64}(undef, m); te = similar(tb); sb = similar(tb); se = similar(tb)
+    for (k, (i_target, i_source)) in enumerate(direct_list)
+        t = target_tree.branches[i_target].bodies_index[1]
+        s = source_tree.branches[i_source].bodies_index[1]
+        tb[k] = first(t) - 1; te[k] = last(t); sb[k] = first(s) - 1; se[k] = last(s)
+    end
+    return tb, te, sb, se
+end
+
+"""
+    DirectListPlan
+
+One FastMultipole direct list of a field, planned once on the device
+(`vpm_plan_ranges`) and served to the near-field U/J (`nearfield_plan!`) and
+then to E_str (`estr_plan!`), as UJ_fmm hands one `direct_list` to the near
+field and then to `Estr_fmm!` (src/FLOWVPM_UJ.jl:90-125).  The particles are
+taken in tree order (`tree.sort_index_list[1]`) into a device field of the
+plan's own, so the tree's `bodies_index` ranges are used as they are, and
+results are scattered back to `pfield.particles` in storage order.
+"""
+mutable struct DirectListPlan
+    field::Ptr{Cvoid}
+    plan::Ptr{Cvoid}
+    perm::Vector{Int}
+    n::Int
+end
+
+function plan_direct_list(pfield::vpm.ParticleField, tree, direct_list)
+    n = pfield.np
+    perm = collect(tree.sort_index_list[1][1:n])
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:vpm_create, libvpm), Cint, (Cint, Int64, Ref{Ptr{Cvoid}}),
+                pfield.useGPU > 0 ? pfield.useGPU - 1 : 0, max(n, 1), r))
+    p = DirectListPlan(r[], C_NULL, perm, n)
+    finalizer(p) do q
+        q.plan != C_NULL && ccall((:vpm_plan_destroy, libvpm), Cint, (Ptr{Cvoid},), q.plan)
+        q.field != C_NULL && ccall((:vpm_destroy, libvpm), Cint, (Ptr{Cvoid},), q.field)
+    end
+    Psorted = pfield.particles[:, perm]
+    check(ccall((:vpm_set_particles, libvpm), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int64), p.field, Psorted, n))
+    tb, te, sb, se = list_ranges(tree, tree, direct_list)
+    pr = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:vpm_plan_ranges, libvpm), Cint,
+                (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Int64, Ref{Ptr{Cvoid}}),
+                p.field, tb, te, sb, se, length(tb), pr))
+    p.plan = pr[]
+    return p
+end
+
+# upload pfield in the plan's tree order, run `entry` on the plan, scatter
+# rows [row0, row1) (0-based, half-open) back
+function run_plan!(pfield, p::DirectListPlan, entry::Symbol, row0, row1; precision=0)
+    pfield.np == p.n || error("the direct list was planned for $(p.n) particles, the field holds $(pfield.np)")
+    Psorted = pfield.particles[:, p.perm]
+    check(ccall((:vpm_set_particles, libvpm), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int64), p.field, Psorted, p.n))
+    cfg = Ref(config(pfield; precision))
+    if entry === :p2p
+        check(ccall((:vpm_p2p_plan, libvpm), Cint, (Ptr{Cvoid}, Ref{VpmConfig}, Ptr{Cvoid}), p.field, cfg, p.plan))
+    else
+        check(ccall((:vpm_estr_plan, libvpm), Cint, (Ptr{Cvoid}, Ref{VpmConfig}, Ptr{Cvoid}), p.field, cfg, p.plan))
+    end
+    check(ccall((:vpm_get_particles, libvpm), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int64, Int32, Int32),
+                p.field, Psorted, p.n, Int32(row0), Int32(row1)))
+    pfield.particles[row0+1:row1, p.perm] .= Psorted[row0+1:row1, :]
+    return nothing
+end
+
+"""
+    nearfield_plan!(pfield, plan; precision=0)
+
+The near field of UJ_fmm over the planned direct list: U and J of every
+leaf pair accumulated onto `pfield.particles` (fmm.direct! per pair).
+"""
+nearfield_plan!(pfield::vpm.ParticleField, p::DirectListPlan; precision=0) =
+    run_plan!(pfield, p, :p2p, 9, 24; precision)
+
+"""
+    estr_plan!(pfield, plan; precision=0)
+
+Estr_fmm! over the planned direct list (src/FLOWVPM_subfilterscale_models.jl:
+94-188): E_str of every leaf pair onto SFS, using the field's complete J
+(near and far field), with no reset and no static skip.
+"""
+estr_plan!(pfield::vpm.ParticleField, p::DirectListPlan; precision=0) =
+    run_plan!(pfield, p, :estr, 39, 42; precision)
+
+"""
+    estr_fmm_hip!(target_pfield, source_pfield, target_tree, source_tree, direct_list; precision=0)
+
+Estr_fmm! (src/FLOWVPM_subfilterscale_models.jl:94-188) on the device, with
+the reference's signature: the branches' `bodies_index` ranges are mapped
+through `sort_index_list`, as the reference's loop does.  One field acting
+on itself (UJ_fmm's call, src/FLOWVPM_UJ.jl:125).
+"""
+function estr_fmm_hip!(target_pfield::vpm.ParticleField, source_pfield::vpm.ParticleField, target_tree,
+                       source_tree, direct_list; precision=0)
+    (target_pfield === source_pfield && target_tree === source_tree) ||
+        error("estr_fmm_hip!: one field and one tree (UJ_fmm's call)")
+    isempty(direct_list) && return nothing
+    p = plan_direct_list(target_pfield, target_tree, direct_list)
+    try
+        estr_plan!(target_pfield, p; precision)
+    finally
+        finalize(p)
+    end
     return nothing
 end
 

@@ -67,6 +67,7 @@ def lib():
         L.vpmo_reset_particles_sfs.argtypes = [_dp, C.c_int64]
         L.vpmo_uj_direct.argtypes = [_dp, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         L.vpmo_estr_direct.argtypes = [_dp, C.c_int64, C.c_int, C.c_int, C.c_int]
+        L.vpmo_estr_direct_targets.argtypes = [_dp, C.c_int64, C.c_int64, C.c_int64, C.c_int, C.c_int, C.c_int]
         L.vpmo_update_particle_states.argtypes = [_dp, C.c_int64, C.c_double, C.c_double, C.c_double,
                                                   _dp, C.c_double, C.c_double, C.c_double, C.c_int]
         L.vpmo_relax.argtypes = [_dp, C.c_int64, C.c_double, C.c_int]
@@ -74,6 +75,8 @@ def lib():
                                                           C.c_double, C.c_double, C.c_double, C.c_int]
         L.vpmo_uj_adjoint_scale.argtypes = [_dp, C.c_int64, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int]
         L.vpmo_estr_ranges.argtypes = [_dp, C.c_int64, C.c_int, C.c_int, _ip, _ip, _ip, _ip, C.c_int64]
+        L.vpmo_estr_scale.argtypes = [_dp, C.c_int64, C.c_int, C.c_int, _dp, C.c_int]
+        L.vpmo_estr_scale_ranges.argtypes = [_dp, C.c_int64, C.c_int, C.c_int, _ip, _ip, _ip, _ip, C.c_int64, _dp]
         L.vpmo_core_spreading_reset.restype = C.c_int
         L.vpmo_core_spreading_reset.argtypes = [_dp, C.c_int64, C.c_int, C.c_double, C.c_int, C.c_double,
                                                 C.c_int, C.c_int, C.POINTER(C.c_int), _dp]
@@ -175,6 +178,22 @@ def estr_ranges(P, kernel, tb, te, sb, se, transposed=True):
     lib().vpmo_estr_ranges(_check(P), NFIELDS, _kid(kernel), int(transposed), *a, len(a[0]))
 
 
+def estr_scale(P, kernel, transposed=True, nthreads=None):
+    """Summation scale of E_str per particle, s_i = sum zeta |(J_i - J_j)^T
+    Gamma_j| over Estr_direct!'s pairs (static skips included)."""
+    s = np.zeros(P.shape[0])
+    lib().vpmo_estr_scale(_check(P), P.shape[0], _kid(kernel), int(transposed), s, nthreads or nthreads_default())
+    return s
+
+
+def estr_scale_ranges(P, kernel, tb, te, sb, se, transposed=True):
+    """The E_str summation scale over the pairs of a direct list."""
+    a = [np.ascontiguousarray(x, dtype=np.int64) for x in (tb, te, sb, se)]
+    s = np.zeros(P.shape[0])
+    lib().vpmo_estr_scale_ranges(_check(P), NFIELDS, _kid(kernel), int(transposed), *a, len(a[0]), s)
+    return s
+
+
 def core_spreading_reset(P, kernel, sgm0, itmax=15, tol=1e-3, iterror=True, nthreads=None):
     """Core-spreading reset + RBF conjugate gradient, in place; returns
     (status, iterations, residuals)."""
@@ -196,9 +215,13 @@ def zeta_ranges(P, kernel, tb, te, sb, se):
     lib().vpmo_zeta_ranges(_check(P), NFIELDS, _kid(kernel), *a, len(a[0]))
 
 
-def estr_direct(P, kernel, transposed=True, np_=None, nthreads=None):
-    lib().vpmo_estr_direct(_check(P), np_ if np_ is not None else P.shape[0], _kid(kernel),
-                           int(transposed), nthreads or nthreads_default())
+def estr_direct(P, kernel, transposed=True, np_=None, nthreads=None, t_range=None):
+    n = np_ if np_ is not None else P.shape[0]
+    if t_range is None:
+        lib().vpmo_estr_direct(_check(P), n, _kid(kernel), int(transposed), nthreads or nthreads_default())
+    else:
+        lib().vpmo_estr_direct_targets(_check(P), n, int(t_range[0]), int(t_range[1]), _kid(kernel),
+                                       int(transposed), nthreads or nthreads_default())
 
 
 # ---- per-particle schemes ----

@@ -318,11 +318,17 @@ static inline void estr_pair(int kernel, int transposed, double* tp, const doubl
 
 void vpmo_estr_direct(double* P, int64_t np, int kernel, int transposed, int nthreads)
 {
+    vpmo_estr_direct_targets(P, np, 0, np, kernel, transposed, nthreads);
+}
+
+void vpmo_estr_direct_targets(double* P, int64_t np, int64_t t0, int64_t t1, int kernel, int transposed,
+                              int nthreads)
+{
     if (nthreads < 1) nthreads = 1;
     /* The SFS rows of a target are only written by that target's iteration,
      * and J/Gamma/X are only read, so targets are independent. */
 #pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
-    for (int64_t i = 0; i < np; i++) {
+    for (int64_t i = t0; i < t1; i++) {
         double* tp = P + i * VPMO_NFIELDS;
         if (is_static(tp)) continue;                                /* :63 */
         const double tx = tp[RX], ty = tp[RX + 1], tz = tp[RX + 2];
@@ -1152,6 +1158,56 @@ void vpmo_estr_ranges(double* P, int64_t stride, int kernel, int transposed,
                 const double dx = sp[RX] - tp[RX], dy = sp[RX + 1] - tp[RX + 1], dz = sp[RX + 2] - tp[RX + 2];
                 const double r = sqrt(dx * dx + dy * dy + dz * dz);
                 estr_pair(kernel, transposed, tp, sp, r);
+            }
+        }
+}
+
+/* ---- E_str summation scale ----
+ * s_i = sum over the pairs of Estr_direct! (subfilterscale_models.jl:16-41,
+ * :63-66 static skips) of zeta_sigma_j(r) |S_ij|, with S_ij the pair's
+ * (J_i - J_j)^T Gamma_j (or the classic form).  Whatever the cancellation in
+ * E_str_i = sum zeta S_ij, an evaluation whose pair terms carry a relative
+ * rounding error u is off by at most a small multiple of u s_i. */
+static inline double estr_pair_abs(int kernel, int transposed, const double* tp, const double* sp, double r)
+{
+    double T[VPMO_NFIELDS];
+    for (int c = 0; c < VPMO_NFIELDS; c++) T[c] = tp[c];
+    T[RSFS] = T[RSFS + 1] = T[RSFS + 2] = 0.0;
+    estr_pair(kernel, transposed, T, sp, r);
+    return sqrt(T[RSFS] * T[RSFS] + T[RSFS + 1] * T[RSFS + 1] + T[RSFS + 2] * T[RSFS + 2]);
+}
+
+void vpmo_estr_scale(const double* P, int64_t np, int kernel, int transposed, double* s, int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
+    for (int64_t i = 0; i < np; i++) {
+        const double* tp = P + i * VPMO_NFIELDS;
+        double acc = 0.0;
+        if (!is_static(tp)) {
+            for (int64_t j = 0; j < np; j++) {
+                const double* sp = P + j * VPMO_NFIELDS;
+                if (is_static(sp)) continue;
+                const double dx = sp[RX] - tp[RX], dy = sp[RX + 1] - tp[RX + 1], dz = sp[RX + 2] - tp[RX + 2];
+                acc += estr_pair_abs(kernel, transposed, tp, sp, sqrt(dx * dx + dy * dy + dz * dz));
+            }
+        }
+        s[i] = acc;
+    }
+}
+
+/* the same over the pairs of vpmo_estr_ranges (accumulated onto s) */
+void vpmo_estr_scale_ranges(const double* P, int64_t stride, int kernel, int transposed,
+                            const int64_t* tb, const int64_t* te, const int64_t* sb, const int64_t* se,
+                            int64_t npairs, double* s)
+{
+    for (int64_t k = 0; k < npairs; k++)
+        for (int64_t j = sb[k]; j < se[k]; j++) {
+            const double* sp = P + j * stride;
+            for (int64_t i = tb[k]; i < te[k]; i++) {
+                const double* tp = P + i * stride;
+                const double dx = sp[RX] - tp[RX], dy = sp[RX + 1] - tp[RX + 1], dz = sp[RX + 2] - tp[RX + 2];
+                s[i] += estr_pair_abs(kernel, transposed, tp, sp, sqrt(dx * dx + dy * dy + dz * dz));
             }
         }
 }

@@ -48,6 +48,15 @@ void vpmo_uj_direct(double* P, int64_t np, int kernel, int reset, int reset_sfs,
                     int sfs, int transposed, int nthreads);
 /* Estr_direct! (subfilterscale_models.jl:16-92) */
 void vpmo_estr_direct(double* P, int64_t np, int kernel, int transposed, int nthreads);
+/* E_str summation scale s_i = sum zeta |S_ij| over Estr_direct!'s pairs,
+ * or (accumulated onto s) over the pairs of vpmo_estr_ranges */
+void vpmo_estr_scale(const double* P, int64_t np, int kernel, int transposed, double* s, int nthreads);
+void vpmo_estr_scale_ranges(const double* P, int64_t stride, int kernel, int transposed,
+                            const int64_t* tb, const int64_t* te, const int64_t* sb, const int64_t* se,
+                            int64_t npairs, double* s);
+/* the same for targets [t0, t1) only (sources: all np) */
+void vpmo_estr_direct_targets(double* P, int64_t np, int64_t t0, int64_t t1, int kernel, int transposed,
+                              int nthreads);
 
 /* per-particle FP64 schemes ------------------------------------------------ */
 /* update_particle_states, rVPM (timeintegration.jl:463-532) */

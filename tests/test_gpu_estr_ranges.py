@@ -3,15 +3,26 @@ src/FLOWVPM_subfilterscale_models.jl:157-188) against the oracle's
 restatement: leaf pairs with overlapping and repeated target ranges, static
 particles (not skipped by Estr_fmm!) and SFS accumulated onto a nonzero
 start.  Tolerances are those of the all-pairs E_str (test_gpu_uj.py): F64
-to FP64 rounding, F32C within ESTR_MAX_TOL / ESTR_RMS_TOL per particle
-(E_str sums cancel by factors of 10-100 on random J)."""
+to FP64 rounding; F32C within ESTR_SCALE_TOL of the summation scale
+s_i = sum zeta |S_ij| over the list per particle (the budget is derived in
+test_gpu_uj.py), with the RMS relative error within ESTR_RMS_TOL (E_str sums
+cancel by factors of 10-100 on random J)."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
-ESTR_MAX_TOL = 2e-4
+ESTR_SCALE_TOL = 5e-5
 ESTR_RMS_TOL = 1e-5
+
+
+def _estr_check(vpm, orc, A, ref, P, kernel, lists, what, transposed=True):
+    scale = orc.estr_scale_ranges(P, kernel, *lists, transposed=transposed)
+    e = np.linalg.norm(A - ref, axis=1)
+    ratio = np.where(scale > 0, e / np.where(scale > 0, scale, 1.0), np.where(e > 0, np.inf, 0.0))
+    st = vpm.metrics.err_stats(A, ref)
+    print(what, "err/scale max %.2e" % ratio.max(), "relative", st, flush=True)
+    assert ratio.max() < ESTR_SCALE_TOL and st["rms"] < ESTR_RMS_TOL, (ratio.max(), st)
 
 
 def _leaf_lists(n, nleaves, seed, density=0.4):
@@ -43,12 +54,13 @@ def test_estr_ranges_vs_oracle(vpm, orc, kernel, precision):
     A = P.copy()
     vpm.api.estr_ranges(A, tb, te, sb, se, kernel=kernel, precision=precision)
     assert np.array_equal(A[:, 0:39], P[:, 0:39])
-    st = vpm.metrics.err_stats(A[:, 39:42], ref[:, 39:42])
-    print("estr_ranges", kernel, precision, st, flush=True)
     if precision == "f64" or kernel == "singular":
+        st = vpm.metrics.err_stats(A[:, 39:42], ref[:, 39:42])
+        print("estr_ranges", kernel, precision, st, flush=True)
         assert st["max"] < 1e-11, st
     else:
-        assert st["max"] < ESTR_MAX_TOL and st["rms"] < ESTR_RMS_TOL, st
+        _estr_check(vpm, orc, A[:, 39:42], ref[:, 39:42], P, kernel, (tb, te, sb, se),
+                    "estr_ranges %s %s" % (kernel, precision))
 
 
 def test_estr_ranges_whole_field_equals_estr_direct(vpm, orc):
@@ -95,6 +107,5 @@ def test_estr_ranges_small_leaves(vpm, orc, leaf, kernel):
     A = P.copy()
     vpm.api.estr_ranges(A, tb, te, sb, se, kernel=kernel, precision="f32c")
     assert np.array_equal(A[:, 0:39], P[:, 0:39])
-    st = vpm.metrics.err_stats(A[:, 39:42], ref[:, 39:42])
-    print("estr_ranges leaves", leaf, kernel, st, flush=True)
-    assert st["max"] < ESTR_MAX_TOL and st["rms"] < ESTR_RMS_TOL, st
+    _estr_check(vpm, orc, A[:, 39:42], ref[:, 39:42], P, kernel, (tb, te, sb, se),
+                "estr_ranges leaves %s %s" % (leaf, kernel))

@@ -287,7 +287,7 @@ def test_run_vpm_runtime_function_edits_field_vs_oracle(vpm, orc, resident):
         return k == 4
 
     api.run_vpm(pf, dt, 10, runtime_function=runtime, resident=resident)
-    assert pf.nt == 4 and pf.np == n + 3
+    assert pf.nt == 4 and pf.np == n + 5           # +2, +2 - 1, +2
     s = orc.make_settings(kernel="gaussianerf", f=0.0, g=0.2)
     R = P.copy()
     for k in range(1, 5):

@@ -13,12 +13,27 @@ pytestmark = pytest.mark.gpu
 KERNELS = ["singular", "gaussian", "gaussianerf", "winckelmans"]
 F32_TOL = 1e-5
 F64_TOL = 1e-12
-# E_str in FP32: on random clouds with random J the E_str sum cancels
-# strongly (sum |terms| / |sum| of order 10-100), and the FP32 rounding of
-# zeta_sigma and of the products is amplified by that factor; the bound is
-# stated on the max and on the RMS separately.
-ESTR_MAX_TOL = 2e-4
+# E_str in FP32 is held to its summation scale s_i = sum_j zeta |S_ij|
+# (oracle.estr_scale): |E32_i - E64_i| <= ESTR_SCALE_TOL s_i per particle.
+# Budget: a pair term carries <= ~30 eps of relative rounding (zeta =
+# b exp2(a r^2): the FP32 argument rounding times |a r^2| <= 35 near the
+# cut, plus a, b and the products), FP32 chains of <= 32 terms add <= 32 eps
+# of the partial sums, and the factored form J'_i^T A - B sums two parts
+# that can each reach a few times s_i: <= ~180 eps s_i = 1.1e-5 s_i; the
+# bound keeps a factor 4.5 above that.  On random J the sums cancel by
+# factors of 10-100, so the plain relative error is only reported (its RMS
+# is held to ESTR_RMS_TOL).
+ESTR_SCALE_TOL = 5e-5
 ESTR_RMS_TOL = 1e-5
+
+
+def _estr_check(vpm, A, ref, scale, what):
+    e = np.linalg.norm(A - ref, axis=1)
+    ratio = e / np.maximum(scale, 1e-300)
+    ratio[scale == 0] = np.where(e[scale == 0] == 0, 0.0, np.inf)
+    st = vpm.metrics.err_stats(A, ref)
+    _say(what, "err/scale max %.2e" % ratio.max(), "relative", st)
+    assert ratio.max() < ESTR_SCALE_TOL and st["rms"] < ESTR_RMS_TOL, (ratio.max(), st)
 
 
 def _say(*a):
@@ -150,29 +165,32 @@ def test_estr_vs_oracle(vpm, orc, kernel, transposed):
     orc.uj_direct(P, kernel)               # J from the oracle, shared by both
     ref = P.copy()
     orc.estr_direct(ref, kernel, transposed=transposed)
+    scale = orc.estr_scale(P, kernel, transposed=transposed)
     for prec in ("f64", "f32c"):
         A = P.copy()
         vpm.Estr_hip(A, kernel=kernel, precision=prec, transposed=transposed)
-        st = vpm.metrics.err_stats(A[:, 39:42], ref[:, 39:42])
-        _say("estr", kernel, transposed, prec, st)
         if prec == "f64" or kernel == "singular":
+            st = vpm.metrics.err_stats(A[:, 39:42], ref[:, 39:42])
+            _say("estr", kernel, transposed, prec, st)
             assert st["max"] < F64_TOL, st
         else:
-            assert st["max"] < ESTR_MAX_TOL and st["rms"] < ESTR_RMS_TOL, st
+            _estr_check(vpm, A[:, 39:42], ref[:, 39:42], scale, "estr %s %s %s" % (kernel, transposed, prec))
         assert np.all(A[[5, 6], 39:42] == P[[5, 6], 39:42])
 
 
-def test_estr_golden(vpm):
+def test_estr_golden(vpm, orc):
     g = golden("estr.json")
     for kernel in KERNELS:
         for tr in (True, False):
             P = vpm.fields.make_particles(g["X"], g["Gamma"], g["sigma"], static=g["static"])
             P[:, 15:24] = np.array(g["J"])
+            scale = orc.estr_scale(P, kernel, transposed=tr)
             vpm.Estr_hip(P, kernel=kernel, precision="f32c", transposed=tr)
             ref = np.array(g["%s_%s" % (kernel, "T" if tr else "C")])
-            st = vpm.metrics.err_stats(P[:, 39:42], ref)
-            _say("estr golden", kernel, tr, st)
-            assert st["max"] < ESTR_MAX_TOL and st["rms"] < ESTR_RMS_TOL, (kernel, tr, st)
+            if kernel == "singular":      # FP64 kernel (zeta is a delta)
+                assert vpm.metrics.err_stats(P[:, 39:42], ref)["max"] < 1e-12
+            else:
+                _estr_check(vpm, P[:, 39:42], ref, scale, "estr golden %s %s" % (kernel, tr))
 
 
 def test_uj_with_sfs_flag(vpm, orc):
@@ -182,9 +200,15 @@ def test_uj_with_sfs_flag(vpm, orc):
     orc.uj_direct(ref, "gaussianerf", reset_sfs=True, sfs=True)
     A = P.copy()
     vpm.UJ_hip(A, kernel="gaussianerf", reset_sfs=True, sfs=True)
-    st = vpm.metrics.err_stats(A[:, 39:42], ref[:, 39:42])
-    # E_str here also inherits the FP32c error of J from the same call
-    assert st["max"] < ESTR_MAX_TOL, st
+    _say("uj+sfs vs oracle: SFS", vpm.metrics.err_stats(A[:, 39:42], ref[:, 39:42]))
+    eu, ej = _uj_err(vpm, A, ref)
+    assert eu["max"] < F32_TOL and ej["max"] < F32_TOL
+    # E_str of this call inherits the FP32c error of J; the E_str kernel
+    # itself is checked on the J it was given
+    ref2 = A.copy()
+    ref2[:, 39:42] = 0.0
+    orc.estr_direct(ref2, "gaussianerf")
+    _estr_check(vpm, A[:, 39:42], ref2[:, 39:42], orc.estr_scale(ref2, "gaussianerf"), "uj+sfs E_str")
 
 
 @pytest.mark.parametrize("precision", ["f32c", "f64"])
@@ -247,9 +271,14 @@ def test_deterministic(vpm):
     np.testing.assert_array_equal(A, B)
 
 
-def test_cfg1_full_size_sample(vpm, orc):
-    """Config 1 at full size (N=65,536), Gaussian-erf and Winckelmans, FP32c
-    and FP64, against the oracle on a sample of 1,024 targets."""
+def test_cfg1_full_size(vpm, orc):
+    """Config 1 at full size (N=65,536), Gaussian-erf and Winckelmans: FP64
+    against the oracle on a sample of 1,024 targets, and FP32c against that
+    FP64 result on every one of the 65,536 targets (U and J within 1e-5 per
+    particle).  The cloud's U sums cancel by up to ~2,400x at some particles
+    (sum |term| ~ 28 against |U| ~ 0.011), which is what the near-pair folds
+    of the F32C kernel are for.  dGamma/dt = J^T Gamma is reported against
+    the FP64 result and against its scale |J| |Gamma|."""
     P = vpm.workloads.random_cloud(65536, seed=1)
     idx = np.arange(0, 65536, 64)
     for kernel in ("gaussianerf", "winckelmans"):
@@ -257,14 +286,24 @@ def test_cfg1_full_size_sample(vpm, orc):
         ref[:, 9:24] = 0
         sub = ref[idx].copy()
         orc.direct(sub, kernel, source=P)
-        for prec in ("f32c", "f64"):
-            A = P.copy()
-            vpm.UJ_hip(A, kernel=kernel, precision=prec)
-            eu, ej = _uj_err(vpm, A[idx], sub)
-            _say("cfg1", kernel, prec, "U", eu, "J", ej)
-            tol = F64_TOL if prec == "f64" else F32_TOL
-            assert eu["max"] < tol and ej["max"] < tol
-            assert np.all(np.isfinite(A[:, 9:24]))
+        B = P.copy()
+        vpm.UJ_hip(B, kernel=kernel, precision="f64")
+        eu, ej = _uj_err(vpm, B[idx], sub)
+        _say("cfg1", kernel, "f64 vs oracle sample U", eu, "J", ej)
+        assert eu["max"] < F64_TOL and ej["max"] < F64_TOL
+        A = P.copy()
+        vpm.UJ_hip(A, kernel=kernel, precision="f32c")
+        assert np.all(np.isfinite(A[:, 9:24]))
+        eu, ej = _uj_err(vpm, A, B)
+        G = P[:, 3:6]
+        S32, S64 = vpm.metrics.stretching(A[:, 15:24], G), vpm.metrics.stretching(B[:, 15:24], G)
+        es = vpm.metrics.err_stats(S32, S64)
+        scale = np.linalg.norm(B[:, 15:24], axis=1) * np.linalg.norm(G, axis=1)
+        es_scaled = float((np.linalg.norm(S32 - S64, axis=1) / scale).max())
+        _say("cfg1", kernel, "f32c vs f64, all targets: U", eu, "J", ej, "dGamma/dt", es,
+             "dGamma/dt vs |J||Gamma| max %.2e" % es_scaled)
+        assert eu["max"] < F32_TOL and ej["max"] < F32_TOL, (eu, ej)
+        assert es_scaled < F32_TOL
 
 
 @pytest.mark.parametrize("leaf", [5, 8, 16, 32, 64, 200])

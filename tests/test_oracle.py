@@ -1035,3 +1035,38 @@ def test_sfs_lagrangian_memory(orc, vpm):
             np.testing.assert_allclose(R[live, 36], (np.abs(c1) if fp else c1)[live], rtol=1e-12)
             assert np.all(R[live, 27:36] == 0.0)
         assert np.array_equal(R[~live], P[~live])
+
+
+@pytest.mark.parametrize("transposed", [True, False])
+def test_estr_scale_bounds_estr(orc, vpm, transposed):
+    """The E_str summation scale s_i = sum zeta |S_ij| (the bound the F32C
+    E_str tests are held to) dominates |E_str_i| for every particle, equals
+    it when each target has a single source term, skips static particles as
+    Estr_direct! does, and over a list of sub-ranges sums to the all-pairs
+    value."""
+    n = 400
+    P = vpm.workloads.random_cloud(n, seed=51)
+    P[:, 6] *= 3.0
+    P[:, 15:24] = np.random.default_rng(52).normal(size=(n, 9))
+    P[[7, 90], 42] = 1.0
+    ref = P.copy()
+    orc.estr_direct(ref, "gaussianerf", transposed=transposed)
+    s = orc.estr_scale(P, "gaussianerf", transposed=transposed)
+    e = np.linalg.norm(ref[:, 39:42], axis=1)
+    assert np.all(e <= s * (1 + 1e-12))
+    assert np.all(s[[7, 90]] == 0.0) and np.all(s[np.arange(n) != 7] >= 0)
+    assert np.median(s / np.maximum(e, 1e-300)) > 1.5          # the sums do cancel
+    Q = P[:2].copy()
+    Q[:, 42] = 0.0
+    r2 = Q.copy()
+    orc.estr_direct(r2, "gaussianerf", transposed=transposed)
+    np.testing.assert_allclose(orc.estr_scale(Q, "gaussianerf", transposed=transposed),
+                               np.linalg.norm(r2[:, 39:42], axis=1), rtol=1e-14)
+    P[:, 42] = 0.0
+    cuts = [0, 37, 150, 151, 400]
+    tb, te, sb, se = [], [], [], []
+    for a, b in zip(cuts[:-1], cuts[1:]):
+        for c, d in zip(cuts[:-1], cuts[1:]):
+            tb.append(a); te.append(b); sb.append(c); se.append(d)
+    np.testing.assert_allclose(orc.estr_scale_ranges(P, "gaussianerf", tb, te, sb, se, transposed=transposed),
+                               orc.estr_scale(P, "gaussianerf", transposed=transposed), rtol=1e-12)
