@@ -10,7 +10,10 @@
 //     sum_j zeta_j (J'_i - J'_j)^T Gamma_j = J'_i^T A_i - B_i,
 //     A_i = sum_j zeta_j Gamma_j,  B_i = sum_j zeta_j J'_j^T Gamma_j,
 // so a pair costs one zeta and six FMAs, and every FP32 rounding is relative
-// to the local variation of J (not to |J|).  Distances use the hi/lo split of
+// to the local variation of J (not to |J|).  The self pair, exactly zero in
+// the reference, is left out: in the factored form it would enter A and B as
+// two terms zeta(0) J'_i^T Gamma_i that cancel, which for the Gaussian kernel
+// (zeta(0) = e^8 zeta(2)) can dwarf every other term.  Distances use the hi/lo split of
 // positions relative to the block origin (zeta is steep, so r needs full
 // FP32 relative accuracy).  Sources beyond q_far, where zeta is below FP32
 // resolution of the near sum, are skipped per wave.  A and B use the same
@@ -187,7 +190,10 @@ __global__ __launch_bounds__(kBlock) void estr_f32c_kernel(
                     dy += tly - lo.y;
                     dz += tlz - lo.z;
                     r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    const float z = zeta_sig<K>(r2, a.w, b.w);
+                    // the self pair adds (J_i - J_i)^T Gamma_i = 0 exactly in
+                    // the reference, but the factored form would add it as
+                    // zeta(0) J'_i^T Gamma_i twice with opposite signs
+                    const float z = base + s == i ? 0.f : zeta_sig<K>(r2, a.w, b.w);
                     acc[0] = fmaf(z, b.x, acc[0]);
                     acc[1] = fmaf(z, b.y, acc[1]);
                     acc[2] = fmaf(z, b.z, acc[2]);

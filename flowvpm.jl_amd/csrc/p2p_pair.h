@@ -222,13 +222,30 @@ __device__ __forceinline__ void fold_twosum(const float (&acc)[kAccF32], float (
     }
 }
 
-// Near pairs per fold of the FP32 partials (tile_f32c).  On the N = 65,536
-// random cloud, whose near sums cancel by up to ~2,400x, the worst
-// particle's U error against FP64 falls from 1.3e-5 (one fold per tile) to
-// 7.9e-6 and its J error from 3.4e-6 to 1.2e-6 (folds every 1, 2, 4 or 8
-// near pairs: 8.7, 8.6, 7.9, 9.0e-6 in U), measured in an FP32 emulation of
-// the kernel; what remains is the rounding of the terms themselves.
+// Near pairs per fold of the U partials (tile_f32c over 256-source tiles).
+// On the N = 65,536 random cloud, whose U sums cancel by up to ~2,400x at
+// some particles, the worst particle's U error against FP64 falls from
+// 1.5e-5 (one fold per tile) to 8.1e-6, measured on the GPU; an FP32
+// emulation of the kernel gave 8.7, 8.6, 7.9, 9.0 and 10.3e-6 for folds
+// every 1, 2, 4, 8 and 16 near pairs, what remains being the rounding of
+// the terms themselves.  Folding U alone costs 0.7 % on the jet and 5 % on
+// that cloud; folding all 14 accumulators also takes J from 3.3e-6 to
+// 1.3e-6 but costs 1 % and 12 %, and J is well inside 1e-5 without it.
 constexpr int kNearFold = 4;
+
+// TwoSum of the U partials (acc[0..2]) into (sum, err), partials reset
+__device__ __forceinline__ void fold_u(float (&acc)[kAccF32], float (&sum)[kAccF32], float (&err)[kAccF32])
+{
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float x = acc[c];
+        const float t = sum[c] + x;
+        const float bp = t - sum[c];
+        err[c] += (sum[c] - (t - bp)) + (x - bp);
+        sum[c] = t;
+        acc[c] = 0.f;
+    }
+}
 
 // One LDS tile of n <= 64 NG sources against the lane's target (F32C).
 // a = (s'_hi, far threshold), b = (Gamma', 1/sigma), l = s'_lo.
@@ -278,8 +295,10 @@ __device__ __forceinline__ void tile_f32c(int n, const float4* lpos, const float
         }
     }
     // Near pairs carry the largest terms of a sum that may cancel to far
-    // below them, so their FP32 partials are folded into the TwoSum pairs
-    // every kNearFold near pairs (the count is wave-uniform).
+    // below them, so over 256-source tiles the U partials are folded into
+    // the TwoSum pairs every kNearFold near pairs (the count is
+    // wave-uniform).  One-group tiles (the wave-per-leaf list kernels) fold
+    // every 64 sources anyway.
     int nn = 0;
 #pragma unroll
     for (int g = 0; g < NG; ++g) {
@@ -288,14 +307,15 @@ __device__ __forceinline__ void tile_f32c(int n, const float4* lpos, const float
             const int s = g * 64 + __builtin_ctzll(nb);
             nb &= nb - 1;
             near_pair<K>(tx, ty, tz, tlx, tly, tlz, lpos[s], lgs[s], llo[s], acc);
-            if (++nn == kNearFold) {
-                fold_twosum(acc, sum, err);
-#pragma unroll
-                for (int c = 0; c < kAccF32; ++c) acc[c] = 0.f;
-                nn = 0;
+            if constexpr (NG > 1) {
+                if (++nn == kNearFold) {
+                    fold_u(acc, sum, err);
+                    nn = 0;
+                }
             }
         }
     }
+    (void)nn;
 }
 
 // hi/lo split of a position relative to the block origin

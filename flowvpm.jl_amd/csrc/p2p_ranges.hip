@@ -99,8 +99,11 @@ __global__ __launch_bounds__(kBlock) void ranges_f32c_kernel(const RangeItem* __
 // the far radius (then every pair has r > 0), near otherwise.  Per-lane FP32
 // sums fold into TwoSum pairs every ~64 pairs; at the end the R groups'
 // FP64 values are added in a fixed butterfly and group 0 writes.
+// R > 1: five waves per SIMD (95 VGPRs, no spills) instead of the four that
+// 107 VGPRs allow, 6 % faster on 16-particle leaves.  R = 1 would spill at
+// five and keeps four.
 template <int K, int R>
-__global__ __launch_bounds__(kBlock) void ranges_wave_f32c_kernel(const RangeItem* __restrict__ items,
+__global__ __launch_bounds__(kBlock, R > 1 ? 5 : 4) void ranges_wave_f32c_kernel(const RangeItem* __restrict__ items,
                                                                   int64_t nitems,
                                                                   const int2* __restrict__ lsrc,
                                                                   const double* __restrict__ pos, int64_t ld,
@@ -114,7 +117,9 @@ __global__ __launch_bounds__(kBlock) void ranges_wave_f32c_kernel(const RangeIte
     __shared__ float4 lgs_all[kBlock / 64][64];
     __shared__ float4 llo_all[kBlock / 64][64];
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    // wave-uniform (kept in scalar registers, so the item and its source
+    // ranges are read with scalar loads and the stream bookkeeping is SALU)
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t w = (int64_t)blockIdx.x * (kBlock / 64) + wave;
     if (w >= nitems) return;   // no block barrier below
     float4* lpos = lpos_all[wave];
@@ -524,7 +529,9 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_f32c_kernel(const RangeIte
                         dz += tlz - lo.z;
                         r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                         float z;
-                        if constexpr (K == VPM_KERNEL_GAUSSIANERF) {
+                        if (base + s == t) {
+                            z = 0.f;   // self pair: exactly zero (see estr.hip)
+                        } else if constexpr (K == VPM_KERNEL_GAUSSIANERF) {
                             z = g.w * ex2(a.w * r2);
                         } else if constexpr (K == VPM_KERNEL_GAUSSIAN) {
                             const float r = r2 > 0.f ? r2 * rsq(r2) : 0.f;
@@ -597,7 +604,9 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
     __shared__ float4 lq_all[kBlock / 64][64];   // J'^T Gamma (or J' Gamma), cut
     __shared__ float4 ll_all[kBlock / 64][64];   // s'_lo
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    // wave-uniform (kept in scalar registers, so the item and its source
+    // ranges are read with scalar loads and the stream bookkeeping is SALU)
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t w = (int64_t)blockIdx.x * (kBlock / 64) + wave;
     if (w >= nitems) return;   // no block barrier below
     float4* lp = lp_all[wave];
@@ -655,7 +664,7 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
             else if (K == VPM_KERNEL_GAUSSIAN) { a = -l2e * s3; b = kConst3 * s3; cut = 9.0 * sg * sg; }
             else { a = s2; b = kConst4 * 7.5 * s3; cut = (double)FLT_MAX; }
             p.w = (float)a;
-            lo.w = 0.f;
+            lo.w = __builtin_bit_cast(float, (int)j);   // the source, for the self-pair test
             const double G0 = sr[RG], G1 = sr[RG + 1], G2 = sr[RG + 2];
             double Jp[9];
 #pragma unroll
@@ -674,7 +683,7 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
         const bool real = lane < filled;
         const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
         lp[lane] = sel4(real, p, make_float4(1e15f, 1e15f, 1e15f, 0.f));
-        ll[lane] = sel4(real, lo, z4);
+        ll[lane] = sel4(real, lo, make_float4(0.f, 0.f, 0.f, __builtin_bit_cast(float, -1)));
         lq[lane] = sel4(real, q, z4);
         lg[lane] = sel4(real, gq, z4);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -696,7 +705,9 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
                     dz += tlz - lo.z;
                     r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                     float z;
-                    if constexpr (K == VPM_KERNEL_GAUSSIANERF) {
+                    if (__builtin_bit_cast(int, lo.w) == (int)t) {
+                        z = 0.f;   // self pair: exactly zero (see estr.hip)
+                    } else if constexpr (K == VPM_KERNEL_GAUSSIANERF) {
                         z = gg.w * ex2(a.w * r2);
                     } else if constexpr (K == VPM_KERNEL_GAUSSIAN) {
                         const float r = r2 > 0.f ? r2 * rsq(r2) : 0.f;
@@ -994,7 +1005,9 @@ __global__ __launch_bounds__(kBlock) void zeta_ranges_wave_f32c_kernel(const Ran
     __shared__ float4 lg_all[kBlock / 64][64];   // Gamma, b
     __shared__ float4 ll_all[kBlock / 64][64];   // s'_lo, a
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    // wave-uniform (kept in scalar registers, so the item and its source
+    // ranges are read with scalar loads and the stream bookkeeping is SALU)
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t w = (int64_t)blockIdx.x * (kBlock / 64) + wave;
     if (w >= nitems) return;   // no block barrier below
     float4* lp = lp_all[wave];

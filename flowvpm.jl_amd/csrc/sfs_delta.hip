@@ -374,7 +374,11 @@ __global__ __launch_bounds__(kBlock) void de_f32c_kernel(const double* __restric
                     zeta_and_rate<K>(r2, fmaf(w, kGlX2, alpha * sg), z_, r2v);
                     float rq;
                     zeta_and_rate<K>(r2, fmaf(w, kGlX0, alpha * sg), z_, rq);
-                    const float dz_ = w * (kGlW0 * (rq + r2v) + kGlW1 * r1);
+                    float dz_ = w * (kGlW0 * (rq + r2v) + kGlW1 * r1);
+                    // the self pair is exactly zero in the reference ((J_i -
+                    // J_i) and (dJ_i - dJ_i)); the factored form would add it
+                    // as cancelling terms
+                    if (base + s == i) { dz_ = 0.f; zt = 0.f; }
                     const float4 c1 = l1[s], c2 = l2[s];
                     acc[0] = fmaf(dz_, g.x, acc[0]);
                     acc[1] = fmaf(dz_, g.y, acc[1]);
