@@ -32,7 +32,7 @@ PEAK_FP64 = 78.6e12
 # HBM bytes per U/J launch from two rocprofv3 --pmc runs of this script
 # (FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0, the factors a 256 MiB
 # device copy reads back), keyed by (N, kernel)
-TRAFFIC_BYTES = {(1 << 20, "gaussianerf"): (5.1060e6 * 2.0 + 9.1750e5) * 1024}
+TRAFFIC_BYTES = {(1 << 20, "gaussianerf"): (9.0304e6 * 2.0 + 9.1750e5) * 1024}
 
 
 def log(*a):

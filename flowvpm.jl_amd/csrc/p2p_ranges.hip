@@ -689,47 +689,107 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // k R < filled <= 64 and 64 is a multiple of R, so sidx < 64
-        for (int k = 0; k * R < filled; ++k) {
-            const int sidx = k * R + g;
-            const float4 a = lp[sidx];
-            const float4 c = lq[sidx];
-            float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
-            float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            if (__any(r2 < c.w)) {
-                {
-                    const float4 lo = ll[sidx];
-                    const float4 gg = lg[sidx];
-                    dx += tlx - lo.x;
-                    dy += tly - lo.y;
-                    dz += tlz - lo.z;
-                    r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    float z;
-                    if (__builtin_bit_cast(int, lo.w) == (int)t) {
-                        z = 0.f;   // self pair: exactly zero (see estr.hip)
-                    } else if constexpr (K == VPM_KERNEL_GAUSSIANERF) {
-                        z = gg.w * ex2(a.w * r2);
+        if constexpr (R == 1) {
+            // whole-wave leaves: two steps per iteration, so the two pairs'
+            // LDS reads and exp2 overlap (10 % faster on 64-particle leaves;
+            // 4 % slower with lane sharing, which keeps one step)
+            // steps k and k + 1: past the stream the slot is padding, and
+            // k + 1 < 64 / R whenever k R < filled <= 64 with k even
+            for (int k = 0; k * R < filled; k += 2) {
+                const int s0 = k * R + g, s1 = s0 + R;
+                const float4 a0 = lp[s0], a1 = lp[s1];
+                const float4 c0 = lq[s0], c1 = lq[s1];
+                float dx0 = tx - a0.x, dy0 = ty - a0.y, dz0 = tz - a0.z;
+                float dx1 = tx - a1.x, dy1 = ty - a1.y, dz1 = tz - a1.z;
+                float r20 = fmaf(dz0, dz0, fmaf(dy0, dy0, dx0 * dx0));
+                float r21 = fmaf(dz1, dz1, fmaf(dy1, dy1, dx1 * dx1));
+                if (__any(r20 < c0.w || r21 < c1.w)) {
+                    const float4 lo0 = ll[s0], lo1 = ll[s1];
+                    const float4 g0 = lg[s0], g1 = lg[s1];
+                    dx0 += tlx - lo0.x; dy0 += tly - lo0.y; dz0 += tlz - lo0.z;
+                    dx1 += tlx - lo1.x; dy1 += tly - lo1.y; dz1 += tlz - lo1.z;
+                    r20 = fmaf(dz0, dz0, fmaf(dy0, dy0, dx0 * dx0));
+                    r21 = fmaf(dz1, dz1, fmaf(dy1, dy1, dx1 * dx1));
+                    float z0, z1;
+                    if constexpr (K == VPM_KERNEL_GAUSSIANERF) {
+                        z0 = g0.w * ex2(a0.w * r20);
+                        z1 = g1.w * ex2(a1.w * r21);
                     } else if constexpr (K == VPM_KERNEL_GAUSSIAN) {
-                        const float r = r2 > 0.f ? r2 * rsq(r2) : 0.f;
-                        z = gg.w * ex2(a.w * r2 * r);
+                        const float q0 = r20 > 0.f ? r20 * rsq(r20) : 0.f;
+                        const float q1 = r21 > 0.f ? r21 * rsq(r21) : 0.f;
+                        z0 = g0.w * ex2(a0.w * r20 * q0);
+                        z1 = g1.w * ex2(a1.w * r21 * q1);
                     } else {
-                        const float ww = rsq(fmaf(a.w, r2, 1.f));
-                        const float w2 = ww * ww;
-                        z = gg.w * (w2 * w2) * (w2 * ww);
+                        const float w0 = rsq(fmaf(a0.w, r20, 1.f)), w1 = rsq(fmaf(a1.w, r21, 1.f));
+                        const float v0 = w0 * w0, v1 = w1 * w1;
+                        z0 = g0.w * (v0 * v0) * (v0 * w0);
+                        z1 = g1.w * (v1 * v1) * (v1 * w1);
                     }
-                    acc[0] = fmaf(z, gg.x, acc[0]);
-                    acc[1] = fmaf(z, gg.y, acc[1]);
-                    acc[2] = fmaf(z, gg.z, acc[2]);
-                    acc[3] = fmaf(z, c.x, acc[3]);
-                    acc[4] = fmaf(z, c.y, acc[4]);
-                    acc[5] = fmaf(z, c.z, acc[5]);
+                    if (__builtin_bit_cast(int, lo0.w) == (int)t) z0 = 0.f;   // self pair
+                    if (__builtin_bit_cast(int, lo1.w) == (int)t) z1 = 0.f;
+                    acc[0] = fmaf(z0, g0.x, acc[0]);
+                    acc[1] = fmaf(z0, g0.y, acc[1]);
+                    acc[2] = fmaf(z0, g0.z, acc[2]);
+                    acc[3] = fmaf(z0, c0.x, acc[3]);
+                    acc[4] = fmaf(z0, c0.y, acc[4]);
+                    acc[5] = fmaf(z0, c0.z, acc[5]);
+                    acc[0] = fmaf(z1, g1.x, acc[0]);
+                    acc[1] = fmaf(z1, g1.y, acc[1]);
+                    acc[2] = fmaf(z1, g1.z, acc[2]);
+                    acc[3] = fmaf(z1, c1.x, acc[3]);
+                    acc[4] = fmaf(z1, c1.y, acc[4]);
+                    acc[5] = fmaf(z1, c1.z, acc[5]);
+                }
+                since = __builtin_amdgcn_readfirstlane(since + 2);   // wave-uniform
+                if (since >= kEstrFold) {
+    #pragma unroll
+                    for (int c = 0; c < 6; ++c) { A[c] += (double)acc[c]; acc[c] = 0.f; }
+                    since = 0;
                 }
             }
-            since = __builtin_amdgcn_readfirstlane(since + 1);   // wave-uniform
-            if (since == kEstrFold) {
-#pragma unroll
-                for (int c = 0; c < 6; ++c) { A[c] += (double)acc[c]; acc[c] = 0.f; }
-                since = 0;
+        } else {
+            // k R < filled <= 64 and 64 is a multiple of R, so sidx < 64
+            for (int k = 0; k * R < filled; ++k) {
+                const int sidx = k * R + g;
+                const float4 a = lp[sidx];
+                const float4 c = lq[sidx];
+                float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
+                float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                if (__any(r2 < c.w)) {
+                    {
+                        const float4 lo = ll[sidx];
+                        const float4 gg = lg[sidx];
+                        dx += tlx - lo.x;
+                        dy += tly - lo.y;
+                        dz += tlz - lo.z;
+                        r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                        float z;
+                        if (__builtin_bit_cast(int, lo.w) == (int)t) {
+                            z = 0.f;   // self pair: exactly zero (see estr.hip)
+                        } else if constexpr (K == VPM_KERNEL_GAUSSIANERF) {
+                            z = gg.w * ex2(a.w * r2);
+                        } else if constexpr (K == VPM_KERNEL_GAUSSIAN) {
+                            const float r = r2 > 0.f ? r2 * rsq(r2) : 0.f;
+                            z = gg.w * ex2(a.w * r2 * r);
+                        } else {
+                            const float ww = rsq(fmaf(a.w, r2, 1.f));
+                            const float w2 = ww * ww;
+                            z = gg.w * (w2 * w2) * (w2 * ww);
+                        }
+                        acc[0] = fmaf(z, gg.x, acc[0]);
+                        acc[1] = fmaf(z, gg.y, acc[1]);
+                        acc[2] = fmaf(z, gg.z, acc[2]);
+                        acc[3] = fmaf(z, c.x, acc[3]);
+                        acc[4] = fmaf(z, c.y, acc[4]);
+                        acc[5] = fmaf(z, c.z, acc[5]);
+                    }
+                }
+                since = __builtin_amdgcn_readfirstlane(since + 1);   // wave-uniform
+                if (since == kEstrFold) {
+    #pragma unroll
+                    for (int c = 0; c < 6; ++c) { A[c] += (double)acc[c]; acc[c] = 0.f; }
+                    since = 0;
+                }
             }
         }
     }
