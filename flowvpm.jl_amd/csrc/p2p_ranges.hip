@@ -18,6 +18,18 @@
 
 namespace vpm {
 
+// Blocks are dealt round-robin to the 8 XCDs (block b runs on XCD b mod 8).
+// This renumbering (a bijection of [0, G)) gives each XCD a contiguous run
+// of items, so the items one XCD runs at once are neighbours in the list and
+// find their shared sources in that XCD's L2: on the 2^20 jet's 16-particle
+// leaf lists E_str and zeta take a third less time, U/J (compute-bound) the
+// same.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t G)
+{
+    const int64_t x = b & 7, q = G >> 3, r = G & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
 // Component-wise select of two float4 values (kept in registers: a ?: on
 // the vector type itself can go through private memory).
 __device__ __forceinline__ float4 sel4(bool c, float4 a, float4 b)
@@ -36,7 +48,7 @@ __global__ __launch_bounds__(kBlock) void ranges_f32c_kernel(const RangeItem* __
     __shared__ float4 lpos[kTile];
     __shared__ float4 lgs[kTile];
     __shared__ float4 llo[kTile];
-    const RangeItem it = items[blockIdx.x];
+    const RangeItem it = items[xcd_block(blockIdx.x, gridDim.x)];
     const int tid = threadIdx.x;
     const int64_t ib = it.t_begin;
     const double ox = pos[ib], oy = pos[ld + ib], oz = pos[2 * ld + ib];
@@ -120,7 +132,7 @@ __global__ __launch_bounds__(kBlock, R > 1 ? 5 : 4) void ranges_wave_f32c_kernel
     // wave-uniform (kept in scalar registers, so the item and its source
     // ranges are read with scalar loads and the stream bookkeeping is SALU)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t w = (int64_t)blockIdx.x * (kBlock / 64) + wave;
+    const int64_t w = xcd_block(blockIdx.x, gridDim.x) * (kBlock / 64) + wave;
     if (w >= nitems) return;   // no block barrier below
     float4* lpos = lpos_all[wave];
     float4* lgs = lgs_all[wave];
@@ -273,7 +285,7 @@ __global__ __launch_bounds__(kBlock) void ranges_f64_kernel(const RangeItem* __r
 {
     __shared__ double4 lpos[kTile];
     __shared__ double4 lgam[kTile];
-    const RangeItem it = items[blockIdx.x];
+    const RangeItem it = items[xcd_block(blockIdx.x, gridDim.x)];
     const int tid = threadIdx.x;
     const int64_t t = it.t_begin + tid;
     const bool active = t < it.t_end;
@@ -455,7 +467,7 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_f32c_kernel(const RangeIte
     __shared__ float4 lg[kTile];   // Gamma, b
     __shared__ float4 lq[kTile];   // J'^T Gamma (or J' Gamma), cut
     __shared__ float4 ll[kTile];   // s'_lo
-    const RangeItem it = items[blockIdx.x];
+    const RangeItem it = items[xcd_block(blockIdx.x, gridDim.x)];
     const int tid = threadIdx.x;
     const double* r0 = P + it.t_begin * NF;
     const double ox = r0[RX], oy = r0[RX + 1], oz = r0[RX + 2];
@@ -599,15 +611,15 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
 {
     static_assert(R == 1 || R == 2 || R == 4 || R == 8, "replication");
     constexpr int TW = 64 / R;
-    __shared__ float4 lp_all[kBlock / 64][64];   // s'_hi, a
+    __shared__ float4 lp_all[kBlock / 64][64];   // s'_hi, cut
     __shared__ float4 lg_all[kBlock / 64][64];   // Gamma, b
-    __shared__ float4 lq_all[kBlock / 64][64];   // J'^T Gamma (or J' Gamma), cut
+    __shared__ float4 lq_all[kBlock / 64][64];   // J'^T Gamma (or J' Gamma), a
     __shared__ float4 ll_all[kBlock / 64][64];   // s'_lo
     const int lane = threadIdx.x & 63;
     // wave-uniform (kept in scalar registers, so the item and its source
     // ranges are read with scalar loads and the stream bookkeeping is SALU)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t w = (int64_t)blockIdx.x * (kBlock / 64) + wave;
+    const int64_t w = xcd_block(blockIdx.x, gridDim.x) * (kBlock / 64) + wave;
     if (w >= nitems) return;   // no block barrier below
     float4* lp = lp_all[wave];
     float4* lg = lg_all[wave];
@@ -663,7 +675,7 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
             if (K == VPM_KERNEL_GAUSSIANERF) { a = -0.5 * l2e * s2; b = kConst1 * s3; cut = 49.0 * sg * sg; }
             else if (K == VPM_KERNEL_GAUSSIAN) { a = -l2e * s3; b = kConst3 * s3; cut = 9.0 * sg * sg; }
             else { a = s2; b = kConst4 * 7.5 * s3; cut = (double)FLT_MAX; }
-            p.w = (float)a;
+            p.w = (float)cut;
             lo.w = __builtin_bit_cast(float, (int)j);   // the source, for the self-pair test
             const double G0 = sr[RG], G1 = sr[RG + 1], G2 = sr[RG + 2];
             double Jp[9];
@@ -672,11 +684,11 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
             if (transposed) {
                 q = make_float4((float)(Jp[0] * G0 + Jp[1] * G1 + Jp[2] * G2),
                                 (float)(Jp[3] * G0 + Jp[4] * G1 + Jp[5] * G2),
-                                (float)(Jp[6] * G0 + Jp[7] * G1 + Jp[8] * G2), (float)cut);
+                                (float)(Jp[6] * G0 + Jp[7] * G1 + Jp[8] * G2), (float)a);
             } else {
                 q = make_float4((float)(Jp[0] * G0 + Jp[3] * G1 + Jp[6] * G2),
                                 (float)(Jp[1] * G0 + Jp[4] * G1 + Jp[7] * G2),
-                                (float)(Jp[2] * G0 + Jp[5] * G1 + Jp[8] * G2), (float)cut);
+                                (float)(Jp[2] * G0 + Jp[5] * G1 + Jp[8] * G2), (float)a);
             }
             gq = make_float4((float)G0, (float)G1, (float)G2, (float)b);
         }
@@ -698,12 +710,12 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
             for (int k = 0; k * R < filled; k += 2) {
                 const int s0 = k * R + g, s1 = s0 + R;
                 const float4 a0 = lp[s0], a1 = lp[s1];
-                const float4 c0 = lq[s0], c1 = lq[s1];
                 float dx0 = tx - a0.x, dy0 = ty - a0.y, dz0 = tz - a0.z;
                 float dx1 = tx - a1.x, dy1 = ty - a1.y, dz1 = tz - a1.z;
                 float r20 = fmaf(dz0, dz0, fmaf(dy0, dy0, dx0 * dx0));
                 float r21 = fmaf(dz1, dz1, fmaf(dy1, dy1, dx1 * dx1));
-                if (__any(r20 < c0.w || r21 < c1.w)) {
+                if (__any(r20 < a0.w || r21 < a1.w)) {
+                    const float4 c0 = lq[s0], c1 = lq[s1];
                     const float4 lo0 = ll[s0], lo1 = ll[s1];
                     const float4 g0 = lg[s0], g1 = lg[s1];
                     dx0 += tlx - lo0.x; dy0 += tly - lo0.y; dz0 += tlz - lo0.z;
@@ -712,15 +724,15 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
                     r21 = fmaf(dz1, dz1, fmaf(dy1, dy1, dx1 * dx1));
                     float z0, z1;
                     if constexpr (K == VPM_KERNEL_GAUSSIANERF) {
-                        z0 = g0.w * ex2(a0.w * r20);
-                        z1 = g1.w * ex2(a1.w * r21);
+                        z0 = g0.w * ex2(c0.w * r20);
+                        z1 = g1.w * ex2(c1.w * r21);
                     } else if constexpr (K == VPM_KERNEL_GAUSSIAN) {
                         const float q0 = r20 > 0.f ? r20 * rsq(r20) : 0.f;
                         const float q1 = r21 > 0.f ? r21 * rsq(r21) : 0.f;
-                        z0 = g0.w * ex2(a0.w * r20 * q0);
-                        z1 = g1.w * ex2(a1.w * r21 * q1);
+                        z0 = g0.w * ex2(c0.w * r20 * q0);
+                        z1 = g1.w * ex2(c1.w * r21 * q1);
                     } else {
-                        const float w0 = rsq(fmaf(a0.w, r20, 1.f)), w1 = rsq(fmaf(a1.w, r21, 1.f));
+                        const float w0 = rsq(fmaf(c0.w, r20, 1.f)), w1 = rsq(fmaf(c1.w, r21, 1.f));
                         const float v0 = w0 * w0, v1 = w1 * w1;
                         z0 = g0.w * (v0 * v0) * (v0 * w0);
                         z1 = g1.w * (v1 * v1) * (v1 * w1);
@@ -752,10 +764,10 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
             for (int k = 0; k * R < filled; ++k) {
                 const int sidx = k * R + g;
                 const float4 a = lp[sidx];
-                const float4 c = lq[sidx];
                 float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
                 float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                if (__any(r2 < c.w)) {
+                if (__any(r2 < a.w)) {
+                    const float4 c = lq[sidx];
                     {
                         const float4 lo = ll[sidx];
                         const float4 gg = lg[sidx];
@@ -767,12 +779,12 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
                         if (__builtin_bit_cast(int, lo.w) == (int)t) {
                             z = 0.f;   // self pair: exactly zero (see estr.hip)
                         } else if constexpr (K == VPM_KERNEL_GAUSSIANERF) {
-                            z = gg.w * ex2(a.w * r2);
+                            z = gg.w * ex2(c.w * r2);
                         } else if constexpr (K == VPM_KERNEL_GAUSSIAN) {
                             const float r = r2 > 0.f ? r2 * rsq(r2) : 0.f;
-                            z = gg.w * ex2(a.w * r2 * r);
+                            z = gg.w * ex2(c.w * r2 * r);
                         } else {
-                            const float ww = rsq(fmaf(a.w, r2, 1.f));
+                            const float ww = rsq(fmaf(c.w, r2, 1.f));
                             const float w2 = ww * ww;
                             z = gg.w * (w2 * w2) * (w2 * ww);
                         }
@@ -840,7 +852,7 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_f64_kernel(const RangeItem
     __shared__ double lx[kTile][4];
     __shared__ double lg[kTile][3];
     __shared__ double lj[kTile][9];
-    const RangeItem it = items[blockIdx.x];
+    const RangeItem it = items[xcd_block(blockIdx.x, gridDim.x)];
     const int tid = threadIdx.x;
     const int64_t t = it.t_begin + tid;
     const bool active = t < it.t_end;
@@ -972,7 +984,7 @@ __global__ __launch_bounds__(BS) void zeta_ranges_f32c_kernel(const RangeItem* _
     __shared__ float4 lp[BS];   // s'_hi, cut
     __shared__ float4 lg[BS];   // Gamma, b
     __shared__ float4 ll[BS];   // s'_lo, a
-    const RangeItem it = items[blockIdx.x];
+    const RangeItem it = items[xcd_block(blockIdx.x, gridDim.x)];
     const int tid = threadIdx.x;
     const double* r0 = P + it.t_begin * NF;
     const double ox = r0[RX], oy = r0[RX + 1], oz = r0[RX + 2];
@@ -1068,7 +1080,7 @@ __global__ __launch_bounds__(kBlock) void zeta_ranges_wave_f32c_kernel(const Ran
     // wave-uniform (kept in scalar registers, so the item and its source
     // ranges are read with scalar loads and the stream bookkeeping is SALU)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t w = (int64_t)blockIdx.x * (kBlock / 64) + wave;
+    const int64_t w = xcd_block(blockIdx.x, gridDim.x) * (kBlock / 64) + wave;
     if (w >= nitems) return;   // no block barrier below
     float4* lp = lp_all[wave];
     float4* lg = lg_all[wave];
@@ -1180,7 +1192,7 @@ __global__ __launch_bounds__(kBlock) void zeta_ranges_f64_kernel(const RangeItem
 {
     __shared__ double4 ls[kTile];   // x, y, z, sigma
     __shared__ double4 lg[kTile];   // Gamma
-    const RangeItem it = items[blockIdx.x];
+    const RangeItem it = items[xcd_block(blockIdx.x, gridDim.x)];
     const int tid = threadIdx.x;
     const int64_t t = it.t_begin + tid;
     const bool active = t < it.t_end;
