@@ -588,6 +588,112 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_f32c_kernel(const RangeIte
     }
 }
 
+// Source records of the F32C list kernels for E_str and zeta, packed once
+// per call.  The list kernels read each source once for every target leaf
+// that lists it (about 140 times on FMM leaf lists); gathering 48 (zeta) or
+// 72 (E_str) contiguous bytes per source instead of parts of its 368-byte
+// particle record cuts that traffic threefold.  Per particle: the position
+// (FP64, SoA, stride ld), (Gamma, b) and (a, cut) in FP32, and for E_str
+// J^T Gamma (transposed) or J Gamma in FP64, formed with Gamma already
+// rounded to FP32, so that the kernel's J_ref^T Gamma subtracts the same
+// Gamma and (J_j - J_ref)^T Gamma keeps FP64 accuracy whatever |J_ref|.
+struct ListSrc {
+    const double* pos = nullptr;   // [3][ld]
+    int64_t ld = 0;
+    const float4* gb = nullptr;    // Gamma, b
+    const float2* ac = nullptr;    // a, cut
+    const double* jg = nullptr;    // [3][ld] (E_str only)
+};
+
+// E_str's zeta_sigma = b exp2(a r^2) (Gaussian-erf), b exp2(a r^3)
+// (Gaussian), b (1 + a r^2)^-5/2 (Winckelmans), and the r^2 past which it is
+// below FP32 resolution of the sum (q = 7, 3; none).
+template <int K>
+__device__ __forceinline__ void estr_params(double sg, double& a, double& b, double& cut)
+{
+    const double l2e = 1.4426950408889634;
+    const double si = 1.0 / sg, s2 = si * si, s3 = s2 * si;
+    if (K == VPM_KERNEL_GAUSSIANERF) { a = -0.5 * l2e * s2; b = kConst1 * s3; cut = 49.0 * sg * sg; }
+    else if (K == VPM_KERNEL_GAUSSIAN) { a = -l2e * s3; b = kConst3 * s3; cut = 9.0 * sg * sg; }
+    else { a = s2; b = kConst4 * 7.5 * s3; cut = (double)FLT_MAX; }
+}
+
+// MODE 0: zeta; 1: E_str with J^T Gamma; 2: E_str with J Gamma.
+template <int K, int MODE>
+__global__ __launch_bounds__(kBlock) void pack_list_src_kernel(const double* __restrict__ P, int64_t np,
+                                                               double* __restrict__ pos, int64_t ld,
+                                                               float4* __restrict__ gb, float2* __restrict__ ac,
+                                                               double* __restrict__ jg)
+{
+    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (j >= np) return;
+    const double* sr = P + j * NF;
+    pos[j] = sr[RX];
+    pos[ld + j] = sr[RX + 1];
+    pos[2 * ld + j] = sr[RX + 2];
+    const double sg = sr[RSIG];
+    double a, b;
+    float cut;
+    if constexpr (MODE == 0) {
+        zeta_params<K>(sg, a, b);
+        cut = zeta_cut<K>(sg);
+    } else {
+        double c;
+        estr_params<K>(sg, a, b, c);
+        cut = (float)c;
+    }
+    const float G0 = (float)sr[RG], G1 = (float)sr[RG + 1], G2 = (float)sr[RG + 2];
+    gb[j] = make_float4(G0, G1, G2, (float)b);
+    ac[j] = make_float2((float)a, cut);
+    if constexpr (MODE != 0) {
+        const double* J = sr + RJ;
+        const double g0 = G0, g1 = G1, g2 = G2;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            jg[c * ld + j] = MODE == 1 ? J[3 * c] * g0 + J[3 * c + 1] * g1 + J[3 * c + 2] * g2
+                                       : J[c] * g0 + J[3 + c] * g1 + J[6 + c] * g2;
+    }
+}
+
+// Packs the source records of the F32C E_str (mode 1, 2) and zeta (mode 0)
+// list kernels on the field's stream.
+int pack_list_src(vpm_field f, int kernel, int mode, ListSrc& out)
+{
+    const int64_t np = f->np, ld = f->cap;
+    const size_t gb = sizeof(float4) * (size_t)ld, ac = sizeof(float2) * (size_t)ld;
+    const size_t jg = mode ? sizeof(double) * 3 * (size_t)ld : 0;
+    if (int rc = ensure(f->pos, sizeof(double) * 3 * (size_t)ld)) return rc;
+    if (int rc = ensure(f->lrec, gb + ac + jg)) return rc;
+    char* base = (char*)f->lrec.p;
+    double* pos = (double*)f->pos.p;
+    float4* g4 = (float4*)base;
+    float2* a2 = (float2*)(base + gb);
+    double* j3 = mode ? (double*)(base + gb + ac) : nullptr;
+    out.pos = pos;
+    out.ld = ld;
+    out.gb = g4;
+    out.ac = a2;
+    out.jg = j3;
+    if (np == 0) return VPM_OK;
+    const dim3 grid((unsigned)((np + kBlock - 1) / kBlock));
+#define VPM_PK(KID, M) hipLaunchKernelGGL((pack_list_src_kernel<KID, M>), grid, dim3(kBlock), 0, f->stream, f->P, np, pos, ld, g4, a2, j3)
+#define VPM_PKK(KID)                         \
+    case KID:                                \
+        if (mode == 1) VPM_PK(KID, 1);       \
+        else if (mode == 2) VPM_PK(KID, 2);  \
+        else VPM_PK(KID, 0);                 \
+        break;
+    switch (kernel) {
+        VPM_PKK(VPM_KERNEL_GAUSSIAN) VPM_PKK(VPM_KERNEL_GAUSSIANERF) VPM_PKK(VPM_KERNEL_WINCKELMANS)
+    default: return fail(VPM_EINVAL, "unknown kernel id");
+    }
+#undef VPM_PKK
+#undef VPM_PK
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "pack_list_src kernel");
+    return VPM_OK;
+}
+
 // E_str over short target intervals (FMM leaves), as ranges_wave_f32c_kernel:
 // one wave per item of <= 64 / R targets, each target replicated over R lane
 // groups that split the item's source stream (its ranges in list order,
@@ -607,6 +713,7 @@ template <int K, int R>
 __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const RangeItem* __restrict__ items,
                                                                        int64_t nitems,
                                                                        const int2* __restrict__ lsrc,
+                                                                       const ListSrc S,
                                                                        double* __restrict__ P, int transposed)
 {
     static_assert(R == 1 || R == 2 || R == 4 || R == 8, "replication");
@@ -643,7 +750,6 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
     double A[6];
 #pragma unroll
     for (int c = 0; c < 6; ++c) { acc[c] = 0.f; A[c] = 0.0; }
-    const double l2e = 1.4426950408889634;
     int since = 0;
     int64_t l = it.l_begin, off = 0;
     while (l < it.l_end) {
@@ -665,32 +771,23 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
         // lanes read source j = 0 (a valid index) and select the padding
         float4 p, lo, q, gq;
         {
-            const double* sr = P + j * NF;
-            split_rel(sr[RX], ox, p.x, lo.x);
-            split_rel(sr[RX + 1], oy, p.y, lo.y);
-            split_rel(sr[RX + 2], oz, p.z, lo.z);
-            const double sg = sr[RSIG];
-            const double si = 1.0 / sg, s2 = si * si, s3 = s2 * si;
-            double a, b, cut;
-            if (K == VPM_KERNEL_GAUSSIANERF) { a = -0.5 * l2e * s2; b = kConst1 * s3; cut = 49.0 * sg * sg; }
-            else if (K == VPM_KERNEL_GAUSSIAN) { a = -l2e * s3; b = kConst3 * s3; cut = 9.0 * sg * sg; }
-            else { a = s2; b = kConst4 * 7.5 * s3; cut = (double)FLT_MAX; }
-            p.w = (float)cut;
+            split_rel(S.pos[j], ox, p.x, lo.x);
+            split_rel(S.pos[S.ld + j], oy, p.y, lo.y);
+            split_rel(S.pos[2 * S.ld + j], oz, p.z, lo.z);
+            gq = S.gb[j];
+            const float2 ac = S.ac[j];
+            p.w = ac.y;
             lo.w = __builtin_bit_cast(float, (int)j);   // the source, for the self-pair test
-            const double G0 = sr[RG], G1 = sr[RG + 1], G2 = sr[RG + 2];
-            double Jp[9];
+            // (J_j - J_ref)^T Gamma_j (or (J_j - J_ref) Gamma_j)
+            const double G0 = gq.x, G1 = gq.y, G2 = gq.z;
+            double q3[3];
 #pragma unroll
-            for (int c = 0; c < 9; ++c) Jp[c] = sr[RJ + c] - Jr[c];
-            if (transposed) {
-                q = make_float4((float)(Jp[0] * G0 + Jp[1] * G1 + Jp[2] * G2),
-                                (float)(Jp[3] * G0 + Jp[4] * G1 + Jp[5] * G2),
-                                (float)(Jp[6] * G0 + Jp[7] * G1 + Jp[8] * G2), (float)a);
-            } else {
-                q = make_float4((float)(Jp[0] * G0 + Jp[3] * G1 + Jp[6] * G2),
-                                (float)(Jp[1] * G0 + Jp[4] * G1 + Jp[7] * G2),
-                                (float)(Jp[2] * G0 + Jp[5] * G1 + Jp[8] * G2), (float)a);
+            for (int c = 0; c < 3; ++c) {
+                const double ref = transposed ? Jr[3 * c] * G0 + Jr[3 * c + 1] * G1 + Jr[3 * c + 2] * G2
+                                              : Jr[c] * G0 + Jr[3 + c] * G1 + Jr[6 + c] * G2;
+                q3[c] = S.jg[c * S.ld + j] - ref;
             }
-            gq = make_float4((float)G0, (float)G1, (float)G2, (float)b);
+            q = make_float4((float)q3[0], (float)q3[1], (float)q3[2], ac.x);
         }
         const bool real = lane < filled;
         const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -917,11 +1014,13 @@ int launch_estr_plan(vpm_field f, int kernel, int precision, int transposed, con
     const int2* d_src = dp.src;
     const dim3 grid((unsigned)dp.nitems);
     timing_begin(f);
+    ListSrc S;
     if (!f64 && dp.wave) {
+        if (int rc = pack_list_src(f, kernel, transposed ? 1 : 2, S)) return rc;
         const int64_t ni = dp.nitems;
         const dim3 wgrid((unsigned)((ni + kBlock / 64 - 1) / (kBlock / 64)));
         switch (kernel) {
-#define VPM_EWR(KID, R) hipLaunchKernelGGL((estr_ranges_wave_f32c_kernel<KID, R>), wgrid, dim3(kBlock), 0, f->stream, d_items, ni, d_src, f->P, transposed)
+#define VPM_EWR(KID, R) hipLaunchKernelGGL((estr_ranges_wave_f32c_kernel<KID, R>), wgrid, dim3(kBlock), 0, f->stream, d_items, ni, d_src, S, f->P, transposed)
 #define VPM_EW(KID)                                                                   \
     case KID:                                                                         \
         switch (dp.R) {                                                               \
@@ -979,6 +1078,7 @@ int launch_estr_ranges(vpm_field f, int kernel, int precision, int transposed, c
 template <int K, int BS>
 __global__ __launch_bounds__(BS) void zeta_ranges_f32c_kernel(const RangeItem* __restrict__ items,
                                                               const int2* __restrict__ lsrc,
+                                                              const ListSrc S,
                                                               double* __restrict__ P)
 {
     __shared__ float4 lp[BS];   // s'_hi, cut
@@ -1003,18 +1103,17 @@ __global__ __launch_bounds__(BS) void zeta_ranges_f32c_kernel(const RangeItem* _
             const int n = (int)((se - base) < BS ? (se - base) : BS);
             __syncthreads();
             if (tid < n) {
-                const double* sr = P + (base + tid) * NF;
+                const int64_t j = base + tid;
                 float4 p, lo;
-                split_rel(sr[RX], ox, p.x, lo.x);
-                split_rel(sr[RX + 1], oy, p.y, lo.y);
-                split_rel(sr[RX + 2], oz, p.z, lo.z);
-                double a, b;
-                zeta_params<K>(sr[RSIG], a, b);
-                p.w = zeta_cut<K>(sr[RSIG]);
-                lo.w = (float)a;
+                split_rel(S.pos[j], ox, p.x, lo.x);
+                split_rel(S.pos[S.ld + j], oy, p.y, lo.y);
+                split_rel(S.pos[2 * S.ld + j], oz, p.z, lo.z);
+                const float2 ac = S.ac[j];
+                p.w = ac.y;
+                lo.w = ac.x;
                 lp[tid] = p;
                 ll[tid] = lo;
-                lg[tid] = make_float4((float)sr[RG], (float)sr[RG + 1], (float)sr[RG + 2], (float)b);
+                lg[tid] = S.gb[j];
             }
             __syncthreads();
             for (int s0 = 0; s0 < n; s0 += 32) {
@@ -1069,6 +1168,7 @@ template <int K, int R>
 __global__ __launch_bounds__(kBlock) void zeta_ranges_wave_f32c_kernel(const RangeItem* __restrict__ items,
                                                                        int64_t nitems,
                                                                        const int2* __restrict__ lsrc,
+                                                                       const ListSrc S,
                                                                        double* __restrict__ P)
 {
     static_assert(R == 2 || R == 4 || R == 8, "replication");
@@ -1120,15 +1220,13 @@ __global__ __launch_bounds__(kBlock) void zeta_ranges_wave_f32c_kernel(const Ran
             // those lanes read source j = 0 (a valid index) and select the
             // padding
             float4 p, lo;
-            const double* sr = P + j * NF;
-            split_rel(sr[RX], ox, p.x, lo.x);
-            split_rel(sr[RX + 1], oy, p.y, lo.y);
-            split_rel(sr[RX + 2], oz, p.z, lo.z);
-            double a, b;
-            zeta_params<K>(sr[RSIG], a, b);
-            p.w = zeta_cut<K>(sr[RSIG]);
-            lo.w = (float)a;
-            const float4 gq = make_float4((float)sr[RG], (float)sr[RG + 1], (float)sr[RG + 2], (float)b);
+            split_rel(S.pos[j], ox, p.x, lo.x);
+            split_rel(S.pos[S.ld + j], oy, p.y, lo.y);
+            split_rel(S.pos[2 * S.ld + j], oz, p.z, lo.z);
+            const float2 ac = S.ac[j];
+            p.w = ac.y;
+            lo.w = ac.x;
+            const float4 gq = S.gb[j];
             const bool real = lane < filled;
             const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
             lp[lane] = sel4(real, p, make_float4(1e15f, 1e15f, 1e15f, 0.f));
@@ -1241,6 +1339,10 @@ int launch_zeta_plan(vpm_field f, int kernel, int precision, const DevPlan& dp)
     const int2* d_src = dp.src;
     const dim3 grid((unsigned)dp.nitems);
     timing_begin(f);
+    ListSrc S;
+    if (!f64) {
+        if (int rc = pack_list_src(f, kernel, 0, S)) return rc;
+    }
     if (f64) {
         switch (kernel) {
 #define VPM_Z64(KID) case KID: hipLaunchKernelGGL(zeta_ranges_f64_kernel<KID>, grid, dim3(kBlock), 0, f->stream, d_items, d_src, f->P); break;
@@ -1252,14 +1354,14 @@ int launch_zeta_plan(vpm_field f, int kernel, int precision, const DevPlan& dp)
         const int64_t ni = dp.nitems;
         const dim3 wgrid((unsigned)((ni + kBlock / 64 - 1) / (kBlock / 64)));
         switch (kernel) {
-#define VPM_ZWR(KID, R) hipLaunchKernelGGL((zeta_ranges_wave_f32c_kernel<KID, R>), wgrid, dim3(kBlock), 0, f->stream, d_items, ni, d_src, f->P)
+#define VPM_ZWR(KID, R) hipLaunchKernelGGL((zeta_ranges_wave_f32c_kernel<KID, R>), wgrid, dim3(kBlock), 0, f->stream, d_items, ni, d_src, S, f->P)
 #define VPM_ZW(KID)                                                                                                   \
     case KID:                                                                                                         \
         switch (dp.R) {                                                                                               \
         case 8: VPM_ZWR(KID, 8); break;                                                                               \
         case 4: VPM_ZWR(KID, 4); break;                                                                               \
         case 2: VPM_ZWR(KID, 2); break;                                                                               \
-        default: hipLaunchKernelGGL((zeta_ranges_f32c_kernel<KID, 64>), grid, dim3(64), 0, f->stream, d_items, d_src, f->P); break; \
+        default: hipLaunchKernelGGL((zeta_ranges_f32c_kernel<KID, 64>), grid, dim3(64), 0, f->stream, d_items, d_src, S, f->P); break; \
         }                                                                                                             \
         break;
             VPM_ZW(VPM_KERNEL_GAUSSIAN) VPM_ZW(VPM_KERNEL_GAUSSIANERF) VPM_ZW(VPM_KERNEL_WINCKELMANS)
@@ -1269,7 +1371,7 @@ int launch_zeta_plan(vpm_field f, int kernel, int precision, const DevPlan& dp)
         }
     } else {
         switch (kernel) {
-#define VPM_Z32(KID) case KID: hipLaunchKernelGGL((zeta_ranges_f32c_kernel<KID, kBlock>), grid, dim3(kBlock), 0, f->stream, d_items, d_src, f->P); break;
+#define VPM_Z32(KID) case KID: hipLaunchKernelGGL((zeta_ranges_f32c_kernel<KID, kBlock>), grid, dim3(kBlock), 0, f->stream, d_items, d_src, S, f->P); break;
             VPM_Z32(VPM_KERNEL_GAUSSIAN) VPM_Z32(VPM_KERNEL_GAUSSIANERF) VPM_Z32(VPM_KERNEL_WINCKELMANS)
 #undef VPM_Z32
         default: return fail(VPM_EINVAL, "unknown kernel id");

@@ -249,7 +249,8 @@ int vpm_destroy(vpm_field f)
         (void)hipSetDevice(f->device);
         (void)hipStreamSynchronize(f->stream);
         DevBuf* bufs[] = {&f->pos, &f->gs, &f->cut, &f->g64, &f->part, &f->keys, &f->keys2, &f->perm,
-                          &f->iota, &f->sorttmp, &f->bbox, &f->flag, &f->adj, &f->rng, &f->bar, &f->xfer};
+                          &f->iota, &f->sorttmp, &f->bbox, &f->flag, &f->adj, &f->rng, &f->bar, &f->xfer,
+                          &f->lrec};
         for (DevBuf* b : bufs) release(*b);
         if (f->hstage.p) (void)hipHostFree(f->hstage.p);
         f->hstage.p = nullptr;

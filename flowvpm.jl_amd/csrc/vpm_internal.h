@@ -81,6 +81,7 @@ struct vpm_field_s {
     vpm::DevBuf rng;                 // leaf-pair range lists
     vpm::DevBuf bar;                 // cotangent records of the step adjoint
     vpm::DevBuf xfer;                // gathered rows for partial downloads
+    vpm::DevBuf lrec;                // E_str / zeta list-kernel source records
     vpm::DevBuf hstage;              // pinned host staging of range plans (hipHostMalloc)
     bool profiling = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
