@@ -857,15 +857,20 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
                 }
             }
         } else {
-            // k R < filled <= 64 and 64 is a multiple of R, so sidx < 64
-            for (int k = 0; k * R < filled; ++k) {
-                const int sidx = k * R + g;
-                const float4 a = lp[sidx];
-                float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
-                float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                if (__any(r2 < a.w)) {
-                    const float4 c = lq[sidx];
-                    {
+            // groups of kEstrFold steps, each chain folded after its group:
+            // 64 / R is a multiple of kEstrFold, so with k0 R < filled <= 64
+            // every step k < k0 + kEstrFold has sidx < 64 (past `filled` the
+            // slot is padding, whose zero cut skips the step)
+            static_assert((64 / R) % kEstrFold == 0, "fold groups must tile the slot");
+            for (int k0 = 0; k0 * R < filled; k0 += kEstrFold) {
+#pragma unroll
+                for (int kk = 0; kk < kEstrFold; ++kk) {
+                    const int sidx = (k0 + kk) * R + g;
+                    const float4 a = lp[sidx];
+                    float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
+                    float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    if (__any(r2 < a.w)) {
+                        const float4 c = lq[sidx];
                         const float4 lo = ll[sidx];
                         const float4 gg = lg[sidx];
                         dx += tlx - lo.x;
@@ -893,12 +898,8 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
                         acc[5] = fmaf(z, c.z, acc[5]);
                     }
                 }
-                since = __builtin_amdgcn_readfirstlane(since + 1);   // wave-uniform
-                if (since == kEstrFold) {
-    #pragma unroll
-                    for (int c = 0; c < 6; ++c) { A[c] += (double)acc[c]; acc[c] = 0.f; }
-                    since = 0;
-                }
+#pragma unroll
+                for (int c = 0; c < 6; ++c) { A[c] += (double)acc[c]; acc[c] = 0.f; }
             }
         }
     }
@@ -1158,8 +1159,9 @@ __global__ __launch_bounds__(BS) void zeta_ranges_f32c_kernel(const RangeItem* _
 // estr_ranges_wave_f32c_kernel: one wave per item of <= 64 / R targets, each
 // target replicated over R lane groups that split the item's source stream
 // (its ranges in list order, staged 64 at a time through the wave's own LDS
-// slot).  FP32 chains of kZetaFold lane sources are added into FP64 sums,
-// which the R groups combine in a fixed butterfly.  The zeta sum has no
+// slot).  FP32 chains of kZetaFold lane sources (8 when R = 8, whose slot
+// holds 8 steps) are added into FP64 sums, which the R groups combine in a
+// fixed butterfly.  The zeta sum has no
 // cancelling J'^T A - B form, so its chains are longer than E_str's.  Leaves
 // of a whole wave (R = 1) take zeta_ranges_f32c_kernel<K, 64>, which was 22 %
 // faster there (6.35 against 8.13 ms on the 2^20 jet's 64-particle leaves).
@@ -1173,6 +1175,7 @@ __global__ __launch_bounds__(kBlock) void zeta_ranges_wave_f32c_kernel(const Ran
 {
     static_assert(R == 2 || R == 4 || R == 8, "replication");
     constexpr int TW = 64 / R;
+    constexpr int ZG = kZetaFold < 64 / R ? kZetaFold : 64 / R;   // steps per FP32 chain
     __shared__ float4 lp_all[kBlock / 64][64];   // s'_hi, cut
     __shared__ float4 lg_all[kBlock / 64][64];   // Gamma, b
     __shared__ float4 ll_all[kBlock / 64][64];   // s'_lo, a
@@ -1198,7 +1201,6 @@ __global__ __launch_bounds__(kBlock) void zeta_ranges_wave_f32c_kernel(const Ran
     split_rel(tr[RX + 2], oz, tz, tlz);
     float acc[3] = {0.f, 0.f, 0.f};
     double A[3] = {0.0, 0.0, 0.0};
-    int since = 0;
     int64_t l = it.l_begin, off = 0;
     while (l < it.l_end) {
         int filled = 0;
@@ -1236,14 +1238,18 @@ __global__ __launch_bounds__(kBlock) void zeta_ranges_wave_f32c_kernel(const Ran
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // k R < filled <= 64 and 64 is a multiple of R, so sidx < 64
-        for (int k = 0; k * R < filled; ++k) {
-            const int sidx = k * R + g;
-            const float4 a = lp[sidx];
-            float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
-            float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            if (__any(r2 < a.w)) {
-                {
+        // groups of ZG steps, each chain folded after its group: 64 / R is a
+        // multiple of ZG, so with k0 R < filled <= 64 every step k < k0 + ZG
+        // has sidx < 64 (past `filled` the slot is padding, whose zero cut
+        // skips the step)
+        for (int k0 = 0; k0 * R < filled; k0 += ZG) {
+#pragma unroll
+            for (int kk = 0; kk < ZG; ++kk) {
+                const int sidx = (k0 + kk) * R + g;
+                const float4 a = lp[sidx];
+                float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
+                float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                if (__any(r2 < a.w)) {
                     const float4 lo = ll[sidx];
                     const float4 gg = lg[sidx];
                     dx += tlx - lo.x;
@@ -1256,12 +1262,8 @@ __global__ __launch_bounds__(kBlock) void zeta_ranges_wave_f32c_kernel(const Ran
                     acc[2] = fmaf(gg.z, z, acc[2]);
                 }
             }
-            since = __builtin_amdgcn_readfirstlane(since + 1);   // wave-uniform
-            if (since == kZetaFold) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) { A[c] += (double)acc[c]; acc[c] = 0.f; }
-                since = 0;
-            }
+            for (int c = 0; c < 3; ++c) { A[c] += (double)acc[c]; acc[c] = 0.f; }
         }
     }
 #pragma unroll
