@@ -111,11 +111,12 @@ __global__ __launch_bounds__(kBlock) void ranges_f32c_kernel(const RangeItem* __
 // the far radius (then every pair has r > 0), near otherwise.  Per-lane FP32
 // sums fold into TwoSum pairs every ~64 pairs; at the end the R groups'
 // FP64 values are added in a fixed butterfly and group 0 writes.
-// R > 1: five waves per SIMD (95 VGPRs, no spills) instead of the four that
-// 107 VGPRs allow, 6 % faster on 16-particle leaves.  R = 1 would spill at
-// five and keeps four.
+// R > 1: six waves per SIMD (80 VGPRs; R = 4 and 8 spill 12 bytes per lane,
+// touched only around the per-chunk folds) instead of the five that 81-84
+// VGPRs allow, 3 % faster on 16-particle leaves.  R = 1 would spill at five
+// and keeps four.
 template <int K, int R>
-__global__ __launch_bounds__(kBlock, R > 1 ? 5 : 4) void ranges_wave_f32c_kernel(const RangeItem* __restrict__ items,
+__global__ __launch_bounds__(kBlock, R > 1 ? 6 : 4) void ranges_wave_f32c_kernel(const RangeItem* __restrict__ items,
                                                                   int64_t nitems,
                                                                   const int2* __restrict__ lsrc,
                                                                   const double* __restrict__ pos, int64_t ld,
