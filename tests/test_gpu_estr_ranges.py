@@ -80,11 +80,14 @@ def test_estr_ranges_whole_field_equals_estr_direct(vpm, orc):
 
 
 @pytest.mark.parametrize("leaf", [5, 12, 24, 48, "ragged"])
-@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans"])
-def test_estr_ranges_small_leaves(vpm, orc, leaf, kernel):
+@pytest.mark.parametrize("kernel,transposed", [("gaussianerf", True), ("winckelmans", True), ("gaussian", True),
+                                               ("gaussianerf", False)])
+def test_estr_ranges_small_leaves(vpm, orc, leaf, kernel, transposed):
     """FMM-style lists over leaves of 5, 12, 24 and 48 particles and of uneven
     size (3 to 40), with an empty source range per leaf: the wave kernel, its
-    lanes shared 8, 4, 2 or 1 ways, against the oracle."""
+    lanes shared 8, 4, 2 or 1 ways, against the oracle; every kernel with a
+    cut, and the classic (not transposed) form, whose packed source records
+    hold J Gamma instead of J^T Gamma."""
     n = 1000
     P = vpm.workloads.random_cloud(n, seed=41)
     P[:, 6] *= 3.0
@@ -106,9 +109,9 @@ def test_estr_ranges_small_leaves(vpm, orc, leaf, kernel):
                 tb.append(starts[i]); te.append(ends[i]); sb.append(starts[j]); se.append(ends[j])
         tb.append(starts[i]); te.append(ends[i]); sb.append(starts[i]); se.append(starts[i])
     ref = P.copy()
-    orc.estr_ranges(ref, kernel, tb, te, sb, se)
+    orc.estr_ranges(ref, kernel, tb, te, sb, se, transposed=transposed)
     A = P.copy()
-    vpm.api.estr_ranges(A, tb, te, sb, se, kernel=kernel, precision="f32c")
+    vpm.api.estr_ranges(A, tb, te, sb, se, kernel=kernel, precision="f32c", transposed=transposed)
     assert np.array_equal(A[:, 0:39], P[:, 0:39])
     _estr_check(vpm, orc, A[:, 39:42], ref[:, 39:42], P, kernel, (tb, te, sb, se),
-                "estr_ranges leaves %s %s" % (leaf, kernel))
+                "estr_ranges leaves %s %s transposed=%s" % (leaf, kernel, transposed), transposed=transposed)
