@@ -30,6 +30,62 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t G)
     return x * q + min(x, r) + (b >> 3);
 }
 
+// The next <= 64 sources of an item's source stream (its ranges in list
+// order, concatenated): lane m gets stream slot m in j (0 past `filled`),
+// and (l, off) -- the current range and the offset in it, wave-uniform --
+// move past them.  Up to 64 ranges are read at once, one per lane; an
+// inclusive wave scan of their lengths (each capped at 64) gives every
+// range its first slot B_i, and slot m belongs to the last range with
+// B_i <= m (B is non-decreasing, so a 6-step binary search over the lanes
+// finds it; ranges after that one start at or past its end, so an empty
+// range never owns a slot).  Returns filled, which is 0 only when the ranges
+// read were all empty (then l has moved past them).  The lane-shared kernels
+// use it at R = 8, where a 64-source chunk spans ~8 ranges of ~8 sources:
+// against the range-at-a-time walk (a dependent scalar load and ~30 scalar
+// instructions per range) it took the 2^20 jet's 8-particle leaf lists from
+// 1.58 to 1.46 ms (E_str) and 1.20 to 1.01 ms (zeta); at R = 4 (~4 ranges
+// per chunk) it was 3-5 % slower, so R < 8 keeps the walk.
+__device__ __forceinline__ int stream_fill(const int2* __restrict__ lsrc, int64_t& l, int64_t l_end, int64_t& off,
+                                           int lane, int64_t& j)
+{
+    const int nr = (int)min<int64_t>(64, l_end - l);   // wave-uniform, >= 1
+    int2 rs = make_int2(0, 0);
+    if (lane < nr) rs = lsrc[l + lane];
+    const int b = rs.x + (lane == 0 ? (int)off : 0);
+    const int len = lane < nr ? rs.y - b : 0;
+    const int lc = min(len, 64);
+    int S = lc;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int u = __shfl_up(S, d);
+        if (lane >= d) S += u;
+    }
+    const int B = S - lc;
+    const int total = __builtin_amdgcn_readlane(S, 63);
+    const int filled = min(total, 64);
+    int o = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1)
+        if (__shfl(B, o + step) <= lane) o += step;
+    const int Bo = __shfl(B, o), bo = __shfl(b, o);
+    j = lane < filled ? (int64_t)bo + (lane - Bo) : 0;
+    if (total < 64) {   // no length was capped: every range read is used up
+        l += nr;
+        off = 0;
+    } else {            // slot 63's range ends in this chunk or continues
+        const int o63 = __builtin_amdgcn_readlane(o, 63);
+        const int B63 = __builtin_amdgcn_readlane(B, o63);
+        if (B63 + __builtin_amdgcn_readlane(len, o63) == 64) {
+            l += o63 + 1;
+            off = 0;
+        } else {
+            off = (o63 == 0 ? off : 0) + (64 - B63);
+            l += o63;
+        }
+    }
+    return filled;
+}
+
 // Component-wise select of two float4 values (kept in registers: a ?: on
 // the vector type itself can go through private memory).
 __device__ __forceinline__ float4 sel4(bool c, float4 a, float4 b)
@@ -201,16 +257,21 @@ __global__ __launch_bounds__(kBlock, R > 1 ? 6 : 4) void ranges_wave_f32c_kernel
         // the next <= 64 sources of the stream; lane m takes stream slot m
         int filled = 0;
         int64_t j = 0;
-        while (filled < 64 && l < it.l_end) {
-            const int2 rs = lsrc[l];
-            const int64_t len = (int64_t)rs.y - rs.x;
-            const int take = (int)min<int64_t>(len - off, (int64_t)(64 - filled));
-            if (lane >= filled && lane < filled + take) j = rs.x + off + (lane - filled);
-            filled += take;
-            off += take;
-            if (off == len) { ++l; off = 0; }
+        if constexpr (R == 8) {
+            filled = stream_fill(lsrc, l, it.l_end, off, lane, j);
+            if (filled == 0) continue;          // only empty ranges were read
+        } else {
+            while (filled < 64 && l < it.l_end) {
+                const int2 rs = lsrc[l];
+                const int64_t len = (int64_t)rs.y - rs.x;
+                const int take = (int)min<int64_t>(len - off, (int64_t)(64 - filled));
+                if (lane >= filled && lane < filled + take) j = rs.x + off + (lane - filled);
+                filled += take;
+                off += take;
+                if (off == len) { ++l; off = 0; }
+            }
+            if (filled == 0) break;                 // only empty ranges were left
         }
-        if (filled == 0) break;                 // only empty ranges were left
         __builtin_amdgcn_wave_barrier();
         {
             // slots past `filled` hold a zero-strength source far away, so
@@ -756,16 +817,21 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
     while (l < it.l_end) {
         int filled = 0;
         int64_t j = 0;
-        while (filled < 64 && l < it.l_end) {
-            const int2 rs = lsrc[l];
-            const int64_t len = (int64_t)rs.y - rs.x;
-            const int take = (int)min<int64_t>(len - off, (int64_t)(64 - filled));
-            if (lane >= filled && lane < filled + take) j = rs.x + off + (lane - filled);
-            filled += take;
-            off += take;
-            if (off == len) { ++l; off = 0; }
+        if constexpr (R == 8) {
+            filled = stream_fill(lsrc, l, it.l_end, off, lane, j);
+            if (filled == 0) continue;          // only empty ranges were read
+        } else {
+            while (filled < 64 && l < it.l_end) {
+                const int2 rs = lsrc[l];
+                const int64_t len = (int64_t)rs.y - rs.x;
+                const int take = (int)min<int64_t>(len - off, (int64_t)(64 - filled));
+                if (lane >= filled && lane < filled + take) j = rs.x + off + (lane - filled);
+                filled += take;
+                off += take;
+                if (off == len) { ++l; off = 0; }
+            }
+            if (filled == 0) break;
         }
-        if (filled == 0) break;
         __builtin_amdgcn_wave_barrier();
         // slots past `filled`: a zero-strength source far away with a zero
         // cut (it never starts a step, and adds exact zeros to one); those
@@ -1206,16 +1272,21 @@ __global__ __launch_bounds__(kBlock) void zeta_ranges_wave_f32c_kernel(const Ran
     while (l < it.l_end) {
         int filled = 0;
         int64_t j = 0;
-        while (filled < 64 && l < it.l_end) {
-            const int2 rs = lsrc[l];
-            const int64_t len = (int64_t)rs.y - rs.x;
-            const int take = (int)min<int64_t>(len - off, (int64_t)(64 - filled));
-            if (lane >= filled && lane < filled + take) j = rs.x + off + (lane - filled);
-            filled += take;
-            off += take;
-            if (off == len) { ++l; off = 0; }
+        if constexpr (R == 8) {
+            filled = stream_fill(lsrc, l, it.l_end, off, lane, j);
+            if (filled == 0) continue;          // only empty ranges were read
+        } else {
+            while (filled < 64 && l < it.l_end) {
+                const int2 rs = lsrc[l];
+                const int64_t len = (int64_t)rs.y - rs.x;
+                const int take = (int)min<int64_t>(len - off, (int64_t)(64 - filled));
+                if (lane >= filled && lane < filled + take) j = rs.x + off + (lane - filled);
+                filled += take;
+                off += take;
+                if (off == len) { ++l; off = 0; }
+            }
+            if (filled == 0) break;
         }
-        if (filled == 0) break;
         __builtin_amdgcn_wave_barrier();
         {
             // slots past `filled`: a zero-strength source far away with a

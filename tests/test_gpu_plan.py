@@ -149,3 +149,69 @@ def test_plan_rejects_stale_and_foreign_fields(vpm):
     finally:
         fa.close()
         fb.close()
+
+
+@pytest.mark.parametrize("tleaf", [5, 8, 13])
+def test_plan_small_targets_long_and_empty_sources(vpm, orc, tleaf):
+    """Target leaves of 5, 8 and 13 particles (lanes shared 8 and 4 ways)
+    whose source streams mix ranges longer than a 64-source chunk, ranges of
+    one particle and empty ranges, so chunks end inside a range, after an
+    empty one and exactly at a range's end.  U/J, E_str on the J just
+    computed and zeta each match the oracle; the planned call equals the
+    per-call list bit for bit."""
+    api = vpm.api
+    n = 1200
+    P = vpm.workloads.random_cloud(n, seed=64)
+    P[:, 6] *= 3.0
+    rng = np.random.default_rng(65)
+    P[:, 39:42] = rng.normal(0, 1e-4, (n, 3))
+    srcs = [(0, 150), (300, 301), (400, 400), (500, 564), (564, 700), (700, 701), (701, 701), (900, 1000)]
+    tb, te, sb, se = [], [], [], []
+    for a in range(0, n, tleaf):
+        b = min(a + tleaf, n)
+        k = (a // tleaf) % len(srcs)
+        for (c, d) in srcs[k:] + srcs[:k]:          # a different start per leaf
+            tb.append(a); te.append(b); sb.append(c); se.append(d)
+        tb.append(a); te.append(b); sb.append(a); se.append(b)
+    lists = [np.array(x, dtype=np.int64) for x in (tb, te, sb, se)]
+    kern = "gaussianerf"
+    _say("small targets leaf=%d pairs=%d" % (tleaf, len(tb)))
+    fa, fb = api.Field(n), api.Field(n)
+    try:
+        fa.set_particles(P)
+        fb.set_particles(P)
+        plan = fa.plan_ranges(*lists)
+        plan.p2p(kern, "f32c")
+        _per_call(vpm, fb, "p2p", lists, kern, "f32c")
+        A, B = np.zeros_like(P), np.zeros_like(P)
+        fa.get_particles(A)
+        fb.get_particles(B)
+        assert np.array_equal(A, B)
+        ref = P.copy()
+        orc.direct_ranges(ref, kern, *lists)
+        eu = vpm.metrics.err_stats(A[:, 9:12], ref[:, 9:12])
+        ej = vpm.metrics.err_stats(A[:, 15:24], ref[:, 15:24])
+        assert eu["max"] < F32_TOL and ej["max"] < F32_TOL, (eu, ej)
+        plan.estr(kern, "f32c", True)
+        fa.get_particles(A)
+        ref2 = ref.copy()
+        ref2[:, 9:24] = A[:, 9:24]
+        orc.estr_ranges(ref2, kern, *lists)
+        scale = orc.estr_scale_ranges(ref2, kern, *lists)
+        ratio = np.linalg.norm(A[:, 39:42] - ref2[:, 39:42], axis=1) / np.maximum(scale, 1e-6 * scale.max())
+        st = vpm.metrics.err_stats(A[:, 39:42], ref2[:, 39:42])
+        assert ratio.max() < ESTR_SCALE_TOL and st["rms"] < ESTR_RMS_TOL, (ratio.max(), st)
+        R = A.copy()
+        orc.zeta_ranges(R, kern, *lists)
+        S = A.copy()
+        S[:, 3:6] = np.abs(S[:, 3:6])
+        S[:, 15:18] = np.abs(S[:, 15:18])
+        orc.zeta_ranges(S, kern, *lists)
+        plan.zeta(kern, "f32c")
+        fa.get_particles(A)
+        scaled = (np.abs(A[:, 15:18] - R[:, 15:18]) / np.maximum(S[:, 15:18], 1e-300)).max()
+        assert scaled < 1e-5, scaled
+        plan.close()
+    finally:
+        fa.close()
+        fb.close()
