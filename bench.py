@@ -146,7 +146,7 @@ def rollout_adjoint(vpm, P, kernel, nsteps=3, dt=0.005):
 
 def leaf_list(vpm, P, kernel, allpairs_rate, leaf=64, reps=3):
     """The near-field service of an FMM (FastMultipole's direct list through
-    fmm.direct!, src/FLOWVPM_fmm.jl:102-168) on the jet: leaves of 64
+    fmm.direct!, src/FLOWVPM_fmm.jl:102-168) on the jet: leaves of `leaf`
     Morton-consecutive particles, direct pairs within 3 median leaf radii,
     one vpm_p2p_ranges call per evaluation.  Then Estr_fmm! over the same
     list (src/FLOWVPM_subfilterscale_models.jl:157-188), one vpm_estr_ranges
@@ -615,6 +615,9 @@ def main():
     if not args.no_leaf:
         res["fmm_leaf_list"] = leaf_list(vpm, P, kernel, value)
         res["fmm_leaf_list_16"] = leaf_list(vpm, P, kernel, value, leaf=16)
+        # FMM's default ncrit = 10 (src/FLOWVPM_particlefield.jl:60) gives
+        # leaves of at most 10 particles
+        res["fmm_leaf_list_8"] = leaf_list(vpm, P, kernel, value, leaf=8)
 
     if not args.no_reset:
         res["core_spreading_reset"] = core_spreading_reset(vpm, P, kernel)
