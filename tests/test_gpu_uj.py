@@ -345,6 +345,32 @@ def test_p2p_ranges_leaf_lists(vpm, orc, leaf, precision):
         assert eu["max"] < tol and ej["max"] < tol, (leaf, eu, ej)
 
 
+@pytest.mark.parametrize("leaf", [5, 16])
+@pytest.mark.parametrize("kernel", ["gaussian", "winckelmans", "singular"])
+def test_p2p_ranges_small_leaves_every_kernel(vpm, orc, leaf, kernel):
+    """The lane-shared list kernels (8 and 4 ways) for the kernels other than
+    Gaussian-erf, F32C against the oracle: the near/far step choice, the
+    padded slots and the stream reads do not depend on the kernel's form."""
+    n = 1500
+    P = vpm.workloads.random_cloud(n, seed=27)
+    P[:, 6] *= 3.0
+    starts = np.arange(0, n, leaf)
+    ends = np.minimum(starts + leaf, n)
+    C = np.array([P[a:b, 0:3].mean(0) for a, b in zip(starts, ends)])
+    tb, te, sb, se = [], [], [], []
+    for i in range(len(starts)):
+        for j in range(len(starts)):
+            if np.linalg.norm(C[i] - C[j]) < 0.35:
+                tb.append(starts[i]); te.append(ends[i]); sb.append(starts[j]); se.append(ends[j])
+    ref = P.copy()
+    orc.direct_ranges(ref, kernel, tb, te, sb, se)
+    A = P.copy()
+    vpm.p2p_ranges(A, tb, te, sb, se, kernel=kernel, precision="f32c")
+    eu, ej = _uj_err(vpm, A, ref)
+    _say("p2p_ranges leaves", leaf, kernel, "U", eu, "J", ej)
+    assert eu["max"] < F32_TOL and ej["max"] < F32_TOL, (leaf, kernel, eu, ej)
+
+
 @pytest.mark.parametrize("precision", ["f32c", "f64"])
 def test_p2p_ranges_ragged_leaves(vpm, orc, precision):
     """Leaves of uneven size (3 to 40 particles), lists with empty source
