@@ -655,16 +655,18 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_f32c_kernel(const RangeIte
 // that lists it (about 140 times on FMM leaf lists); gathering 48 (zeta) or
 // 72 (E_str) contiguous bytes per source instead of parts of its 368-byte
 // particle record cuts that traffic threefold.  Per particle: the position
-// (FP64, SoA, stride ld), (Gamma, b) and (a, cut) in FP32, and for E_str
-// J^T Gamma (transposed) or J Gamma in FP64, formed with Gamma already
-// rounded to FP32, so that the kernel's J_ref^T Gamma subtracts the same
-// Gamma and (J_j - J_ref)^T Gamma keeps FP64 accuracy whatever |J_ref|.
+// relative to particle 0 as an FP32 hi + lo pair, with the cut and a beside
+// it, (Gamma, b) in FP32, and for E_str J^T Gamma (transposed) or J Gamma
+// in FP64, formed with Gamma already rounded to FP32, so that the kernel's
+// J_ref^T Gamma subtracts the same Gamma and (J_j - J_ref)^T Gamma keeps
+// FP64 accuracy whatever |J_ref|.  With the positions split once, staging
+// a source (or a target) is a copy: no FP64 work per item.
 struct ListSrc {
-    const double* pos = nullptr;   // [3][ld]
-    int64_t ld = 0;
+    const float4* hi = nullptr;    // x - x_0 in FP32 (hi part), cut
+    const float4* lo = nullptr;    // the FP32 remainder of x - x_0 (lo part), a
     const float4* gb = nullptr;    // Gamma, b
-    const float2* ac = nullptr;    // a, cut
     const double* jg = nullptr;    // [3][ld] (E_str only)
+    int64_t ld = 0;
 };
 
 // E_str's zeta_sigma = b exp2(a r^2) (Gaussian-erf), b exp2(a r^3)
@@ -683,16 +685,13 @@ __device__ __forceinline__ void estr_params(double sg, double& a, double& b, dou
 // MODE 0: zeta; 1: E_str with J^T Gamma; 2: E_str with J Gamma.
 template <int K, int MODE>
 __global__ __launch_bounds__(kBlock) void pack_list_src_kernel(const double* __restrict__ P, int64_t np,
-                                                               double* __restrict__ pos, int64_t ld,
-                                                               float4* __restrict__ gb, float2* __restrict__ ac,
-                                                               double* __restrict__ jg)
+                                                               float4* __restrict__ hi, float4* __restrict__ lo,
+                                                               float4* __restrict__ gb, double* __restrict__ jg,
+                                                               int64_t ld)
 {
     const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (j >= np) return;
     const double* sr = P + j * NF;
-    pos[j] = sr[RX];
-    pos[ld + j] = sr[RX + 1];
-    pos[2 * ld + j] = sr[RX + 2];
     const double sg = sr[RSIG];
     double a, b;
     float cut;
@@ -704,9 +703,21 @@ __global__ __launch_bounds__(kBlock) void pack_list_src_kernel(const double* __r
         estr_params<K>(sg, a, b, c);
         cut = (float)c;
     }
+    // positions relative to particle 0, split into FP32 hi + lo: for a pair
+    // (t, s) the difference of the hi parts is exact whenever |t - s| is
+    // below the hi parts themselves (Sterbenz), so d = (t_hi - s_hi) +
+    // (t_lo - s_lo) keeps FP32 relative accuracy at any distance, as with
+    // an origin per item
+    float4 h, l;
+    split_rel(sr[RX], P[RX], h.x, l.x);
+    split_rel(sr[RX + 1], P[RX + 1], h.y, l.y);
+    split_rel(sr[RX + 2], P[RX + 2], h.z, l.z);
+    h.w = cut;
+    l.w = (float)a;
+    hi[j] = h;
+    lo[j] = l;
     const float G0 = (float)sr[RG], G1 = (float)sr[RG + 1], G2 = (float)sr[RG + 2];
     gb[j] = make_float4(G0, G1, G2, (float)b);
-    ac[j] = make_float2((float)a, cut);
     if constexpr (MODE != 0) {
         const double* J = sr + RJ;
         const double g0 = G0, g1 = G1, g2 = G2;
@@ -722,23 +733,22 @@ __global__ __launch_bounds__(kBlock) void pack_list_src_kernel(const double* __r
 int pack_list_src(vpm_field f, int kernel, int mode, ListSrc& out)
 {
     const int64_t np = f->np, ld = f->cap;
-    const size_t gb = sizeof(float4) * (size_t)ld, ac = sizeof(float2) * (size_t)ld;
+    const size_t f4 = sizeof(float4) * (size_t)ld;
     const size_t jg = mode ? sizeof(double) * 3 * (size_t)ld : 0;
-    if (int rc = ensure(f->pos, sizeof(double) * 3 * (size_t)ld)) return rc;
-    if (int rc = ensure(f->lrec, gb + ac + jg)) return rc;
+    if (int rc = ensure(f->lrec, 3 * f4 + jg)) return rc;
     char* base = (char*)f->lrec.p;
-    double* pos = (double*)f->pos.p;
-    float4* g4 = (float4*)base;
-    float2* a2 = (float2*)(base + gb);
-    double* j3 = mode ? (double*)(base + gb + ac) : nullptr;
-    out.pos = pos;
-    out.ld = ld;
+    float4* h4 = (float4*)base;
+    float4* l4 = (float4*)(base + f4);
+    float4* g4 = (float4*)(base + 2 * f4);
+    double* j3 = mode ? (double*)(base + 3 * f4) : nullptr;
+    out.hi = h4;
+    out.lo = l4;
     out.gb = g4;
-    out.ac = a2;
     out.jg = j3;
+    out.ld = ld;
     if (np == 0) return VPM_OK;
     const dim3 grid((unsigned)((np + kBlock - 1) / kBlock));
-#define VPM_PK(KID, M) hipLaunchKernelGGL((pack_list_src_kernel<KID, M>), grid, dim3(kBlock), 0, f->stream, f->P, np, pos, ld, g4, a2, j3)
+#define VPM_PK(KID, M) hipLaunchKernelGGL((pack_list_src_kernel<KID, M>), grid, dim3(kBlock), 0, f->stream, f->P, np, h4, l4, g4, j3, ld)
 #define VPM_PKK(KID)                         \
     case KID:                                \
         if (mode == 1) VPM_PK(KID, 1);       \
@@ -796,18 +806,15 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
     float4* ll = ll_all[wave];
     const RangeItem it = items[w];
     const double* r0 = P + it.t_begin * NF;
-    const double ox = r0[RX], oy = r0[RX + 1], oz = r0[RX + 2];
     double Jr[9];
 #pragma unroll
     for (int c = 0; c < 9; ++c) Jr[c] = r0[RJ + c];
     const int g = lane / TW;
     const int64_t t = it.t_begin + (lane % TW);
     const bool active = t < it.t_end;
-    const double* tr = P + (active ? t : it.t_begin) * NF;
-    float tx, ty, tz, tlx, tly, tlz;
-    split_rel(tr[RX], ox, tx, tlx);
-    split_rel(tr[RX + 1], oy, ty, tly);
-    split_rel(tr[RX + 2], oz, tz, tlz);
+    const int64_t tt = active ? t : it.t_begin;
+    const float4 th = S.hi[tt], tl = S.lo[tt];
+    const float tx = th.x, ty = th.y, tz = th.z, tlx = tl.x, tly = tl.y, tlz = tl.z;
     float acc[6];
     double A[6];
 #pragma unroll
@@ -838,12 +845,10 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
         // lanes read source j = 0 (a valid index) and select the padding
         float4 p, lo, q, gq;
         {
-            split_rel(S.pos[j], ox, p.x, lo.x);
-            split_rel(S.pos[S.ld + j], oy, p.y, lo.y);
-            split_rel(S.pos[2 * S.ld + j], oz, p.z, lo.z);
+            p = S.hi[j];                                // s_hi, cut
+            lo = S.lo[j];                               // s_lo, a
             gq = S.gb[j];
-            const float2 ac = S.ac[j];
-            p.w = ac.y;
+            const float a = lo.w;
             lo.w = __builtin_bit_cast(float, (int)j);   // the source, for the self-pair test
             // (J_j - J_ref)^T Gamma_j (or (J_j - J_ref) Gamma_j)
             const double G0 = gq.x, G1 = gq.y, G2 = gq.z;
@@ -854,7 +859,7 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
                                               : Jr[c] * G0 + Jr[3 + c] * G1 + Jr[6 + c] * G2;
                 q3[c] = S.jg[c * S.ld + j] - ref;
             }
-            q = make_float4((float)q3[0], (float)q3[1], (float)q3[2], ac.x);
+            q = make_float4((float)q3[0], (float)q3[1], (float)q3[2], a);
         }
         const bool real = lane < filled;
         const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1154,15 +1159,11 @@ __global__ __launch_bounds__(BS) void zeta_ranges_f32c_kernel(const RangeItem* _
     __shared__ float4 ll[BS];   // s'_lo, a
     const RangeItem it = items[xcd_block(blockIdx.x, gridDim.x)];
     const int tid = threadIdx.x;
-    const double* r0 = P + it.t_begin * NF;
-    const double ox = r0[RX], oy = r0[RX + 1], oz = r0[RX + 2];
     const int64_t t = it.t_begin + tid;
     const bool active = t < it.t_end;
-    const double* tr = P + (active ? t : it.t_begin) * NF;
-    float tx, ty, tz, tlx, tly, tlz;
-    split_rel(tr[RX], ox, tx, tlx);
-    split_rel(tr[RX + 1], oy, ty, tly);
-    split_rel(tr[RX + 2], oz, tz, tlz);
+    const int64_t tt = active ? t : it.t_begin;
+    const float4 th = S.hi[tt], tl = S.lo[tt];
+    const float tx = th.x, ty = th.y, tz = th.z, tlx = tl.x, tly = tl.y, tlz = tl.z;
     float sum[3] = {0.f, 0.f, 0.f}, err[3] = {0.f, 0.f, 0.f};
     for (int64_t l = it.l_begin; l < it.l_end; ++l) {
         const int2 rs = lsrc[l];
@@ -1172,15 +1173,8 @@ __global__ __launch_bounds__(BS) void zeta_ranges_f32c_kernel(const RangeItem* _
             __syncthreads();
             if (tid < n) {
                 const int64_t j = base + tid;
-                float4 p, lo;
-                split_rel(S.pos[j], ox, p.x, lo.x);
-                split_rel(S.pos[S.ld + j], oy, p.y, lo.y);
-                split_rel(S.pos[2 * S.ld + j], oz, p.z, lo.z);
-                const float2 ac = S.ac[j];
-                p.w = ac.y;
-                lo.w = ac.x;
-                lp[tid] = p;
-                ll[tid] = lo;
+                lp[tid] = S.hi[j];   // s_hi, cut
+                ll[tid] = S.lo[j];   // s_lo, a
                 lg[tid] = S.gb[j];
             }
             __syncthreads();
@@ -1256,16 +1250,12 @@ __global__ __launch_bounds__(kBlock) void zeta_ranges_wave_f32c_kernel(const Ran
     float4* lg = lg_all[wave];
     float4* ll = ll_all[wave];
     const RangeItem it = items[w];
-    const double* r0 = P + it.t_begin * NF;
-    const double ox = r0[RX], oy = r0[RX + 1], oz = r0[RX + 2];
     const int g = lane / TW;
     const int64_t t = it.t_begin + (lane % TW);
     const bool active = t < it.t_end;
-    const double* tr = P + (active ? t : it.t_begin) * NF;
-    float tx, ty, tz, tlx, tly, tlz;
-    split_rel(tr[RX], ox, tx, tlx);
-    split_rel(tr[RX + 1], oy, ty, tly);
-    split_rel(tr[RX + 2], oz, tz, tlz);
+    const int64_t tt = active ? t : it.t_begin;
+    const float4 th = S.hi[tt], tl = S.lo[tt];
+    const float tx = th.x, ty = th.y, tz = th.z, tlx = tl.x, tly = tl.y, tlz = tl.z;
     float acc[3] = {0.f, 0.f, 0.f};
     double A[3] = {0.0, 0.0, 0.0};
     int64_t l = it.l_begin, off = 0;
@@ -1293,13 +1283,8 @@ __global__ __launch_bounds__(kBlock) void zeta_ranges_wave_f32c_kernel(const Ran
             // zero cut (it never starts a step, and adds exact zeros to one);
             // those lanes read source j = 0 (a valid index) and select the
             // padding
-            float4 p, lo;
-            split_rel(S.pos[j], ox, p.x, lo.x);
-            split_rel(S.pos[S.ld + j], oy, p.y, lo.y);
-            split_rel(S.pos[2 * S.ld + j], oz, p.z, lo.z);
-            const float2 ac = S.ac[j];
-            p.w = ac.y;
-            lo.w = ac.x;
+            const float4 p = S.hi[j];    // s_hi, cut
+            const float4 lo = S.lo[j];   // s_lo, a
             const float4 gq = S.gb[j];
             const bool real = lane < filled;
             const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
