@@ -30,23 +30,59 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t G)
     return x * q + min(x, r) + (b >> 3);
 }
 
+// Wave-wide inclusive scans over the 64 lanes by DPP: row shifts by 1, 2, 4
+// and 8 within each 16-lane row, then the last lane of row 0 and of rows 0-1
+// broadcast into the rows above (row_bcast:15 into rows 1 and 3, row_bcast:31
+// into rows 2 and 3).  No LDS round trip; lanes a shift leaves without a
+// source take 0, which neither sum nor max (of values >= 0) sees.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_src(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
+}
+
+__device__ __forceinline__ int wave_scan_add(int v)
+{
+    v += dpp_src<0x111, 0xf>(v);
+    v += dpp_src<0x112, 0xf>(v);
+    v += dpp_src<0x114, 0xf>(v);
+    v += dpp_src<0x118, 0xf>(v);
+    v += dpp_src<0x142, 0xa>(v);
+    v += dpp_src<0x143, 0xc>(v);
+    return v;
+}
+
+__device__ __forceinline__ int wave_scan_max(int v)   // v >= 0
+{
+    v = max(v, dpp_src<0x111, 0xf>(v));
+    v = max(v, dpp_src<0x112, 0xf>(v));
+    v = max(v, dpp_src<0x114, 0xf>(v));
+    v = max(v, dpp_src<0x118, 0xf>(v));
+    v = max(v, dpp_src<0x142, 0xa>(v));
+    v = max(v, dpp_src<0x143, 0xc>(v));
+    return v;
+}
+
 // The next <= 64 sources of an item's source stream (its ranges in list
 // order, concatenated): lane m gets stream slot m in j (0 past `filled`),
 // and (l, off) -- the current range and the offset in it, wave-uniform --
 // move past them.  Up to 64 ranges are read at once, one per lane; an
 // inclusive wave scan of their lengths (each capped at 64) gives every
-// range its first slot B_i, and slot m belongs to the last range with
-// B_i <= m (B is non-decreasing, so a 6-step binary search over the lanes
-// finds it; ranges after that one start at or past its end, so an empty
-// range never owns a slot).  Returns filled, which is 0 only when the ranges
-// read were all empty (then l has moved past them).  The lane-shared kernels
-// use it at R = 8, where a 64-source chunk spans ~8 ranges of ~8 sources:
-// against the range-at-a-time walk (a dependent scalar load and ~30 scalar
-// instructions per range) it took the 2^20 jet's 8-particle leaf lists from
-// 1.58 to 1.46 ms (E_str) and 1.20 to 1.01 ms (zeta); at R = 4 (~4 ranges
-// per chunk) it was 3-5 % slower, so R < 8 keeps the walk.
+// range its first slot B_i.  Slot m belongs to the last range with B_i <=
+// m: each nonempty range marks its first slot with its index (an LDS max,
+// so of the ranges that start at one slot the last, nonempty one wins) and
+// a max-scan over the slots carries the marks forward.  `mark` is 64 ints
+// of the wave's own LDS, free at the call.  Returns filled, which is 0 only
+// when the ranges read were all empty (then l has moved past them).  The
+// lane-shared kernels use it at R = 8, where a 64-source chunk spans ~8
+// ranges of ~8 sources: against the range-at-a-time walk (a dependent scalar
+// load and ~30 scalar instructions per range) it took the 2^20 jet's
+// 8-particle leaf lists from 1.58 to 1.44 ms (E_str) and 1.20 to 1.01 ms
+// (zeta) with LDS shuffles for the scan and a binary search for the owner;
+// at R = 4 (~4 ranges per chunk) that form was 3-5 % slower, so R < 8
+// keeps the walk.
 __device__ __forceinline__ int stream_fill(const int2* __restrict__ lsrc, int64_t& l, int64_t l_end, int64_t& off,
-                                           int lane, int64_t& j)
+                                           int lane, int64_t& j, int* mark)
 {
     const int nr = (int)min<int64_t>(64, l_end - l);   // wave-uniform, >= 1
     int2 rs = make_int2(0, 0);
@@ -54,19 +90,24 @@ __device__ __forceinline__ int stream_fill(const int2* __restrict__ lsrc, int64_
     const int b = rs.x + (lane == 0 ? (int)off : 0);
     const int len = lane < nr ? rs.y - b : 0;
     const int lc = min(len, 64);
-    int S = lc;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int u = __shfl_up(S, d);
-        if (lane >= d) S += u;
-    }
+    const int S = wave_scan_add(lc);
     const int B = S - lc;
     const int total = __builtin_amdgcn_readlane(S, 63);
     const int filled = min(total, 64);
-    int o = 0;
-#pragma unroll
-    for (int step = 32; step > 0; step >>= 1)
-        if (__shfl(B, o + step) <= lane) o += step;
+    // `mark` aliases the staging rows the previous steps read, under
+    // another type: wave-scope fences keep every access to it in order
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    mark[lane] = 0;
+    if (lc > 0 && B < 64) atomicMax(&mark[B], lane);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int o = wave_scan_max(mark[lane]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const int Bo = __shfl(B, o), bo = __shfl(b, o);
     j = lane < filled ? (int64_t)bo + (lane - Bo) : 0;
     if (total < 64) {   // no length was capped: every range read is used up
@@ -258,7 +299,7 @@ __global__ __launch_bounds__(kBlock, R > 1 ? 6 : 4) void ranges_wave_f32c_kernel
         int filled = 0;
         int64_t j = 0;
         if constexpr (R == 8) {
-            filled = stream_fill(lsrc, l, it.l_end, off, lane, j);
+            filled = stream_fill(lsrc, l, it.l_end, off, lane, j, (int*)lpos);
             if (filled == 0) continue;          // only empty ranges were read
         } else {
             while (filled < 64 && l < it.l_end) {
@@ -825,7 +866,7 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_wave_f32c_kernel(const Ran
         int filled = 0;
         int64_t j = 0;
         if constexpr (R == 8) {
-            filled = stream_fill(lsrc, l, it.l_end, off, lane, j);
+            filled = stream_fill(lsrc, l, it.l_end, off, lane, j, (int*)lp);
             if (filled == 0) continue;          // only empty ranges were read
         } else {
             while (filled < 64 && l < it.l_end) {
@@ -1263,7 +1304,7 @@ __global__ __launch_bounds__(kBlock) void zeta_ranges_wave_f32c_kernel(const Ran
         int filled = 0;
         int64_t j = 0;
         if constexpr (R == 8) {
-            filled = stream_fill(lsrc, l, it.l_end, off, lane, j);
+            filled = stream_fill(lsrc, l, it.l_end, off, lane, j, (int*)lp);
             if (filled == 0) continue;          // only empty ranges were read
         } else {
             while (filled < 64 && l < it.l_end) {
