@@ -77,10 +77,9 @@ __device__ __forceinline__ int wave_scan_max(int v)   // v >= 0
 // lane-shared kernels use it at R = 8, where a 64-source chunk spans ~8
 // ranges of ~8 sources: against the range-at-a-time walk (a dependent scalar
 // load and ~30 scalar instructions per range) it took the 2^20 jet's
-// 8-particle leaf lists from 1.58 to 1.44 ms (E_str) and 1.20 to 1.01 ms
-// (zeta) with LDS shuffles for the scan and a binary search for the owner;
-// at R = 4 (~4 ranges per chunk) that form was 3-5 % slower, so R < 8
-// keeps the walk.
+// 8-particle leaf lists from 1.58 to 1.22 ms (E_str), 1.20 to 0.88 ms
+// (zeta) and 2.33 to 2.20 ms (U/J); at R = 4 (~4 ranges per chunk) U/J was
+// 3 % slower with it, so R < 8 keeps the walk.
 __device__ __forceinline__ int stream_fill(const int2* __restrict__ lsrc, int64_t& l, int64_t l_end, int64_t& off,
                                            int lane, int64_t& j, int* mark)
 {
