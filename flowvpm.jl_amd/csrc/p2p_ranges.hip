@@ -1277,6 +1277,7 @@ __global__ __launch_bounds__(kBlock) void zeta_ranges_wave_f32c_kernel(const Ran
     static_assert(R == 2 || R == 4 || R == 8, "replication");
     constexpr int TW = 64 / R;
     constexpr int ZG = kZetaFold < 64 / R ? kZetaFold : 64 / R;   // steps per FP32 chain
+    static_assert((64 / R) % ZG == 0, "chain groups must tile the slot");
     __shared__ float4 lp_all[kBlock / 64][64];   // s'_hi, cut
     __shared__ float4 lg_all[kBlock / 64][64];   // Gamma, b
     __shared__ float4 ll_all[kBlock / 64][64];   // s'_lo, a
