@@ -549,147 +549,6 @@ int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb,
     return launch_p2p_plan(f, kernel, precision, dp);
 }
 
-// ------------------------------------------------------------ E_str ranges
-// Estr_fmm! over a direct list (src/FLOWVPM_subfilterscale_models.jl:157-188):
-// E_str pairs (:16-41) for every (target range, source range), no static skip,
-// accumulated onto SFS.
-//
-// F32C: within a block (<= 256 targets of one interval), J' = J - J_ref with
-// J_ref the J of the block's first target (FP64), and
-//     sum_j zeta_j (J'_i - J'_j)^T Gamma_j = J'_i^T A - B,
-//     A = sum zeta Gamma, B = sum zeta J'_j^T Gamma_j,
-// positions hi/lo relative to that target, sources beyond q_far skipped per
-// wave (Gaussian-erf, Gaussian), sums folded into TwoSum every 32 sources.
-template <int K>
-__global__ __launch_bounds__(kBlock) void estr_ranges_f32c_kernel(const RangeItem* __restrict__ items,
-                                                                  const int2* __restrict__ lsrc,
-                                                                  double* __restrict__ P, int transposed)
-{
-    __shared__ float4 lp[kTile];   // s'_hi, a
-    __shared__ float4 lg[kTile];   // Gamma, b
-    __shared__ float4 lq[kTile];   // J'^T Gamma (or J' Gamma), cut
-    __shared__ float4 ll[kTile];   // s'_lo
-    const RangeItem it = items[xcd_block(blockIdx.x, gridDim.x)];
-    const int tid = threadIdx.x;
-    const double* r0 = P + it.t_begin * NF;
-    const double ox = r0[RX], oy = r0[RX + 1], oz = r0[RX + 2];
-    double Jr[9];
-#pragma unroll
-    for (int c = 0; c < 9; ++c) Jr[c] = r0[RJ + c];
-    const int64_t t = it.t_begin + tid;
-    const bool active = t < it.t_end;
-    const double* tr = P + (active ? t : it.t_begin) * NF;
-    float tx, ty, tz, tlx, tly, tlz;
-    split_rel(tr[RX], ox, tx, tlx);
-    split_rel(tr[RX + 1], oy, ty, tly);
-    split_rel(tr[RX + 2], oz, tz, tlz);
-    float sum[6], err[6];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) { sum[c] = 0.f; err[c] = 0.f; }
-    const double l2e = 1.4426950408889634;
-    for (int64_t l = it.l_begin; l < it.l_end; ++l) {
-        const int2 rs = lsrc[l];
-        const int64_t sb = rs.x, se = rs.y;
-        for (int64_t base = sb; base < se; base += kTile) {
-            const int n = (int)((se - base) < kTile ? (se - base) : kTile);
-            __syncthreads();
-            if (tid < n) {
-                const double* sr = P + (base + tid) * NF;
-                float4 p, lo;
-                split_rel(sr[RX], ox, p.x, lo.x);
-                split_rel(sr[RX + 1], oy, p.y, lo.y);
-                split_rel(sr[RX + 2], oz, p.z, lo.z);
-                const double sg = sr[RSIG];
-                const double si = 1.0 / sg, s2 = si * si, s3 = s2 * si;
-                double a, b, cut;
-                if (K == VPM_KERNEL_GAUSSIANERF) { a = -0.5 * l2e * s2; b = kConst1 * s3; cut = 49.0 * sg * sg; }
-                else if (K == VPM_KERNEL_GAUSSIAN) { a = -l2e * s3; b = kConst3 * s3; cut = 9.0 * sg * sg; }
-                else { a = s2; b = kConst4 * 7.5 * s3; cut = (double)FLT_MAX; }
-                p.w = (float)a;
-                lo.w = 0.f;
-                const double G0 = sr[RG], G1 = sr[RG + 1], G2 = sr[RG + 2];
-                double Jp[9];
-#pragma unroll
-                for (int c = 0; c < 9; ++c) Jp[c] = sr[RJ + c] - Jr[c];
-                float4 q;
-                if (transposed) {
-                    q = make_float4((float)(Jp[0] * G0 + Jp[1] * G1 + Jp[2] * G2),
-                                    (float)(Jp[3] * G0 + Jp[4] * G1 + Jp[5] * G2),
-                                    (float)(Jp[6] * G0 + Jp[7] * G1 + Jp[8] * G2), (float)cut);
-                } else {
-                    q = make_float4((float)(Jp[0] * G0 + Jp[3] * G1 + Jp[6] * G2),
-                                    (float)(Jp[1] * G0 + Jp[4] * G1 + Jp[7] * G2),
-                                    (float)(Jp[2] * G0 + Jp[5] * G1 + Jp[8] * G2), (float)cut);
-                }
-                lp[tid] = p;
-                ll[tid] = lo;
-                lq[tid] = q;
-                lg[tid] = make_float4((float)G0, (float)G1, (float)G2, (float)b);
-            }
-            __syncthreads();
-            for (int s0 = 0; s0 < n; s0 += 32) {
-                const int s1 = min(s0 + 32, n);
-                float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-                for (int s = s0; s < s1; ++s) {
-                    const float4 a = lp[s];
-                    const float4 c = lq[s];
-                    float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
-                    float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    if (__any(r2 < c.w)) {
-                        const float4 lo = ll[s];
-                        const float4 g = lg[s];
-                        dx += tlx - lo.x;
-                        dy += tly - lo.y;
-                        dz += tlz - lo.z;
-                        r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                        float z;
-                        if (base + s == t) {
-                            z = 0.f;   // self pair: exactly zero (see estr.hip)
-                        } else if constexpr (K == VPM_KERNEL_GAUSSIANERF) {
-                            z = g.w * ex2(a.w * r2);
-                        } else if constexpr (K == VPM_KERNEL_GAUSSIAN) {
-                            const float r = r2 > 0.f ? r2 * rsq(r2) : 0.f;
-                            z = g.w * ex2(a.w * r2 * r);
-                        } else {
-                            const float w = rsq(fmaf(a.w, r2, 1.f));
-                            const float w2 = w * w;
-                            z = g.w * (w2 * w2) * (w2 * w);
-                        }
-                        acc[0] = fmaf(z, g.x, acc[0]);
-                        acc[1] = fmaf(z, g.y, acc[1]);
-                        acc[2] = fmaf(z, g.z, acc[2]);
-                        acc[3] = fmaf(z, c.x, acc[3]);
-                        acc[4] = fmaf(z, c.y, acc[4]);
-                        acc[5] = fmaf(z, c.z, acc[5]);
-                    }
-                }
-#pragma unroll
-                for (int c = 0; c < 6; ++c) {
-                    const float x = acc[c];
-                    const float tt = sum[c] + x;
-                    const float bp = tt - sum[c];
-                    err[c] += (sum[c] - (tt - bp)) + (x - bp);
-                    sum[c] = tt;
-                }
-            }
-        }
-    }
-    if (active) {
-        double* rec = P + t * NF;
-        double A[6];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) A[c] = (double)sum[c] + (double)err[c];
-        double Jp[9];
-#pragma unroll
-        for (int c = 0; c < 9; ++c) Jp[c] = rec[RJ + c] - Jr[c];
-        for (int q = 0; q < 3; ++q) {
-            const double S = transposed ? Jp[3 * q] * A[0] + Jp[3 * q + 1] * A[1] + Jp[3 * q + 2] * A[2] - A[3 + q]
-                                        : Jp[q] * A[0] + Jp[3 + q] * A[1] + Jp[6 + q] * A[2] - A[3 + q];
-            rec[RSFS + q] += S;
-        }
-    }
-}
-
 // Source records of the F32C list kernels for E_str and zeta, packed once
 // per call.  The list kernels read each source once for every target leaf
 // that lists it (about 140 times on FMM leaf lists); gathering 48 (zeta) or
@@ -804,6 +663,130 @@ int pack_list_src(vpm_field f, int kernel, int mode, ListSrc& out)
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "pack_list_src kernel");
     return VPM_OK;
+}
+
+// ------------------------------------------------------------ E_str ranges
+// Estr_fmm! over a direct list (src/FLOWVPM_subfilterscale_models.jl:157-188):
+// E_str pairs (:16-41) for every (target range, source range), no static skip,
+// accumulated onto SFS.
+//
+// F32C: within a block (<= 256 targets of one interval), J' = J - J_ref with
+// J_ref the J of the block's first target (FP64), and
+//     sum_j zeta_j (J'_i - J'_j)^T Gamma_j = J'_i^T A - B,
+//     A = sum zeta Gamma, B = sum zeta J'_j^T Gamma_j,
+// the packed sources of pack_list_src (hi/lo positions relative to particle
+// 0, J^T Gamma in FP64), sources beyond q_far skipped per wave (Gaussian-erf,
+// Gaussian), sums folded into TwoSum every 32 sources.
+template <int K>
+__global__ __launch_bounds__(kBlock) void estr_ranges_f32c_kernel(const RangeItem* __restrict__ items,
+                                                                  const int2* __restrict__ lsrc,
+                                                                  const ListSrc S,
+                                                                  double* __restrict__ P, int transposed)
+{
+    __shared__ float4 lp[kTile];   // s_hi, cut
+    __shared__ float4 lg[kTile];   // Gamma, b
+    __shared__ float4 lq[kTile];   // J'^T Gamma (or J' Gamma), a
+    __shared__ float4 ll[kTile];   // s_lo
+    const RangeItem it = items[xcd_block(blockIdx.x, gridDim.x)];
+    const int tid = threadIdx.x;
+    const double* r0 = P + it.t_begin * NF;
+    double Jr[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) Jr[c] = r0[RJ + c];
+    const int64_t t = it.t_begin + tid;
+    const bool active = t < it.t_end;
+    const int64_t tt = active ? t : it.t_begin;
+    const float4 th = S.hi[tt], tl = S.lo[tt];
+    const float tx = th.x, ty = th.y, tz = th.z, tlx = tl.x, tly = tl.y, tlz = tl.z;
+    float sum[6], err[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) { sum[c] = 0.f; err[c] = 0.f; }
+    for (int64_t l = it.l_begin; l < it.l_end; ++l) {
+        const int2 rs = lsrc[l];
+        const int64_t sb = rs.x, se = rs.y;
+        for (int64_t base = sb; base < se; base += kTile) {
+            const int n = (int)((se - base) < kTile ? (se - base) : kTile);
+            __syncthreads();
+            if (tid < n) {
+                const int64_t j = base + tid;
+                const float4 lo = S.lo[j];    // s_lo, a
+                const float4 gq = S.gb[j];    // Gamma, b
+                // (J_j - J_ref)^T Gamma_j (or (J_j - J_ref) Gamma_j)
+                const double G0 = gq.x, G1 = gq.y, G2 = gq.z;
+                double q3[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double ref = transposed ? Jr[3 * c] * G0 + Jr[3 * c + 1] * G1 + Jr[3 * c + 2] * G2
+                                                  : Jr[c] * G0 + Jr[3 + c] * G1 + Jr[6 + c] * G2;
+                    q3[c] = S.jg[c * S.ld + j] - ref;
+                }
+                lp[tid] = S.hi[j];            // s_hi, cut
+                ll[tid] = lo;
+                lq[tid] = make_float4((float)q3[0], (float)q3[1], (float)q3[2], lo.w);
+                lg[tid] = gq;
+            }
+            __syncthreads();
+            for (int s0 = 0; s0 < n; s0 += 32) {
+                const int s1 = min(s0 + 32, n);
+                float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                for (int s = s0; s < s1; ++s) {
+                    const float4 a = lp[s];
+                    float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
+                    float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    if (__any(r2 < a.w)) {
+                        const float4 c = lq[s];
+                        const float4 lo = ll[s];
+                        const float4 g = lg[s];
+                        dx += tlx - lo.x;
+                        dy += tly - lo.y;
+                        dz += tlz - lo.z;
+                        r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                        float z;
+                        if (base + s == t) {
+                            z = 0.f;   // self pair: exactly zero (see estr.hip)
+                        } else if constexpr (K == VPM_KERNEL_GAUSSIANERF) {
+                            z = g.w * ex2(c.w * r2);
+                        } else if constexpr (K == VPM_KERNEL_GAUSSIAN) {
+                            const float r = r2 > 0.f ? r2 * rsq(r2) : 0.f;
+                            z = g.w * ex2(c.w * r2 * r);
+                        } else {
+                            const float w = rsq(fmaf(c.w, r2, 1.f));
+                            const float w2 = w * w;
+                            z = g.w * (w2 * w2) * (w2 * w);
+                        }
+                        acc[0] = fmaf(z, g.x, acc[0]);
+                        acc[1] = fmaf(z, g.y, acc[1]);
+                        acc[2] = fmaf(z, g.z, acc[2]);
+                        acc[3] = fmaf(z, c.x, acc[3]);
+                        acc[4] = fmaf(z, c.y, acc[4]);
+                        acc[5] = fmaf(z, c.z, acc[5]);
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    const float x = acc[c];
+                    const float tt = sum[c] + x;
+                    const float bp = tt - sum[c];
+                    err[c] += (sum[c] - (tt - bp)) + (x - bp);
+                    sum[c] = tt;
+                }
+            }
+        }
+    }
+    if (active) {
+        double* rec = P + t * NF;
+        double A[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) A[c] = (double)sum[c] + (double)err[c];
+        double Jp[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) Jp[c] = rec[RJ + c] - Jr[c];
+        for (int q = 0; q < 3; ++q) {
+            const double S = transposed ? Jp[3 * q] * A[0] + Jp[3 * q + 1] * A[1] + Jp[3 * q + 2] * A[2] - A[3 + q]
+                                        : Jp[q] * A[0] + Jp[3 + q] * A[1] + Jp[6 + q] * A[2] - A[3 + q];
+            rec[RSFS + q] += S;
+        }
+    }
 }
 
 // E_str over short target intervals (FMM leaves), as ranges_wave_f32c_kernel:
@@ -1128,8 +1111,10 @@ int launch_estr_plan(vpm_field f, int kernel, int precision, int transposed, con
     const dim3 grid((unsigned)dp.nitems);
     timing_begin(f);
     ListSrc S;
-    if (!f64 && dp.wave) {
+    if (!f64) {
         if (int rc = pack_list_src(f, kernel, transposed ? 1 : 2, S)) return rc;
+    }
+    if (!f64 && dp.wave) {
         const int64_t ni = dp.nitems;
         const dim3 wgrid((unsigned)((ni + kBlock / 64 - 1) / (kBlock / 64)));
         switch (kernel) {
@@ -1157,7 +1142,7 @@ int launch_estr_plan(vpm_field f, int kernel, int precision, int transposed, con
         }
     } else {
         switch (kernel) {
-#define VPM_E32(KID) case KID: hipLaunchKernelGGL(estr_ranges_f32c_kernel<KID>, grid, dim3(kBlock), 0, f->stream, d_items, d_src, f->P, transposed); break;
+#define VPM_E32(KID) case KID: hipLaunchKernelGGL(estr_ranges_f32c_kernel<KID>, grid, dim3(kBlock), 0, f->stream, d_items, d_src, S, f->P, transposed); break;
             VPM_E32(VPM_KERNEL_GAUSSIAN) VPM_E32(VPM_KERNEL_GAUSSIANERF) VPM_E32(VPM_KERNEL_WINCKELMANS)
 #undef VPM_E32
         default: return fail(VPM_EINVAL, "unknown kernel id");
