@@ -1168,8 +1168,9 @@ int launch_estr_ranges(vpm_field f, int kernel, int precision, int transposed, c
 // for every (target range, source range), J[1:3] of each target += Gamma_j
 // zeta(r/sigma_j)/sigma_j^3 over the sources.  No reset, no static skip.
 //
-// F32C: hi/lo positions relative to the item's first target, (a, b) per
-// source as zeta_params, sources beyond zeta_cut skipped per wave, sums
+// F32C: the packed sources of pack_list_src (hi/lo positions relative to
+// particle 0, (a, b) per source as zeta_params, the cut as zeta_cut),
+// sources beyond the cut skipped per wave, sums
 // folded into TwoSum every 32 sources and added onto the FP64 record.  BS
 // is the block size: 64 (one wave per FMM leaf) for short items, else 256.
 // Leaves of <= 32 targets on average take zeta_ranges_wave_f32c_kernel.
