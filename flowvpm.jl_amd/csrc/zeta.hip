@@ -63,8 +63,13 @@ __global__ __launch_bounds__(kBlock) void zeta_f32c_kernel(const double* __restr
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int64_t i = (int64_t)blockIdx.x * kBlock + tid;
-    const int64_t ib = (int64_t)blockIdx.x * kBlock;
+    // (target block, source chunk) from the linear block index renumbered
+    // per XCD: the blocks one XCD runs at once are neighbouring target
+    // blocks of one chunk, which share their near source tiles in its L2
+    const int64_t L = xcd_block((int64_t)blockIdx.y * gridDim.x + blockIdx.x, (int64_t)gridDim.x * gridDim.y);
+    const int64_t bx = L % gridDim.x, by = L / gridDim.x;
+    const int64_t i = bx * kBlock + tid;
+    const int64_t ib = bx * kBlock;
     const double ox = pos[ib], oy = pos[ld + ib], oz = pos[2 * ld + ib];
     const int64_t it = i < np ? i : ib;
     float tx, ty, tz, tlx, tly, tlz;
@@ -76,7 +81,7 @@ __global__ __launch_bounds__(kBlock) void zeta_f32c_kernel(const double* __restr
     block_box(box, wb, blo, bhi);
     const double org[3] = {ox, oy, oz};
 
-    const int64_t sb = (int64_t)blockIdx.y * chunk;
+    const int64_t sb = by * chunk;
     const int64_t se = min(sb + chunk, np);
     float sum[3] = {0.f, 0.f, 0.f}, err[3] = {0.f, 0.f, 0.f};
     __shared__ int nlist[kBlock];
@@ -137,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void zeta_f32c_kernel(const double* __restr
         }
     }
     if (i < np) {
-        double* out = part + (int64_t)blockIdx.y * 3 * pld + i;
+        double* out = part + by * 3 * pld + i;
 #pragma unroll
         for (int c = 0; c < 3; ++c) out[c * pld] = (double)sum[c] + (double)err[c];
     }
