@@ -194,8 +194,9 @@ __global__ __launch_bounds__(kBlock) void ranges_f32c_kernel(const RangeItem* __
 // leaves still keep all 64 lanes busy.
 // A step of R sources is far when all R are beyond the wave's target box by
 // the far radius (then every pair has r > 0), near otherwise.  Per-lane FP32
-// sums fold into TwoSum pairs every ~64 pairs; at the end the R groups'
-// FP64 values are added in a fixed butterfly and group 0 writes.
+// sums fold into TwoSum pairs, the U partials after every 64-source chunk
+// (64 / R steps), the rest every ~64 pairs; at the end the R groups' FP64
+// values are added in a fixed butterfly and group 0 writes.
 // R > 1: six waves per SIMD (80 VGPRs; R = 4 and 8 spill 12 bytes per lane,
 // touched only around the per-chunk folds) instead of the five that 81-84
 // VGPRs allow, 3 % faster on 16-particle leaves.  R = 1 would spill at five
@@ -343,6 +344,12 @@ __global__ __launch_bounds__(kBlock, R > 1 ? 6 : 4) void ranges_wave_f32c_kernel
                 else
                     far_pair<K>(tx, ty, tz, lpos[sidx], lgs[sidx], acc);
             }
+            // a leaf's near-field U nearly cancels (its neighbourhood
+            // balances): the U partials fold after every chunk (64 / R
+            // steps), the rest after ~64.  With all 14 folded every 64
+            // steps, the 2^20 jet's 16-particle leaf list lost up to 2.5e-5
+            // of U; folded per chunk 8.0e-6, at the same speed
+            fold_u(acc, sum, err);
         }
         since += (filled + R - 1) / R;
         if (since >= 64) {

@@ -77,6 +77,7 @@ def lib():
         L.vpmo_estr_ranges.argtypes = [_dp, C.c_int64, C.c_int, C.c_int, _ip, _ip, _ip, _ip, C.c_int64]
         L.vpmo_estr_scale.argtypes = [_dp, C.c_int64, C.c_int, C.c_int, _dp, C.c_int]
         L.vpmo_estr_scale_ranges.argtypes = [_dp, C.c_int64, C.c_int, C.c_int, _ip, _ip, _ip, _ip, C.c_int64, _dp]
+        L.vpmo_u_scale_ranges.argtypes = [_dp, C.c_int64, C.c_int, _ip, _ip, _ip, _ip, C.c_int64, _dp]
         L.vpmo_core_spreading_reset.restype = C.c_int
         L.vpmo_core_spreading_reset.argtypes = [_dp, C.c_int64, C.c_int, C.c_double, C.c_int, C.c_double,
                                                 C.c_int, C.c_int, C.POINTER(C.c_int), _dp]
@@ -191,6 +192,15 @@ def estr_scale_ranges(P, kernel, tb, te, sb, se, transposed=True):
     a = [np.ascontiguousarray(x, dtype=np.int64) for x in (tb, te, sb, se)]
     s = np.zeros(P.shape[0])
     lib().vpmo_estr_scale_ranges(_check(P), NFIELDS, _kid(kernel), int(transposed), *a, len(a[0]), s)
+    return s
+
+
+def u_scale_ranges(P, kernel, tb, te, sb, se):
+    """The U summation scale over the pairs of a direct list: sum over each
+    target's pairs of |U term| (the rounding bound of any evaluation)."""
+    a = [np.ascontiguousarray(x, dtype=np.int64) for x in (tb, te, sb, se)]
+    s = np.zeros(P.shape[0])
+    lib().vpmo_u_scale_ranges(_check(P), NFIELDS, _kid(kernel), *a, len(a[0]), s)
     return s
 
 

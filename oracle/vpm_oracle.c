@@ -1197,6 +1197,35 @@ void vpmo_estr_scale(const double* P, int64_t np, int kernel, int transposed, do
 }
 
 /* the same over the pairs of vpmo_estr_ranges (accumulated onto s) */
+/* The U summation scale over the pairs of a direct list: s_i accumulates
+ * |g(r/sigma_j) crss_ij| (the 2-norm of the pair's U term, fmm.jl:125-138)
+ * for every pair of vpmo_direct_ranges, coincident pairs skipped as there.
+ * Any evaluation whose terms carry relative errors e_j is off by at most
+ * max_j e_j times s_i, however much the U sum cancels. */
+void vpmo_u_scale_ranges(const double* P, int64_t stride, int kernel,
+                         const int64_t* tb, const int64_t* te, const int64_t* sb, const int64_t* se,
+                         int64_t npairs, double* s)
+{
+    const double c4 = const4();
+    for (int64_t k = 0; k < npairs; k++)
+        for (int64_t i = tb[k]; i < te[k]; i++) {
+            const double* tp = P + i * stride;
+            for (int64_t j = sb[k]; j < se[k]; j++) {
+                const double* sp = P + j * stride;
+                const double dx = tp[RX] - sp[RX], dy = tp[RX + 1] - sp[RX + 1], dz = tp[RX + 2] - sp[RX + 2];
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 == 0.0) continue;
+                const double r = sqrt(r2);
+                double g, dg;
+                vpmo_g_dgdr(kernel, r / sp[RSIG], &g, &dg);
+                const double cx = dy * sp[RG + 2] - dz * sp[RG + 1];
+                const double cy = dz * sp[RG] - dx * sp[RG + 2];
+                const double cz = dx * sp[RG + 1] - dy * sp[RG];
+                s[i] += fabs(g) * c4 / (r2 * r) * sqrt(cx * cx + cy * cy + cz * cz);
+            }
+        }
+}
+
 void vpmo_estr_scale_ranges(const double* P, int64_t stride, int kernel, int transposed,
                             const int64_t* tb, const int64_t* te, const int64_t* sb, const int64_t* se,
                             int64_t npairs, double* s)

@@ -54,6 +54,10 @@ void vpmo_estr_scale(const double* P, int64_t np, int kernel, int transposed, do
 void vpmo_estr_scale_ranges(const double* P, int64_t stride, int kernel, int transposed,
                             const int64_t* tb, const int64_t* te, const int64_t* sb, const int64_t* se,
                             int64_t npairs, double* s);
+/* the U summation scale s_i = sum |U term| over the pairs of vpmo_direct_ranges */
+void vpmo_u_scale_ranges(const double* P, int64_t stride, int kernel,
+                         const int64_t* tb, const int64_t* te, const int64_t* sb, const int64_t* se,
+                         int64_t npairs, double* s);
 /* the same for targets [t0, t1) only (sources: all np) */
 void vpmo_estr_direct_targets(double* P, int64_t np, int64_t t0, int64_t t1, int kernel, int transposed,
                               int nthreads);

@@ -1070,3 +1070,43 @@ def test_estr_scale_bounds_estr(orc, vpm, transposed):
             tb.append(a); te.append(b); sb.append(c); se.append(d)
     np.testing.assert_allclose(orc.estr_scale_ranges(P, "gaussianerf", tb, te, sb, se, transposed=transposed),
                                orc.estr_scale(P, "gaussianerf", transposed=transposed), rtol=1e-12)
+
+
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans", "gaussian", "singular"])
+def test_u_scale_bounds_u(orc, vpm, kernel):
+    """The U summation scale s_i = sum_j |U term_ij| over a direct list (the
+    bound the full-size F32C list test holds U to) dominates |U_i|, equals it
+    when a target has one source, skips coincident pairs as fmm.jl:118 does,
+    is additive over sub-lists, and is independent of the sign of Gamma (so
+    a symmetric neighbourhood, whose U cancels, keeps its full scale)."""
+    n = 300
+    P = vpm.workloads.random_cloud(n, seed=53)
+    P[:, 6] *= 3.0
+    P[:, 9:24] = 0.0
+    full = ([0], [n], [0], [n])
+    ref = P.copy()
+    orc.direct_ranges(ref, kernel, *full)
+    s = orc.u_scale_ranges(P, kernel, *full)
+    u = np.linalg.norm(ref[:, 9:12], axis=1)
+    assert np.all(u <= s * (1 + 1e-12))
+    assert np.median(s / np.maximum(u, 1e-300)) > 1.5          # the sums do cancel
+    # one source per target: the scale is |U|
+    one = ([0, 1], [1, 2], [5, 9], [6, 10])
+    r1 = P.copy()
+    orc.direct_ranges(r1, kernel, *one)
+    np.testing.assert_allclose(orc.u_scale_ranges(P, kernel, *one)[:2], np.linalg.norm(r1[:2, 9:12], axis=1),
+                               rtol=1e-14)
+    # a coincident source adds nothing
+    Q = P.copy()
+    Q[1, 0:3] = Q[0, 0:3]
+    assert orc.u_scale_ranges(Q, kernel, [0], [1], [1], [2])[0] == 0.0
+    # additive over sub-lists, and blind to the sign of Gamma
+    cuts = [0, 37, 150, 151, 300]
+    tb, te, sb, se = [], [], [], []
+    for a, b in zip(cuts[:-1], cuts[1:]):
+        for c, d in zip(cuts[:-1], cuts[1:]):
+            tb.append(a); te.append(b); sb.append(c); se.append(d)
+    np.testing.assert_allclose(orc.u_scale_ranges(P, kernel, tb, te, sb, se), s, rtol=1e-12)
+    M = P.copy()
+    M[:, 3:6] *= -1.0
+    np.testing.assert_allclose(orc.u_scale_ranges(M, kernel, *full), s, rtol=1e-14)
