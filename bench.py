@@ -576,6 +576,31 @@ def main():
                                            "dGamma_dt_vs_J_Gamma_scale": {"max": float(esc.max()),
                                                                           "rms": float(np.sqrt(np.mean(esc * esc)))}}
         log("bench: FP64 %.1f ms; err U %s J %s dGamma/dt %s" % (1e3 * w64, eu, ej, es))
+
+        # FP64 with the far form (VPM_UJ_F64_FAR): one rsqrt per pair, g = 1
+        # past q = 10, the F32C kernel's spatial order
+        log("bench: FP64 far-form GPU evaluation (warmup + 1)")
+        f.profile(False)
+        f.uj_direct(kernel, "f64far")
+        f.synchronize()
+        f.profile(True)
+        t0 = time.perf_counter()
+        f.uj_direct(kernel, "f64far")
+        f.synchronize()
+        w64f = time.perf_counter() - t0
+        k64f, _ = f.timing(1)
+        out64f = np.zeros_like(P)
+        f.get_particles(out64f, rows=(9, 24))
+        res["f64far"] = {"value": pairs / w64f, "unit": "pair-interactions/s", "ms_per_step": 1e3 * w64f,
+                         "kernel_ms": k64f, "speedup_f32c_over_f64far": w64f / (wall / args.steps),
+                         "speedup_f64far_over_f64": w64 / w64f,
+                         "frac_of_fp64_peak": pairs * FLOP_PER_PAIR[kernel] / (k64f / 1e3) / PEAK_FP64,
+                         "error_vs_f64": {"U": vpm.metrics.err_stats(out64f[:, 9:12], out64[:, 9:12]),
+                                          "J": vpm.metrics.err_stats(out64f[:, 15:24], out64[:, 15:24])},
+                         "arithmetic": "FP64 pairs, one rsqrt each, far form F' = r^-3, H'' = r^-5 past q = 10 "
+                                       "(g = 1 and q g' = 0 to FP64 rounding), reference g and dg nearer"}
+        log("bench: FP64 far form %.1f ms (kernel %.1f ms); vs FP64 U %s" % (1e3 * w64f, k64f,
+                                                                             res["f64far"]["error_vs_f64"]["U"]))
     f.profile(False)
 
     if not args.no_adjoint:

@@ -17,7 +17,7 @@ STATUS_NAMES = {0: "VPM_OK", -1: "VPM_EINVAL", -2: "VPM_ERANGE", -3: "VPM_ENOMEM
                 -4: "VPM_EDEVICE", -5: "VPM_ENAN", -6: "VPM_ERESET", -7: "VPM_ECONVERGE"}
 KERNELS = {"singular": 0, "gaussian": 1, "gaussianerf": 2, "winckelmans": 3}
 PRECISIONS = {"f32c": 0, "f64": 1}
-UJ_RESET, UJ_RESET_SFS, UJ_SFS, UJ_NO_SORT = 1, 2, 4, 8
+UJ_RESET, UJ_RESET_SFS, UJ_SFS, UJ_NO_SORT, UJ_F64_FAR = 1, 2, 4, 8, 16
 
 
 class VPMError(RuntimeError):

@@ -179,8 +179,13 @@ class Field:
 
     def uj_direct(self, kernel=kernel_default, precision="f32c", reset=True, reset_sfs=False, sfs=False,
                   transposed=True, sort=True):
+        """vpm_uj_direct.  precision "f64far" is FP64 with the far form
+        (VPM_PREC_F64 + VPM_UJ_F64_FAR); E_str (sfs) then runs in FP64."""
         flags = ((_lib.UJ_RESET if reset else 0) | (_lib.UJ_RESET_SFS if reset_sfs else 0) |
                  (_lib.UJ_SFS if sfs else 0) | (0 if sort else _lib.UJ_NO_SORT))
+        if precision == "f64far":
+            flags |= _lib.UJ_F64_FAR
+            precision = "f64"
         cfg = make_config(kernel, precision, transposed)
         check(lib().vpm_uj_direct(self.handle, C.byref(cfg), flags))
 
@@ -301,6 +306,8 @@ class RangePlan:
 def make_config(kernel=kernel_default, precision="f32c", transposed=True):
     cfg = _lib.Config()
     cfg.kernel = _lib.KERNELS[kernel]
+    if precision not in _lib.PRECISIONS:
+        raise ValueError("precision %r: one of %s (and \"f64far\" for U/J only)" % (precision, sorted(_lib.PRECISIONS)))
     cfg.precision = _lib.PRECISIONS[precision]
     cfg.transposed = int(bool(transposed))
     return cfg

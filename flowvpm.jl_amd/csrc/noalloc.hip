@@ -88,9 +88,9 @@ extern "C" int vpm_uj_direct_noalloc(vpm_field f, const vpm_config* cfg, uint32_
         return fail(VPM_EINVAL, "unknown kernel id " + std::to_string(cfg->kernel));
     if (cfg->precision != VPM_PREC_F32C && cfg->precision != VPM_PREC_F64)
         return fail(VPM_EINVAL, "unknown precision " + std::to_string(cfg->precision));
-    if (flags & VPM_UJ_SFS)
-        return fail(VPM_EINVAL, "vpm_uj_direct_noalloc: VPM_UJ_SFS (E_str) is not served without allocation; "
-                                "call vpm_uj_direct");
+    if (flags & (VPM_UJ_SFS | VPM_UJ_F64_FAR))
+        return fail(VPM_EINVAL, "vpm_uj_direct_noalloc: VPM_UJ_SFS (E_str) and VPM_UJ_F64_FAR are not served "
+                                "without allocation; call vpm_uj_direct");
     std::lock_guard<std::mutex> lock(f->mu);
     hipError_t e = hipSetDevice(f->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");

@@ -450,6 +450,164 @@ __device__ __forceinline__ void pair_f64(double tx, double ty, double tz, const 
 }
 #pragma clang fp contract(on)
 
+// ------------------------------------------- FP64 pair with a far form
+// The FP64 pair of the far-form kernel (VPM_UJ_F64_FAR): one rsqrt per pair
+// instead of a sqrt and three divisions, and past a q where the kernel's g
+// is 1 and q g' is 0 to FP64 rounding, the far form F' = r^-3, H'' = r^-5:
+//  * Gaussian-erf, q >= 10: 1 - g = erfc(q/sqrt2) + sqrt(2/pi) q e^{-q^2/2}
+//    < 2e-21 and q g' = sqrt(2/pi) q^3 e^{-q^2/2} < 2e-19;
+//  * Gaussian, q^2 >= 13 (q >= 3.605): e^{-q^3} < 4e-21, 3 q^3 e^{-q^3} < 6e-19;
+//  * Winckelmans (g -> 1 only as 1 - 1.9/q^4) and singular (r = 0 guard):
+//    every pair takes the near form.
+// Near pairs use the reference's g and dg/dq (g_dgdr_ref) at q = r / sigma.
+template <int K> struct Far64 { static constexpr double q2 = 0.0; static constexpr bool cheap = false; };
+template <> struct Far64<VPM_KERNEL_GAUSSIANERF> { static constexpr double q2 = 100.0; static constexpr bool cheap = true; };
+template <> struct Far64<VPM_KERNEL_GAUSSIAN> { static constexpr double q2 = 13.0; static constexpr bool cheap = true; };
+
+// far-field r^2 threshold of a source of core size sigma (FP64 kernel)
+template <int K>
+__device__ __forceinline__ double far_cut64(double sigma)
+{
+    return Far64<K>::cheap ? Far64<K>::q2 * sigma * sigma : DBL_MAX;
+}
+
+// accum_pair in FP64 (the 14 sums of add_f32c_to_record)
+__device__ __forceinline__ void accum_pair64(double dx, double dy, double dz, double Fp, double Hpp,
+                                             const double4 b, double (&acc)[kAccF32])
+{
+    const double vx = fma(dy, b.z, -dz * b.y);
+    const double vy = fma(dz, b.x, -dx * b.z);
+    const double vz = fma(dx, b.y, -dy * b.x);
+    acc[0] = fma(Fp, vx, acc[0]);
+    acc[1] = fma(Fp, vy, acc[1]);
+    acc[2] = fma(Fp, vz, acc[2]);
+    acc[3] = fma(Fp, b.x, acc[3]);
+    acc[4] = fma(Fp, b.y, acc[4]);
+    acc[5] = fma(Fp, b.z, acc[5]);
+    const double hx = Hpp * vx, hy = Hpp * vy, hz = Hpp * vz;
+    acc[6] = fma(hx, dx, acc[6]);
+    acc[7] = fma(hy, dx, acc[7]);
+    acc[8] = fma(hz, dx, acc[8]);
+    acc[9] = fma(hx, dy, acc[9]);
+    acc[10] = fma(hy, dy, acc[10]);
+    acc[11] = fma(hz, dy, acc[11]);
+    acc[12] = fma(hx, dz, acc[12]);
+    acc[13] = fma(hy, dz, acc[13]);
+}
+
+// a = (x, y, z, far threshold), b = (Gamma' = -Gamma / 4 pi, 1/sigma)
+template <int K>
+__device__ __forceinline__ void near_pair64(double tx, double ty, double tz, const double4 a, const double4 b,
+                                            double (&acc)[kAccF32])
+{
+    const double dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
+    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+    if (r2 > 0.0) {   // coincident pairs contribute nothing (fmm.jl:118)
+        const double ri = rsqrt(r2);
+        const double q = r2 * ri * b.w;
+        double g, dg;
+        g_dgdr_ref<K>(q, g, dg);
+        const double ri2 = ri * ri;
+        const double ri3 = ri2 * ri;
+        accum_pair64(dx, dy, dz, g * ri3, fma((-1.0 / 3.0) * q, dg, g) * ri3 * ri2, b, acc);
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void far_pair64(double tx, double ty, double tz, const double4 a, const double4 b,
+                                           double (&acc)[kAccF32])
+{
+    if constexpr (Far64<K>::cheap) {
+        const double dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
+        const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        const double ri = rsqrt(r2);
+        const double ri2 = ri * ri;
+        const double Fp = ri2 * ri;
+        accum_pair64(dx, dy, dz, Fp, Fp * ri2, b, acc);
+    } else {
+        near_pair64<K>(tx, ty, tz, a, b, acc);
+    }
+}
+
+// Bounding box of the wave's targets in FP64 (wave-uniform)
+struct WaveBox64 { double lo[3], hi[3]; };
+
+__device__ __forceinline__ double uniform_d(double x)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ WaveBox64 wave_box64(double tx, double ty, double tz)
+{
+    WaveBox64 b;
+    const double v[3] = {tx, ty, tz};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double mn = v[c], mx = v[c];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            mn = fmin(mn, __shfl_xor(mn, off));
+            mx = fmax(mx, __shfl_xor(mx, off));
+        }
+        b.lo[c] = uniform_d(mn);
+        b.hi[c] = uniform_d(mx);
+    }
+    return b;
+}
+
+// One LDS tile of n <= kTile sources against the lane's target (FP64 far
+// form), split per 64-source group into near and far sources against the
+// wave's box as tile_f32c does; a far source is beyond q_far of every
+// target of the wave.
+template <int K>
+__device__ __forceinline__ void tile_f64far(int n, const double4* lpos, const double4* lgs, double tx, double ty,
+                                            double tz, const WaveBox64& box, double (&acc)[kAccF32])
+{
+    constexpr int NG = kTile / 64;
+    const int lane = threadIdx.x & 63;
+    uint64_t near[NG], far[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        const int s = g * 64 + lane;
+        bool nr = false, ok = s < n;
+        if (ok) {
+            const double4 a = lpos[s];
+            const double ex = fmax(fmax(box.lo[0] - a.x, a.x - box.hi[0]), 0.0);
+            const double ey = fmax(fmax(box.lo[1] - a.y, a.y - box.hi[1]), 0.0);
+            const double ez = fmax(fmax(box.lo[2] - a.z, a.z - box.hi[2]), 0.0);
+            nr = fma(ez, ez, fma(ey, ey, ex * ex)) < a.w;
+        }
+        near[g] = __ballot(nr);
+        far[g] = __ballot(ok && !nr);
+    }
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        uint64_t fb = far[g];
+        if (fb == ~0ull) {
+#pragma unroll 2
+            for (int s = 0; s < 64; ++s) far_pair64<K>(tx, ty, tz, lpos[g * 64 + s], lgs[g * 64 + s], acc);
+        } else {
+            while (fb) {
+                const int s = g * 64 + __builtin_ctzll(fb);
+                fb &= fb - 1;
+                far_pair64<K>(tx, ty, tz, lpos[s], lgs[s], acc);
+            }
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        uint64_t nb = near[g];
+        while (nb) {
+            const int s = g * 64 + __builtin_ctzll(nb);
+            nb &= nb - 1;
+            near_pair64<K>(tx, ty, tz, lpos[s], lgs[s], acc);
+        }
+    }
+}
+
 // zeta(r/sigma)/sigma^3 of a source in FP32 as b exp2(a r^2) (Gaussian-erf),
 // b exp2(a r^3) (Gaussian), b (1 + a r^2)^-3.5 (Winckelmans), with (a, b)
 // from the source's sigma in FP64 (kernel.jl:51,60,69-74).

@@ -149,6 +149,58 @@ __global__ __launch_bounds__(kBlock) void uj_f64_kernel(
 }
 #pragma clang fp contract(on)
 
+// ----------------------------------------------- F64 far-form kernel
+// FP64 pair arithmetic with the far form of Far64<K> (VPM_UJ_F64_FAR): the
+// sources come in the F32C kernel's spatial order, staged as (x, y, z, far
+// r^2 threshold) and (Gamma', 1/sigma); each wave splits a 64-source group
+// into near and far sources against its target box.  The 14 sums are those
+// of the F32C kernel (add_f32c_to_record), summed in FP64.
+template <int K>
+__global__ __launch_bounds__(kBlock, 5) void uj_f64far_kernel(
+    const double* __restrict__ tpos, int64_t tld, int64_t t0, int64_t nt,
+    const double* __restrict__ spos, int64_t sld, const double4* __restrict__ sg,
+    int64_t s0, int64_t s1, int64_t chunk, double* __restrict__ part, int64_t pld)
+{
+    __shared__ double4 lpos[kTile];   // x, y, z, far-field r^2 threshold
+    __shared__ double4 lgs[kTile];    // Gamma' x, y, z, 1/sigma
+
+    const int tid = threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + tid;
+    const bool active = i < nt;
+    const int64_t it = t0 + (active ? i : (int64_t)blockIdx.x * kBlock);
+    const double tx = tpos[it], ty = tpos[tld + it], tz = tpos[2 * tld + it];
+    const WaveBox64 box = wave_box64(tx, ty, tz);
+    const int64_t sb = s0 + (int64_t)blockIdx.y * chunk;
+    const int64_t se = min(sb + chunk, s1);
+
+    double acc[kAccF32];
+#pragma unroll
+    for (int c = 0; c < kAccF32; ++c) acc[c] = 0.0;
+
+    for (int64_t base = sb; base < se; base += kTile) {
+        const int64_t j = base + tid;
+        double4 p, g;
+        if (j < se) {
+            const double4 r = sg[j];   // Gamma', sigma
+            p = make_double4(spos[j], spos[sld + j], spos[2 * sld + j], far_cut64<K>(r.w));
+            g = make_double4(r.x, r.y, r.z, 1.0 / r.w);
+        } else {   // not visited (s >= n in the tile loop)
+            p = make_double4(0.0, 0.0, 0.0, 0.0);
+            g = make_double4(0.0, 0.0, 0.0, 0.0);
+        }
+        __syncthreads();
+        lpos[tid] = p;
+        lgs[tid] = g;
+        __syncthreads();
+        tile_f64far<K>((int)min((int64_t)kTile, se - base), lpos, lgs, tx, ty, tz, box, acc);
+    }
+    if (active) {
+        double* out = part + (int64_t)blockIdx.y * kAccF32 * pld + i;
+#pragma unroll
+        for (int c = 0; c < kAccF32; ++c) out[c * pld] = acc[c];
+    }
+}
+
 // ------------------------------------------------------------- finalize
 // Sums the K partials of each target in split order and adds the total onto
 // U and J of the particle record (buffer_to_target_system!, fmm.jl:170-176).
@@ -298,6 +350,23 @@ __global__ __launch_bounds__(kBlock) void pack_f64_kernel(const double* __restri
     g64[k] = make_double4(rec[RG], rec[RG + 1], rec[RG + 2], rec[RSIG]);
 }
 
+// sources of the FP64 far-form kernel: positions in the order of perm (or
+// storage order), (-Gamma / 4 pi, sigma)
+__global__ __launch_bounds__(kBlock) void pack_f64far_kernel(const double* __restrict__ P, int64_t np,
+                                                             const uint32_t* __restrict__ perm,
+                                                             double* __restrict__ pos, int64_t ld,
+                                                             double4* __restrict__ g64)
+{
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= np) return;
+    const int64_t p = perm ? (int64_t)perm[k] : k;
+    const double* rec = P + p * NF;
+    pos[k] = rec[RX];
+    pos[ld + k] = rec[RX + 1];
+    pos[2 * ld + k] = rec[RX + 2];
+    g64[k] = make_double4(-kConst4 * rec[RG], -kConst4 * rec[RG + 1], -kConst4 * rec[RG + 2], rec[RSIG]);
+}
+
 static inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 #define VPM_LAUNCH_CHECK(what)                                                 \
@@ -398,6 +467,17 @@ int stage_field(vpm_field f, int kernel, int precision, bool sort, Staged& out, 
         out.g64 = (const double4*)gp;
         return VPM_OK;
     }
+    if (precision == kPrecF64Far) {
+        if ((rc = take_buffer(f, f->g64, sizeof(double4) * (size_t)ld, ar, gp, "sources"))) return rc;
+        const uint32_t* perm = nullptr;
+        if (sort && np > 1 && (rc = sort_field(f, perm, ar))) return rc;
+        hipLaunchKernelGGL(pack_f64far_kernel, dim3(nblocks(np)), dim3(kBlock), 0, f->stream,
+                           f->P, np, perm, (double*)posp, ld, (double4*)gp);
+        VPM_LAUNCH_CHECK("pack_f64far");
+        out.g64 = (const double4*)gp;
+        out.perm = perm;
+        return VPM_OK;
+    }
     if ((rc = take_buffer(f, f->gs, sizeof(float4) * (size_t)ld, ar, gp, "sources"))) return rc;
     if ((rc = take_buffer(f, f->cut, sizeof(float) * (size_t)ld, ar, cp, "sources"))) return rc;
     const uint32_t* perm = nullptr;
@@ -478,6 +558,28 @@ int launch_uj(vpm_field ft, const Staged& T, int64_t t0, int64_t nt, const Stage
         hipLaunchKernelGGL(finalize_f64_kernel, dim3(nblocks(nt)), dim3(kBlock), 0, ft->stream, part, pld,
                            (int)K, nt, T.perm, t0, ft->P);
         VPM_LAUNCH_CHECK("finalize_f64");
+        return VPM_OK;
+    }
+    if (precision == kPrecF64Far) {
+        switch (kernel) {
+#define VPM_UJ64F(KID)                                                                                    \
+    case KID:                                                                                             \
+        hipLaunchKernelGGL(uj_f64far_kernel<KID>, grid, dim3(kBlock), 0, ft->stream, T.pos, T.ld, t0, nt, \
+                           S.pos, S.ld, S.g64, s0, s1, chunk, part, pld);                                 \
+        break;
+            VPM_UJ64F(VPM_KERNEL_SINGULAR)
+            VPM_UJ64F(VPM_KERNEL_GAUSSIAN)
+            VPM_UJ64F(VPM_KERNEL_GAUSSIANERF)
+            VPM_UJ64F(VPM_KERNEL_WINCKELMANS)
+#undef VPM_UJ64F
+        default:
+            return fail(VPM_EINVAL, "unknown kernel id");
+        }
+        VPM_LAUNCH_CHECK("uj_f64far");
+        timing_end(ft, 1);
+        hipLaunchKernelGGL(finalize_f32c_kernel, dim3(nblocks(nt)), dim3(kBlock), 0, ft->stream, part, pld,
+                           (int)K, nt, T.perm, t0, ft->P);
+        VPM_LAUNCH_CHECK("finalize_f32c");
         return VPM_OK;
     }
     switch (kernel) {

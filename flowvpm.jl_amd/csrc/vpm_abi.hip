@@ -134,12 +134,16 @@ static int sync(vpm_field f)
 static int uj_direct_locked(vpm_field f, const vpm_config* cfg, uint32_t flags)
 {
     int rc;
+    const bool far64 = flags & VPM_UJ_F64_FAR;
+    if (far64 && cfg->precision != VPM_PREC_F64)
+        return fail(VPM_EINVAL, "VPM_UJ_F64_FAR needs VPM_PREC_F64");
     if (flags & VPM_UJ_RESET) if ((rc = launch_reset(f, 0))) return rc;
     if (flags & VPM_UJ_RESET_SFS) if ((rc = launch_reset(f, 1))) return rc;
-    const bool sort = cfg->precision == VPM_PREC_F32C && !(flags & VPM_UJ_NO_SORT);
+    const int prec = far64 ? kPrecF64Far : cfg->precision;
+    const bool sort = (cfg->precision == VPM_PREC_F32C || far64) && !(flags & VPM_UJ_NO_SORT);
     Staged st;
-    if ((rc = stage_field(f, cfg->kernel, cfg->precision, sort, st))) return rc;
-    if ((rc = launch_uj(f, st, 0, f->np, st, 0, f->np, cfg->kernel, cfg->precision))) return rc;
+    if ((rc = stage_field(f, cfg->kernel, prec, sort, st))) return rc;
+    if ((rc = launch_uj(f, st, 0, f->np, st, 0, f->np, cfg->kernel, prec))) return rc;
     if (flags & VPM_UJ_SFS) {
         const uint32_t* perm = nullptr;
         if (cfg->precision == VPM_PREC_F32C && (rc = sort_field(f, perm))) return rc;

@@ -30,6 +30,9 @@ constexpr int kBlock = 256;      // threads per block (4 waves of 64)
 constexpr int kTile = 256;       // sources per LDS tile
 constexpr int kAccF32 = 14;      // U'(3) W'(3) Jhv(8) per target, F32C path
 constexpr int kAccF64 = 12;      // U(3) J(9) per target, F64 path
+// internal precision of the FP64 far-form U/J kernel (vpm_uj_direct with
+// VPM_PREC_F64 and VPM_UJ_F64_FAR)
+constexpr int kPrecF64Far = 2;
 
 // Grow-only device buffer owned by a field.
 struct DevBuf {

@@ -21,6 +21,7 @@ const libvpm = get(ENV, "FLOWVPM_HIP_LIB", joinpath(@__DIR__, "..", "lib", "libf
 
 const VPM_OK = Cint(0)
 const VPM_UJ_RESET, VPM_UJ_RESET_SFS, VPM_UJ_SFS = UInt32(1), UInt32(2), UInt32(4)
+const VPM_UJ_F64_FAR = UInt32(16)
 
 struct VpmConfig
     kernel::Int32
@@ -84,16 +85,20 @@ download(pfield, row0, row1) = GC.@preserve pfield check(ccall((:vpm_get_particl
     (Ptr{Cvoid}, Ptr{Float64}, Int64, Int32, Int32), device_field(pfield), pfield.particles, pfield.np, row0, row1))
 
 """
-    UJ_hip(pfield; rbf=false, sfs=false, reset=true, reset_sfs=false, precision=0)
+    UJ_hip(pfield; rbf=false, sfs=false, reset=true, reset_sfs=false, precision=0, far64=false)
 
 UJ_direct semantics on the GPU: reset U, vorticity, J, PSE of non-static
 particles; SFS reset if `reset_sfs`; all-pairs U and J; E_str if `sfs`.
+`far64=true` (with `precision=1`) takes the FP64 far-form pair loop
+(VPM_UJ_F64_FAR): one rsqrt per pair, g = 1 past q = 10 (Gaussian-erf) or
+q^2 = 13 (Gaussian), the reference arithmetic to FP64 rounding.
 """
 function UJ_hip(pfield::vpm.ParticleField; rbf::Bool=false, sfs::Bool=false,
-                reset=true, reset_sfs=false, precision=0, optargs...)
+                reset=true, reset_sfs=false, precision=0, far64::Bool=false, optargs...)
     pfield.np == 0 && return nothing
     upload(pfield)
-    flags = (reset ? VPM_UJ_RESET : 0x0) | (reset_sfs ? VPM_UJ_RESET_SFS : 0x0) | (sfs ? VPM_UJ_SFS : 0x0)
+    flags = (reset ? VPM_UJ_RESET : 0x0) | (reset_sfs ? VPM_UJ_RESET_SFS : 0x0) | (sfs ? VPM_UJ_SFS : 0x0) |
+            (far64 ? VPM_UJ_F64_FAR : 0x0)
     cfg = Ref(config(pfield; precision))
     check(ccall((:vpm_uj_direct, libvpm), Cint, (Ptr{Cvoid}, Ref{VpmConfig}, UInt32),
                 device_field(pfield), cfg, flags))

@@ -46,9 +46,18 @@ enum { VPM_KERNEL_SINGULAR = 0, VPM_KERNEL_GAUSSIAN = 1,
  * FP64 pair arithmetic (fmm.jl:113-161 with custom_erf64). */
 enum { VPM_PREC_F32C = 0, VPM_PREC_F64 = 1 };
 
-/* flags of vpm_uj_direct: the keywords of UJ_direct (src/FLOWVPM_UJ.jl:21-25) */
+/* flags of vpm_uj_direct: the keywords of UJ_direct (src/FLOWVPM_UJ.jl:21-25),
+ * and two of this library's own:
+ *  VPM_UJ_NO_SORT  evaluate in storage order (no spatial sort);
+ *  VPM_UJ_F64_FAR  with VPM_PREC_F64 only: FP64 pair arithmetic with one
+ *                  rsqrt per pair and, for Gaussian-erf (q >= 10) and
+ *                  Gaussian (q^2 >= 13), the far form F' = r^-3, H'' = r^-5
+ *                  where the kernel's g is 1 and q g' is 0 to FP64 rounding;
+ *                  near pairs take the reference's g and dg.  It agrees with
+ *                  the reference arithmetic to FP64 rounding of the sums.
+ *                  Not taken by vpm_uj_direct_noalloc (VPM_EINVAL). */
 enum { VPM_UJ_RESET = 1, VPM_UJ_RESET_SFS = 2, VPM_UJ_SFS = 4,
-       VPM_UJ_NO_SORT = 8 /* evaluate in storage order (no spatial sort) */ };
+       VPM_UJ_NO_SORT = 8, VPM_UJ_F64_FAR = 16 };
 
 typedef struct {
     int32_t kernel;       /* VPM_KERNEL_*                                    */

@@ -1,0 +1,193 @@
+"""The FP64 far-form U/J kernel (vpm_uj_direct with VPM_PREC_F64 and
+VPM_UJ_F64_FAR, precision "f64far" in the Python API), through the C-ABI.
+
+It evaluates the pair of fmm.jl:113-161 in FP64 with one rsqrt per pair and,
+past q = 10 (Gaussian-erf) or q^2 = 13 (Gaussian), the far form g = 1,
+q g' = 0, exact there to FP64 rounding (1 - g < 2e-21, q g' < 6e-19); near
+pairs take the reference's g and dg.  So it is held to the bounds of the
+reference-arithmetic FP64 path: 1e-12 per particle against the oracle and
+the mpmath goldens (3e-7 / 3e-5 for the 16 golden particles at q = 1e-4,
+where the reference's own FP64 formula loses digits), and 1e-12 against
+that path on the jet column, where most pairs take the far form (at 2^20,
+U to its summation scale, as the jet's U sums cancel).  At the
+far threshold itself the pairs between two clusters agree with the
+reference arithmetic to a few ulps on either side."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+KERNELS = ["singular", "gaussian", "gaussianerf", "winckelmans"]
+F64_TOL = 1e-12
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _uj_err(vpm, A, R):
+    return (vpm.metrics.err_stats(A[:, 9:12], R[:, 9:12]), vpm.metrics.err_stats(A[:, 15:24], R[:, 15:24]))
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_golden_uj_f64far(vpm, kernel):
+    _say("golden f64far", kernel)
+    g = golden("uj_%s.json" % kernel)
+    P = vpm.fields.make_particles(g["X"], g["Gamma"], g["sigma"])
+    vpm.UJ_hip(P, kernel=kernel, precision="f64far")
+    U, J = np.array(g["U"]), np.array(g["J"])
+    eu = vpm.metrics.err_stats(P[:, 9:12], U)
+    ej = vpm.metrics.err_stats(P[:, 15:24], J)
+    _say("  U", eu, "J", ej)
+    loose = {"gaussian": 3e-5, "gaussianerf": 3e-7}.get(kernel, 1e-11)
+    assert eu["max"] < loose and ej["max"] < loose
+    eu2 = vpm.metrics.err_stats(P[16:, 9:12], U[16:])
+    ej2 = vpm.metrics.err_stats(P[16:, 15:24], J[16:])
+    assert eu2["max"] < F64_TOL and ej2["max"] < F64_TOL
+
+
+def test_two_body_check_fmm_f64far(vpm):
+    g = golden("check_fmm.json")
+    P = vpm.fields.make_particles(g["X"], g["Gamma"], g["sigma"])
+    vpm.UJ_hip(P, kernel="singular", precision="f64far")
+    eu = vpm.metrics.err_stats(P[:, 9:12], g["U"])
+    ej = vpm.metrics.err_stats(P[:, 15:24], g["J"])
+    assert eu["max"] < 1e-13 and ej["max"] < 1e-13, (eu, ej)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_cloud_vs_oracle_and_reference_arithmetic(vpm, orc, kernel):
+    P = vpm.workloads.random_cloud(3000, seed=11)
+    ref = P.copy()
+    orc.uj_direct(ref, kernel)
+    A, B = P.copy(), P.copy()
+    vpm.UJ_hip(A, kernel=kernel, precision="f64far")
+    vpm.UJ_hip(B, kernel=kernel, precision="f64")
+    eu, ej = _uj_err(vpm, A, ref)
+    eu2, ej2 = _uj_err(vpm, A, B)
+    _say("cloud N=3000", kernel, "f64far vs oracle U", eu, "J", ej, "vs f64 U", eu2, "J", ej2)
+    assert eu["max"] < F64_TOL and ej["max"] < F64_TOL
+    assert eu2["max"] < F64_TOL and ej2["max"] < F64_TOL
+
+
+def test_edge_sizes_coincident_and_storage_order(vpm, orc):
+    """Sizes around the wave and tile widths, coincident particles (skipped,
+    fmm.jl:118), and the storage-order form (no spatial sort)."""
+    for n in (1, 2, 63, 65, 255, 257, 700):
+        P = vpm.workloads.random_cloud(n, seed=n)
+        if n > 8:
+            P[n // 2, 0:3] = P[1, 0:3]           # a coincident pair
+        for kernel in ("gaussianerf", "gaussian", "winckelmans"):
+            ref = P.copy()
+            orc.uj_direct(ref, kernel)
+            for sort in (True, False):
+                A = P.copy()
+                vpm.UJ_hip(A, kernel=kernel, precision="f64far", sort=sort)
+                assert np.all(np.isfinite(A[:, 9:24]))
+                if n == 1:
+                    assert np.all(A[:, 9:24] == 0)
+                    continue
+                eu, ej = _uj_err(vpm, A, ref)
+                assert eu["max"] < F64_TOL and ej["max"] < F64_TOL, (n, kernel, sort, eu, ej)
+
+
+@pytest.mark.parametrize("kernel,qf", [("gaussianerf", 10.0), ("gaussian", 13.0 ** 0.5)])
+def test_far_threshold_pairs(vpm, kernel, qf):
+    """Two clusters of 64 coincident particles (coincident pairs are skipped,
+    so each wave's target box is one point) at q just inside and just
+    outside the far threshold and well on either side of it: every pair
+    between the clusters takes the near form or the far form, against the
+    reference arithmetic to a few ulps."""
+    sig = 0.02
+    rng = np.random.default_rng(5)
+    G = rng.normal(size=(128, 3))
+    e = np.array([1.0, -1.0, 1.0]) / 3 ** 0.5
+    for q in (qf * (1 - 1e-9), qf * (1 + 1e-9), 0.5 * qf, 3 * qf):
+        X = np.vstack([np.tile([0.1, 0.2, 0.3], (64, 1)), np.tile(np.array([0.1, 0.2, 0.3]) + q * sig * e, (64, 1))])
+        P = vpm.fields.make_particles(X, G, np.full(128, sig))
+        A, B = P.copy(), P.copy()
+        vpm.UJ_hip(A, kernel=kernel, precision="f64far")
+        vpm.UJ_hip(B, kernel=kernel, precision="f64")
+        eu, ej = _uj_err(vpm, A, B)
+        _say(kernel, "q/q_far %.9f" % (q / qf), "U", eu["max"], "J", ej["max"])
+        assert np.all(np.abs(B[:, 9:12]).max(axis=1) > 0)
+        assert eu["max"] < 1e-14 and ej["max"] < 1e-14, (q, eu, ej)
+
+
+def test_jet_column_far_form(vpm, orc):
+    """The jet column at 2^16 particles (sigma = 0.015 at a spacing of ~0.036:
+    almost every pair beyond q = 10), Gaussian-erf: every particle against
+    the reference-arithmetic FP64 path, a sample against the oracle."""
+    P = vpm.workloads.jet_column(1 << 16)
+    A, B = P.copy(), P.copy()
+    vpm.UJ_hip(A, kernel="gaussianerf", precision="f64far")
+    vpm.UJ_hip(B, kernel="gaussianerf", precision="f64")
+    eu, ej = _uj_err(vpm, A, B)
+    idx = np.arange(0, P.shape[0], 97)
+    sub = P[idx].copy()
+    sub[:, 9:24] = 0
+    orc.direct(sub, "gaussianerf", source=P)
+    eu2, ej2 = _uj_err(vpm, A[idx], sub)
+    _say("jet 2^16 f64far vs f64 U", eu, "J", ej, "vs oracle sample U", eu2, "J", ej2)
+    assert eu["max"] < F64_TOL and ej["max"] < F64_TOL
+    assert eu2["max"] < F64_TOL and ej2["max"] < F64_TOL
+
+
+def test_full_size_jet_f64far(vpm, orc):
+    """The 2^20 jet column, Gaussian-erf.  The far-form path sums in another
+    order than the reference-arithmetic path (sorted sources, the 14 sums of
+    the F32C kernel), and the jet's U sums cancel, so U is held to its
+    summation scale s_i = sum_j |U term_ij| (`oracle.u_scale_ranges`) on four
+    windows of 64 targets against the oracle, as is the reference-arithmetic
+    path.  Each side sums 2^20 terms (the oracle in one sequential chain),
+    whose rounding walks to ~sqrt(2^20) eps = 1.1e-13 of the scale (both GPU
+    paths measured 1.0e-13): |dU_i| <= 5e-13 s_i, and the far-form path no
+    worse than 1.5 times the reference-arithmetic one.  Over every particle:
+    J within 1e-12 of the reference-arithmetic path, U within 1e-13 RMS."""
+    kern = "gaussianerf"
+    P = vpm.workloads.jet_column(1 << 20)
+    n = P.shape[0]
+    A, B = P.copy(), P.copy()
+    vpm.UJ_hip(A, kernel=kern, precision="f64far")
+    vpm.UJ_hip(B, kernel=kern, precision="f64")
+    eu, ej = _uj_err(vpm, A, B)
+    _say("jet 2^20 f64far vs f64 U", eu, "J", ej)
+    assert eu["rms"] < 1e-13 and ej["max"] < F64_TOL, (eu, ej)
+    worst = {"f64far": 0.0, "f64": 0.0}
+    for t0 in (0, n // 3, n // 2 + 12345, n - 64):
+        t1 = t0 + 64
+        ref = P[t0:t1].copy()
+        ref[:, 9:24] = 0
+        orc.direct(ref, kern, source=P)
+        s = orc.u_scale_ranges(P, kern, [t0], [t1], [0], [n])[t0:t1]
+        for name, X in (("f64far", A), ("f64", B)):
+            e = np.linalg.norm(X[t0:t1, 9:12] - ref[:, 9:12], axis=1)
+            worst[name] = max(worst[name], float((e / s).max()))
+            ejw = vpm.metrics.err_stats(X[t0:t1, 15:24], ref[:, 15:24])
+            assert ejw["max"] < F64_TOL, (name, t0, ejw)
+    _say("jet 2^20 U err/scale vs oracle windows: f64far %.2e, f64 %.2e" % (worst["f64far"], worst["f64"]))
+    assert worst["f64far"] < 5e-13 and worst["f64"] < 5e-13, worst
+    assert worst["f64far"] <= 1.5 * worst["f64"], worst
+
+
+def test_f64far_argument_checks(vpm):
+    """VPM_UJ_F64_FAR needs VPM_PREC_F64, and the no-allocation entry point
+    does not take it: VPM_EINVAL before any device work."""
+    api = vpm.api
+    P = vpm.workloads.random_cloud(100, seed=3)
+    f = api.Field(100, 0)
+    try:
+        f.set_particles(P)
+        for prec, fn in (("f32c", api.lib().vpm_uj_direct), ("f64", api.lib().vpm_uj_direct_noalloc)):
+            cfg = api.make_config("gaussianerf", prec, True)
+            rc = fn(f.handle, C.byref(cfg), api._lib.UJ_RESET | api._lib.UJ_F64_FAR)
+            msg = api.lib().vpm_last_error()
+            _say(prec, rc, msg)
+            assert rc == api._lib.VPM_EINVAL and b"VPM_UJ_F64_FAR" in msg, (rc, msg)
+        with pytest.raises(ValueError):
+            api.make_config("gaussianerf", "f64far", True)
+    finally:
+        f.close()
