@@ -471,6 +471,17 @@ __device__ __forceinline__ double far_cut64(double sigma)
     return Far64<K>::cheap ? Far64<K>::q2 * sigma * sigma : DBL_MAX;
 }
 
+// 1/sqrt(x) for finite x > 0: the hardware estimate y and one third-order
+// step y (1 + e/2 + 3 e^2/8), e = 1 - x y^2 (Householder's iteration for
+// x^-1/2; ocml's rsqrt takes the same step and adds a guard for x = 0 and
+// x = inf, which the pairs here never pass)
+__device__ __forceinline__ double rsqrt_pos(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * y, y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
 // accum_pair in FP64 (the 14 sums of add_f32c_to_record)
 __device__ __forceinline__ void accum_pair64(double dx, double dy, double dz, double Fp, double Hpp,
                                              const double4 b, double (&acc)[kAccF32])
@@ -503,7 +514,7 @@ __device__ __forceinline__ void near_pair64(double tx, double ty, double tz, con
     const double dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
     const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
     if (r2 > 0.0) {   // coincident pairs contribute nothing (fmm.jl:118)
-        const double ri = rsqrt(r2);
+        const double ri = rsqrt_pos(r2);
         const double q = r2 * ri * b.w;
         double g, dg;
         g_dgdr_ref<K>(q, g, dg);
@@ -520,7 +531,7 @@ __device__ __forceinline__ void far_pair64(double tx, double ty, double tz, cons
     if constexpr (Far64<K>::cheap) {
         const double dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-        const double ri = rsqrt(r2);
+        const double ri = rsqrt_pos(r2);
         const double ri2 = ri * ri;
         const double Fp = ri2 * ri;
         accum_pair64(dx, dy, dz, Fp, Fp * ri2, b, acc);
