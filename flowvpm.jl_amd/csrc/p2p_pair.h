@@ -459,7 +459,8 @@ __device__ __forceinline__ void pair_f64(double tx, double ty, double tz, const 
 //  * Gaussian, q^2 >= 13 (q >= 3.605): e^{-q^3} < 4e-21, 3 q^3 e^{-q^3} < 6e-19;
 //  * Winckelmans (g -> 1 only as 1 - 1.9/q^4) and singular (r = 0 guard):
 //    every pair takes the near form.
-// Near pairs use the reference's g and dg/dq (g_dgdr_ref) at q = r / sigma.
+// Near pairs use the reference's g and dg/dq (g_dgdr_ref) at q = r / sigma,
+// except Winckelmans, whose reduced form has no cancellation and no division.
 template <int K> struct Far64 { static constexpr double q2 = 0.0; static constexpr bool cheap = false; };
 template <> struct Far64<VPM_KERNEL_GAUSSIANERF> { static constexpr double q2 = 100.0; static constexpr bool cheap = true; };
 template <> struct Far64<VPM_KERNEL_GAUSSIAN> { static constexpr double q2 = 13.0; static constexpr bool cheap = true; };
@@ -513,6 +514,21 @@ __device__ __forceinline__ void near_pair64(double tx, double ty, double tz, con
 {
     const double dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
     const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+    if constexpr (K == VPM_KERNEL_WINCKELMANS) {
+        // g/q^3 = (t + 5/2)/(t + 1)^{5/2} and (g - q g'/3)/q^5 =
+        // (t + 7/2)/(t + 1)^{7/2}, t = q^2 (kernel.jl:69-74 reduced; no
+        // cancellation at any q)
+        if (r2 > 0.0) {   // coincident pairs contribute nothing (fmm.jl:118)
+            const double s2 = b.w * b.w;
+            const double t = r2 * s2;
+            const double sr = rsqrt_pos(t + 1.0);
+            const double w = sr * sr;
+            const double w2s = w * w * sr;
+            const double s3 = s2 * b.w;
+            accum_pair64(dx, dy, dz, (t + 2.5) * w2s * s3, (t + 3.5) * w2s * w * s3 * s2, b, acc);
+        }
+        return;
+    }
     if (r2 > 0.0) {   // coincident pairs contribute nothing (fmm.jl:118)
         const double ri = rsqrt_pos(r2);
         const double q = r2 * ri * b.w;
