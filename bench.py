@@ -217,8 +217,9 @@ def leaf_list(vpm, P, kernel, allpairs_rate, leaf=64, reps=3):
 
 def cfg1_cloud(vpm, n=65536, reps=5, cpu=True):
     """Random cloud N = 65,536 (X ~ U[0,1]^3, Gamma ~ N(0, 1e-3^2), sigma ~
-    U[0.02, 0.04], seed 1): one U+J evaluation per kernel in FP32c and FP64 on
-    the GPU, throughput and per-output error of FP32c against FP64."""
+    U[0.02, 0.04], seed 1): one U+J evaluation per kernel in FP32c, FP64 and
+    FP64 with the far form on the GPU, throughput and per-output error of
+    FP32c (and of the far form) against FP64."""
     api = vpm.api
     P = vpm.workloads.random_cloud(n, seed=1)
     f = api.Field(n, 0)
@@ -227,7 +228,7 @@ def cfg1_cloud(vpm, n=65536, reps=5, cpu=True):
     for kernel in ("gaussianerf", "winckelmans"):
         r = {}
         rows = {}
-        for prec in ("f32c", "f64"):
+        for prec in ("f32c", "f64", "f64far"):
             f.uj_direct(kernel, prec)
             f.synchronize()
             t0 = time.perf_counter()
@@ -249,6 +250,8 @@ def cfg1_cloud(vpm, n=65536, reps=5, cpu=True):
             "J": vpm.metrics.err_stats(A[:, 15:24], R[:, 15:24]),
             "dGamma_dt": vpm.metrics.err_stats(S32, S64),
             "dGamma_dt_vs_J_Gamma_scale": {"max": float(es.max()), "rms": float(np.sqrt(np.mean(es * es)))}}
+        r["error_f64far_vs_f64"] = {"U": vpm.metrics.err_stats(rows["f64far"][:, 9:12], R[:, 9:12]),
+                                    "J": vpm.metrics.err_stats(rows["f64far"][:, 15:24], R[:, 15:24])}
         if cpu:
             nthreads, info = host_threads()
             rate, nt, dt, Q = cpu_uj_rate(P, kernel, nthreads, 3.0)
