@@ -456,7 +456,7 @@ __device__ __forceinline__ void pair_f64(double tx, double ty, double tz, const 
 // is 1 and q g' is 0 to FP64 rounding, the far form F' = r^-3, H'' = r^-5:
 //  * Gaussian-erf, q >= 10: 1 - g = erfc(q/sqrt2) + sqrt(2/pi) q e^{-q^2/2}
 //    < 2e-21 and q g' = sqrt(2/pi) q^3 e^{-q^2/2} < 2e-19;
-//  * Gaussian, q^2 >= 13 (q >= 3.605): e^{-q^3} < 4e-21, 3 q^3 e^{-q^3} < 6e-19;
+//  * Gaussian, q^2 >= 13 (q >= 3.605): e^{-q^3} < 5e-21, 3 q^3 e^{-q^3} < 7e-19;
 //  * Winckelmans (g -> 1 only as 1 - 1.9/q^4) and singular (r = 0 guard):
 //    every pair takes the near form.
 // Near pairs use the reference's g and dg/dq (g_dgdr_ref) at q = r / sigma,

@@ -3,7 +3,7 @@ VPM_UJ_F64_FAR, precision "f64far" in the Python API), through the C-ABI.
 
 It evaluates the pair of fmm.jl:113-161 in FP64 with one rsqrt per pair and,
 past q = 10 (Gaussian-erf) or q^2 = 13 (Gaussian), the far form g = 1,
-q g' = 0, exact there to FP64 rounding (1 - g < 2e-21, q g' < 6e-19); near
+q g' = 0, exact there to FP64 rounding (1 - g < 5e-21, q g' < 7e-19); near
 pairs take the reference's g and dg.  So it is held to the bounds of the
 reference-arithmetic FP64 path: 1e-12 per particle against the oracle and
 the mpmath goldens (3e-7 / 3e-5 for the 16 golden particles at q = 1e-4,
