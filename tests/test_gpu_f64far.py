@@ -173,6 +173,22 @@ def test_full_size_jet_f64far(vpm, orc):
     assert worst["f64far"] <= 1.5 * worst["f64"], worst
 
 
+def test_f64far_with_sfs_and_accumulate(vpm):
+    """UJ_direct's other flags with the far form: E_str (sfs) runs in FP64 on
+    the far-form J, and reset=False accumulates onto what is there."""
+    P = vpm.workloads.random_cloud(2000, seed=21)
+    A, B = P.copy(), P.copy()
+    vpm.UJ_hip(A, kernel="gaussianerf", precision="f64far", sfs=True, reset_sfs=True)
+    vpm.UJ_hip(B, kernel="gaussianerf", precision="f64", sfs=True, reset_sfs=True)
+    eu, ej = _uj_err(vpm, A, B)
+    es = vpm.metrics.err_stats(A[:, 39:42], B[:, 39:42])
+    _say("sfs: U", eu["max"], "J", ej["max"], "E_str", es)
+    assert eu["max"] < F64_TOL and ej["max"] < F64_TOL and es["max"] < 1e-10, (eu, ej, es)
+    C2 = A.copy()
+    vpm.UJ_hip(C2, kernel="gaussianerf", precision="f64far", reset=False)
+    assert np.allclose(C2[:, 9:12], 2 * A[:, 9:12], rtol=1e-13, atol=0), np.abs(C2[:, 9:12] - 2 * A[:, 9:12]).max()
+
+
 def test_f64far_argument_checks(vpm):
     """VPM_UJ_F64_FAR needs VPM_PREC_F64, and the no-allocation entry point
     does not take it: VPM_EINVAL before any device work."""
