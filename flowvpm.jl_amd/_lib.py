@@ -37,7 +37,7 @@ class StepConfig(C.Structure):
                 ("relax", C.c_int32),
                 ("sfs_alpha", C.c_double), ("sfs_rlxf", C.c_double), ("sfs_minC", C.c_double),
                 ("sfs_maxC", C.c_double), ("rlxf", C.c_double),
-                ("viscous", C.c_int32), ("reserved", C.c_int32),
+                ("viscous", C.c_int32), ("uj_flags", C.c_int32),
                 ("nu", C.c_double), ("sgm0", C.c_double), ("beta", C.c_double), ("t_sgm", C.c_double),
                 ("rbf_itmax", C.c_int32), ("rbf_iterror", C.c_int32), ("rbf_tol", C.c_double)]
 

@@ -306,8 +306,10 @@ class RangePlan:
 def make_config(kernel=kernel_default, precision="f32c", transposed=True):
     cfg = _lib.Config()
     cfg.kernel = _lib.KERNELS[kernel]
+    if precision == "f64far":     # FP64; the far form is a flag of the U/J calls
+        precision = "f64"
     if precision not in _lib.PRECISIONS:
-        raise ValueError("precision %r: one of %s (and \"f64far\" for U/J only)" % (precision, sorted(_lib.PRECISIONS)))
+        raise ValueError("precision %r: one of %s or \"f64far\"" % (precision, sorted(_lib.PRECISIONS)))
     cfg.precision = _lib.PRECISIONS[precision]
     cfg.transposed = int(bool(transposed))
     return cfg
@@ -555,6 +557,7 @@ def Estr_hip(pfield, *, kernel=None, precision=None, transposed=None, device=Non
 def step_config(pfield):
     s = _lib.StepConfig()
     s.uj = make_config(pfield.kernel, pfield.precision, pfield.transposed)
+    s.uj_flags = _lib.UJ_F64_FAR if pfield.precision == "f64far" else 0
     s.f, s.g = pfield.formulation.f, pfield.formulation.g
     s.Uinf[:] = [float(u) for u in pfield.Uinf(pfield.t)]
     sfs = pfield.SFS

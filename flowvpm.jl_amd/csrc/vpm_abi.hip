@@ -152,13 +152,15 @@ static int uj_direct_locked(vpm_field f, const vpm_config* cfg, uint32_t flags)
     return VPM_OK;
 }
 
-static int sfs_before_locked(vpm_field f, const vpm_config* cfg, double alpha, double minC, double maxC)
+// uj_flags: VPM_UJ_F64_FAR or 0, as the step's own U/J evaluations
+static int sfs_before_locked(vpm_field f, const vpm_config* cfg, double alpha, double minC, double maxC,
+                             uint32_t uj_flags = 0)
 {
     if (minC < 0 || maxC < 0 || minC > maxC)                         // subfilterscale.jl:455-461
         return fail(VPM_EINVAL, "Invalid C bounds");
     int rc;
     if ((rc = launch_scale_sigma(f, alpha, 0))) return rc;
-    if ((rc = uj_direct_locked(f, cfg, VPM_UJ_RESET | VPM_UJ_RESET_SFS | VPM_UJ_SFS))) return rc;
+    if ((rc = uj_direct_locked(f, cfg, VPM_UJ_RESET | VPM_UJ_RESET_SFS | VPM_UJ_SFS | uj_flags))) return rc;
     if ((rc = launch_sfs_store_test(f, cfg->transposed))) return rc;
     return launch_scale_sigma(f, alpha, 1);
 }
@@ -635,6 +637,9 @@ static int rk3_step_locked(vpm_field f, vpm_step_config* s, double dt, int32_t r
 {
     const vpm_config* cfg = &s->uj;
     if (int rc = check_cfg(cfg)) return rc;
+    if ((s->uj_flags & ~(int32_t)VPM_UJ_F64_FAR) != 0)
+        return fail(VPM_EINVAL, "vpm_step_config.uj_flags: only VPM_UJ_F64_FAR is taken");
+    if (s->uj_flags && cfg->precision != VPM_PREC_F64) return fail(VPM_EINVAL, "VPM_UJ_F64_FAR needs VPM_PREC_F64");
     if (s->viscous == 1 && cfg->kernel != VPM_KERNEL_GAUSSIANERF)   // FLOWVPM.jl:265-267, utils.jl:58-64
         return fail(VPM_EINVAL, "CoreSpreading requires the Gaussian-erf kernel");
     const double zeta0 = zeta0_of(cfg->kernel);                    // timeintegration.jl:398
@@ -660,8 +665,9 @@ static int rk3_step_locked(vpm_field f, vpm_step_config* s, double dt, int32_t r
                               cfg->kernel != VPM_KERNEL_SINGULAR && 1.0 - s->sfs_alpha <= 0.01 &&
                               s->sfs_alpha < 1.0;
         if (sfs && first && !sfs_diff)
-            if ((rc = sfs_before_locked(f, cfg, s->sfs_alpha, s->sfs_minC, s->sfs_maxC))) return rc;
-        const uint32_t fl = VPM_UJ_RESET | (sfs ? (VPM_UJ_RESET_SFS | VPM_UJ_SFS) : 0u);
+            if ((rc = sfs_before_locked(f, cfg, s->sfs_alpha, s->sfs_minC, s->sfs_maxC, (uint32_t)s->uj_flags)))
+                return rc;
+        const uint32_t fl = VPM_UJ_RESET | (sfs ? (VPM_UJ_RESET_SFS | VPM_UJ_SFS) : 0u) | (uint32_t)s->uj_flags;
         if ((rc = uj_direct_locked(f, cfg, fl))) return rc;        // :422
         if (tape && f->np > 0) {
             hipError_t e = hipMemcpyAsync((char*)tape->rec.p + st * rec, f->P, rec, hipMemcpyDeviceToDevice,
@@ -701,7 +707,7 @@ static int rk3_step_locked(vpm_field f, vpm_step_config* s, double dt, int32_t r
     }
     if (relax_now && s->relax) {                                    // :437-458
         if ((rc = launch_reset(f, 0))) return rc;
-        if ((rc = uj_direct_locked(f, cfg, VPM_UJ_RESET))) return rc;
+        if ((rc = uj_direct_locked(f, cfg, VPM_UJ_RESET | (uint32_t)s->uj_flags))) return rc;
         if ((rc = launch_relax(f, s->rlxf, s->relax == 2))) return rc;
     }
     if ((rc = launch_update_uprev(f))) return rc;                   // particlefield.jl:443-455

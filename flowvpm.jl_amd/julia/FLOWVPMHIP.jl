@@ -36,7 +36,7 @@ mutable struct VpmStepConfig
     sfs::Int32; sfs_force_positive::Int32; sfs_clip::Int32; relax::Int32
     sfs_alpha::Float64; sfs_rlxf::Float64; sfs_minC::Float64; sfs_maxC::Float64
     rlxf::Float64
-    viscous::Int32; reserved::Int32
+    viscous::Int32; uj_flags::Int32     # 0 or VPM_UJ_F64_FAR
     nu::Float64; sgm0::Float64; beta::Float64; t_sgm::Float64
     rbf_itmax::Int32; rbf_iterror::Int32; rbf_tol::Float64   # CoreSpreading itmax, iterror, tol
 end
@@ -371,10 +371,12 @@ function rbf_hip(pfield::vpm.ParticleField, cs; precision=0)
 end
 
 """
-    rungekutta3_hip(pfield, dt; relax=false, precision=0)
+    rungekutta3_hip(pfield, dt; relax=false, precision=0, far64=false)
 
 One rungekutta3 step of ReformulatedVPM with UJ = UJ_direct, entirely on the
-device (pfield.particles is uploaded once and downloaded once).
+device (pfield.particles is uploaded once and downloaded once).  With
+`precision=1, far64=true` the step's U/J evaluations take the FP64 far form
+(VPM_UJ_F64_FAR); everything else runs in FP64 as with `precision=1`.
 """
 """
     sfs_settings(SFS) -> (sfs, force_positive, clip)
@@ -407,7 +409,8 @@ function sfs_settings(SFS)
     return (true, force_positive, clip)
 end
 
-function rungekutta3_hip(pfield::vpm.ParticleField, dt::Real; relax::Bool=false, precision=0, optargs...)
+function rungekutta3_hip(pfield::vpm.ParticleField, dt::Real; relax::Bool=false, precision=0, far64::Bool=false,
+                         optargs...)
     sfs, force_positive, clip = sfs_settings(pfield.SFS)
     upload(pfield)
     visc = pfield.viscous isa vpm.CoreSpreading
@@ -418,7 +421,8 @@ function rungekutta3_hip(pfield::vpm.ParticleField, dt::Real; relax::Bool=false,
             Int32(sfs), Int32(force_positive), Int32(clip), Int32(method),
             sfs ? pfield.SFS.alpha : 0.0, sfs ? pfield.SFS.rlxf : 0.0,
             sfs ? pfield.SFS.minC : 0.0, sfs ? pfield.SFS.maxC : 1.0, rl.rlxf,
-            Int32(visc), 0, visc ? pfield.viscous.nu : 0.0, visc ? pfield.viscous.sgm0 : 1.0,
+            Int32(visc), Int32(far64 ? VPM_UJ_F64_FAR : 0), visc ? pfield.viscous.nu : 0.0,
+            visc ? pfield.viscous.sgm0 : 1.0,
             visc ? pfield.viscous.beta : 1.5, visc ? pfield.viscous.t_sgm : 0.0,
             Int32(visc ? pfield.viscous.itmax : 0), Int32(visc && pfield.viscous.iterror),
             visc ? Float64(pfield.viscous.tol) : 0.0))

@@ -78,7 +78,8 @@ typedef struct {
     double sfs_alpha, sfs_rlxf, sfs_minC, sfs_maxC;
     double rlxf;
     int32_t viscous;           /* 0 Inviscid, 1 CoreSpreading (viscous.jl:65-223) */
-    int32_t reserved;
+    int32_t uj_flags;          /* 0, or VPM_UJ_F64_FAR (with uj.precision VPM_PREC_F64):
+                                  the step's U/J evaluations take the FP64 far form */
     double nu, sgm0, beta;
     double t_sgm;              /* core-spreading timer; updated in place */
     /* CoreSpreading's RBF reset (viscous.jl:202-220, 314-478): itmax, tol,

@@ -190,8 +190,9 @@ def test_f64far_with_sfs_and_accumulate(vpm):
 
 
 def test_f64far_argument_checks(vpm):
-    """VPM_UJ_F64_FAR needs VPM_PREC_F64, and the no-allocation entry point
-    does not take it: VPM_EINVAL before any device work."""
+    """VPM_UJ_F64_FAR needs VPM_PREC_F64, in vpm_uj_direct and in the step's
+    uj_flags (which take no other bit), and the no-allocation entry point does
+    not take it: VPM_EINVAL before any device work."""
     api = vpm.api
     P = vpm.workloads.random_cloud(100, seed=3)
     f = api.Field(100, 0)
@@ -203,7 +204,41 @@ def test_f64far_argument_checks(vpm):
             msg = api.lib().vpm_last_error()
             _say(prec, rc, msg)
             assert rc == api._lib.VPM_EINVAL and b"VPM_UJ_F64_FAR" in msg, (rc, msg)
-        with pytest.raises(ValueError):
-            api.make_config("gaussianerf", "f64far", True)
+        for prec, flags, what in (("f32c", api._lib.UJ_F64_FAR, b"VPM_UJ_F64_FAR"), ("f64", 4, b"uj_flags")):
+            s = api._lib.StepConfig()
+            s.uj = api.make_config("gaussianerf", prec, True)
+            s.f, s.g, s.relax, s.uj_flags = 0.0, 0.2, 0, flags
+            rc = api.lib().vpm_rk3_step(f.handle, C.byref(s), 0.01, 0)
+            msg = api.lib().vpm_last_error()
+            assert rc == api._lib.VPM_EINVAL and what in msg, (rc, msg)
+        assert api.make_config("gaussianerf", "f64far", True).precision == api._lib.PRECISIONS["f64"]
     finally:
         f.close()
+
+
+def test_rk3_steps_f64far_vs_f64(vpm, orc):
+    """Two rungekutta3 steps of the config-2 physics (rVPM, the two-level
+    dynamic SFS, CoreSpreading, Pedrizzetti relaxation) on a reduced leapfrog
+    pair, with the step's U/J in the FP64 far form (ParticleField precision
+    "f64far"), against the oracle's steps: the FP64 bound of the
+    reference-arithmetic step, 1e-10 on the state and 1e-8 on C."""
+    api = vpm.api
+    P = vpm.workloads.leapfrog_cfg2(Nphi=64)                # 8,192 particles
+    sg0 = float(P[0, 6])
+    pf = api.ParticleField(P.shape[0], kernel="gaussianerf", formulation=api.rVPM, relaxation=api.pedrizzetti,
+                           SFS=api.SFS_Cd_twolevel_nobackscatter, viscous=api.CoreSpreading(1e-4, sg0),
+                           precision="f64far")
+    api.add_particles(pf, P)
+    s = orc.make_settings(kernel="gaussianerf", f=0.0, g=0.2, relax=1, rlxf=0.3, sfs=1, sfs_alpha=0.999,
+                          sfs_rlxf=0.005, sfs_minC=0.0, sfs_maxC=1.0, sfs_force_positive=True,
+                          sfs_clip=True, viscous=1, nu=1e-4, sgm0=sg0, beta=1.5)
+    R = P.copy()
+    for step in range(2):
+        assert orc.rk3_step(R, s, 0.01, relax_now=True) == 0
+        api.nextstep(pf, 0.01, relax=True)
+    A = pf.particles[:pf.np]
+    e = {name: vpm.metrics.err_stats(A[:, sl], R[:, sl])["max"]
+         for name, sl in (("X", slice(0, 3)), ("Gamma", slice(3, 6)), ("sigma", slice(6, 7)))}
+    Ce = vpm.metrics.err_stats(A[:, 36:39], R[:, 36:39])["max"]
+    _say("rk3 f64far", e, "C", Ce)
+    assert max(e.values()) < 1e-10 and Ce < 1e-8, (e, Ce)
