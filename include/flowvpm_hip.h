@@ -134,7 +134,7 @@ int vpm_uj_direct(vpm_field f, const vpm_config* cfg, uint32_t flags);
  * GPU that the library holds until the process ends.  np above what fits
  * (about 3.4e5 particles for F32C, 4.9e5 for F64, with one source split)
  * returns VPM_ERANGE; fewer source splits than vpm_uj_direct takes are used
- * when its own do not fit.  VPM_UJ_SFS is rejected (VPM_EINVAL).  Calls on
+ * when its own do not fit.  VPM_UJ_SFS and VPM_UJ_F64_FAR are rejected (VPM_EINVAL).  Calls on
  * one GPU share the array and are serialized by the library (overlapping
  * ones would race on it). */
 int vpm_uj_direct_noalloc(vpm_field f, const vpm_config* cfg, uint32_t flags);
