@@ -327,6 +327,154 @@ __global__ __launch_bounds__(kBlock) void estr_f64_kernel(double* __restrict__ P
 
 static inline unsigned nblk(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
+// ------------------------------------------------------ F64 with a far cut
+// E_str in FP64 for VPM_UJ_F64_FAR: the reference's pair expression
+// (subfilterscale_models.jl:22-40) with zeta_sigma from r^2 (no sqrt or
+// division for Gaussian-erf and Winckelmans), sources in the U/J kernels'
+// spatial order, and only the source tiles and the pairs within q_cut:
+// Gaussian-erf q_cut = 10 (zeta(10)/zeta(0) = e^-50 = 2e-22), Gaussian
+// q_cut^2 = 14 (e^-52.4 = 2e-23), below FP64 resolution of the near terms.
+// Winckelmans (zeta ~ q^-7) has no cut; the singular kernel's zeta is
+// nonzero at r = 0 only, so only coincident pairs are evaluated.
+template <int K>
+__device__ __forceinline__ double estr_q2cut64()
+{
+    return K == VPM_KERNEL_GAUSSIANERF ? 100.0 : K == VPM_KERNEL_GAUSSIAN ? 14.0 : 0.0;
+}
+
+// r^2 cut of a source from its 1/sigma (-1: a static source, never paired)
+template <int K>
+__device__ __forceinline__ double estr_cut64(double si)
+{
+    if (!(si > 0.0)) return -1.0;
+    if constexpr (K == VPM_KERNEL_WINCKELMANS) return DBL_MAX;
+    else if constexpr (K == VPM_KERNEL_SINGULAR) return DBL_MIN;
+    else return estr_q2cut64<K>() / (si * si);
+}
+
+// zeta(r/sigma)/sigma^3 (kernel.jl:51,60,69-74,78)
+template <int K>
+__device__ __forceinline__ double zeta64(double r2, double si)
+{
+    const double s3 = si * si * si;
+    if constexpr (K == VPM_KERNEL_SINGULAR) {
+        return r2 == 0.0 ? s3 : 0.0;
+    } else if constexpr (K == VPM_KERNEL_GAUSSIAN) {
+        const double q = (r2 > 0.0 ? r2 * rsqrt_pos(r2) : 0.0) * si;
+        return kConst3 * exp(-q * q * q) * s3;
+    } else if constexpr (K == VPM_KERNEL_GAUSSIANERF) {
+        return kConst1 * exp(-0.5 * r2 * si * si) * s3;
+    } else {
+        const double w = rsqrt_pos(fma(r2, si * si, 1.0));   // (q^2 + 1)^-1/2
+        const double w2 = w * w;
+        return kConst4 * 7.5 * (w2 * w2 * w2 * w) * s3;
+    }
+}
+
+// Sources of the far-cut kernel in the order of perm: positions and J in
+// FP64 (SoA, stride ld), (Gamma, 1/sigma) with 1/sigma = 0 for static
+// sources, and the r^2 cut in FP32 (rounded up) for the tile boxes.
+template <int K>
+__global__ __launch_bounds__(kBlock) void pack_estr64_kernel(const double* __restrict__ P, int64_t np,
+                                                             const uint32_t* __restrict__ perm,
+                                                             double* __restrict__ pos, double* __restrict__ J,
+                                                             int64_t ld, double4* __restrict__ g4,
+                                                             float* __restrict__ cutf)
+{
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= np) return;
+    const int64_t p = perm ? (int64_t)perm[k] : k;
+    const double* rec = P + p * NF;
+    pos[k] = rec[RX];
+    pos[ld + k] = rec[RX + 1];
+    pos[2 * ld + k] = rec[RX + 2];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) J[c * ld + k] = rec[RJ + c];
+    const double si = rec[RSTATIC] != 0 ? 0.0 : 1.0 / rec[RSIG];
+    g4[k] = make_double4(rec[RG], rec[RG + 1], rec[RG + 2], si);
+    const double c = estr_cut64<K>(si);
+    cutf[k] = c < 0.0 ? 0.f : c >= (double)FLT_MAX ? FLT_MAX : (float)c * 1.0001f;
+}
+
+// One block of 256 targets (sorted) over the near source tiles (NearTiles on
+// the tile boxes); in a tile, a source is evaluated when some lane of the
+// wave is within its cut, and each lane adds the pair only within the cut.
+template <int K>
+__global__ __launch_bounds__(kBlock) void estr_f64far_kernel(const double* __restrict__ pos,
+                                                             const double* __restrict__ J, int64_t ld,
+                                                             const double4* __restrict__ g4,
+                                                             const double* __restrict__ tbox,
+                                                             const uint32_t* __restrict__ perm, int64_t np,
+                                                             int transposed, double* __restrict__ P)
+{
+    __shared__ double4 lp[kTile];     // x, y, z, r^2 cut (-1: static source)
+    __shared__ double4 lg[kTile];     // Gamma, 1/sigma
+    __shared__ double lj[9][kTile];   // J
+    __shared__ float wb[kBlock / 64][6];
+    __shared__ int nlist[kBlock];
+    __shared__ int ncnt[kBlock / 64];
+
+    const int tid = threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + tid;
+    const int64_t ib = (int64_t)blockIdx.x * kBlock;
+    const int64_t it = i < np ? i : ib;
+    const double tx = pos[it], ty = pos[ld + it], tz = pos[2 * ld + it];
+    double JT[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) JT[c] = J[c * ld + it];
+    const double org[3] = {pos[ib], pos[ld + ib], pos[2 * ld + ib]};
+    float blo[3], bhi[3];
+    block_box(wave_box((float)(tx - org[0]), (float)(ty - org[1]), (float)(tz - org[2])), wb, blo, bhi);
+
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    NearTiles near(tbox, 0, (np + kTile - 1) / kTile, org, blo, bhi, 1.0001f, nlist, ncnt);
+    for (int64_t tile; (tile = near.next()) >= 0;) {
+        const int64_t base = tile * kTile;
+        const int64_t j = base + tid;
+        __syncthreads();   // the previous tile's readers are done
+        if (j < np) {
+            const double4 g = g4[j];
+            lp[tid] = make_double4(pos[j], pos[ld + j], pos[2 * ld + j], estr_cut64<K>(g.w));
+            lg[tid] = g;
+#pragma unroll
+            for (int c = 0; c < 9; ++c) lj[c][tid] = J[c * ld + j];
+        }
+        __syncthreads();
+        const int n = (int)min((int64_t)kTile, np - base);
+        for (int s = 0; s < n; ++s) {
+            const double4 a = lp[s];
+            const double dx = a.x - tx, dy = a.y - ty, dz = a.z - tz;
+            const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+            const bool in = r2 < a.w;
+            if (__any(in)) {
+                const double4 b = lg[s];
+                double S1, S2, S3;
+                if (transposed) {
+                    S1 = (JT[0] - lj[0][s]) * b.x + (JT[1] - lj[1][s]) * b.y + (JT[2] - lj[2][s]) * b.z;
+                    S2 = (JT[3] - lj[3][s]) * b.x + (JT[4] - lj[4][s]) * b.y + (JT[5] - lj[5][s]) * b.z;
+                    S3 = (JT[6] - lj[6][s]) * b.x + (JT[7] - lj[7][s]) * b.y + (JT[8] - lj[8][s]) * b.z;
+                } else {
+                    S1 = (JT[0] - lj[0][s]) * b.x + (JT[3] - lj[3][s]) * b.y + (JT[6] - lj[6][s]) * b.z;
+                    S2 = (JT[1] - lj[1][s]) * b.x + (JT[4] - lj[4][s]) * b.y + (JT[7] - lj[7][s]) * b.z;
+                    S3 = (JT[2] - lj[2][s]) * b.x + (JT[5] - lj[5][s]) * b.y + (JT[8] - lj[8][s]) * b.z;
+                }
+                const double z = in ? zeta64<K>(r2, b.w) : 0.0;
+                s1 = fma(z, S1, s1);
+                s2 = fma(z, S2, s2);
+                s3 = fma(z, S3, s3);
+            }
+        }
+    }
+    if (i < np) {
+        double* rec = P + (perm ? (int64_t)perm[i] : i) * NF;
+        if (rec[RSTATIC] == 0) {
+            rec[RSFS] += s1;
+            rec[RSFS + 1] += s2;
+            rec[RSFS + 2] += s3;
+        }
+    }
+}
+
 int launch_estr_impl(vpm_field f, int kernel, int transposed, int precision, const uint32_t* perm)
 {
     const int64_t np = f->np;
@@ -395,6 +543,50 @@ int launch_estr_impl(vpm_field f, int kernel, int transposed, int precision, con
                        perm, transposed, f->P);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "estr_finalize");
+    return VPM_OK;
+}
+
+int launch_estr_f64far(vpm_field f, int kernel, int transposed)
+{
+    const int64_t np = f->np;
+    if (np == 0) return VPM_OK;
+    int rc;
+    const uint32_t* perm = nullptr;
+    if ((rc = sort_field(f, perm))) return rc;
+    const int64_t ld = f->cap;
+    const int64_t ntile = (np + kTile - 1) / kTile;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o_j = up(sizeof(double) * 3 * (size_t)ld), o_g = o_j + up(sizeof(double) * 9 * (size_t)ld);
+    const size_t o_c = o_g + up(sizeof(double4) * (size_t)ld), o_t = o_c + up(sizeof(float) * (size_t)ld);
+    if ((rc = ensure(f->adj, o_t + sizeof(double) * 8 * (size_t)ntile))) return rc;
+    char* base = (char*)f->adj.p;
+    double* pos = (double*)base;
+    double* J = (double*)(base + o_j);
+    double4* g4 = (double4*)(base + o_g);
+    float* cutf = (float*)(base + o_c);
+    double* tbox = (double*)(base + o_t);
+    switch (kernel) {
+#define VPM_PE64(KID) case KID: hipLaunchKernelGGL(pack_estr64_kernel<KID>, dim3(nblk(np)), dim3(kBlock), 0, f->stream, f->P, np, perm, pos, J, ld, g4, cutf); break;
+        VPM_PE64(VPM_KERNEL_SINGULAR) VPM_PE64(VPM_KERNEL_GAUSSIAN) VPM_PE64(VPM_KERNEL_GAUSSIANERF)
+        VPM_PE64(VPM_KERNEL_WINCKELMANS)
+#undef VPM_PE64
+    default: return fail(VPM_EINVAL, "unknown kernel id");
+    }
+    hipLaunchKernelGGL(tile_box_kernel<1>, dim3((unsigned)ntile), dim3(kBlock), 0, f->stream, (const double*)pos, ld,
+                       (const float*)cutf, np, tbox);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "pack_estr64");
+    timing_begin(f);
+    switch (kernel) {
+#define VPM_EF64(KID) case KID: hipLaunchKernelGGL(estr_f64far_kernel<KID>, dim3(nblk(np)), dim3(kBlock), 0, f->stream, (const double*)pos, (const double*)J, ld, (const double4*)g4, (const double*)tbox, perm, np, transposed, f->P); break;
+        VPM_EF64(VPM_KERNEL_SINGULAR) VPM_EF64(VPM_KERNEL_GAUSSIAN) VPM_EF64(VPM_KERNEL_GAUSSIANERF)
+        VPM_EF64(VPM_KERNEL_WINCKELMANS)
+#undef VPM_EF64
+    default: return fail(VPM_EINVAL, "unknown kernel id");
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "estr_f64far");
+    timing_end(f, 2);
     return VPM_OK;
 }
 

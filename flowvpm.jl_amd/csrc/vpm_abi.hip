@@ -145,6 +145,7 @@ static int uj_direct_locked(vpm_field f, const vpm_config* cfg, uint32_t flags)
     if ((rc = stage_field(f, cfg->kernel, prec, sort, st))) return rc;
     if ((rc = launch_uj(f, st, 0, f->np, st, 0, f->np, cfg->kernel, prec))) return rc;
     if (flags & VPM_UJ_SFS) {
+        if (far64) return launch_estr_f64far(f, cfg->kernel, cfg->transposed);
         const uint32_t* perm = nullptr;
         if (cfg->precision == VPM_PREC_F32C && (rc = sort_field(f, perm))) return rc;
         if ((rc = launch_estr_impl(f, cfg->kernel, cfg->transposed, cfg->precision, perm))) return rc;

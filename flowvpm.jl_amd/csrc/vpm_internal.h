@@ -161,6 +161,8 @@ void choose_split(int64_t nt, int64_t ns, int64_t kmax, int64_t& K, int64_t& chu
 // Scratch bytes of sort_field (temp storage of the radix sort included).
 int sort_scratch_bytes(vpm_field f, size_t& bytes);
 int launch_estr_impl(vpm_field f, int kernel, int transposed, int precision, const uint32_t* perm);
+// E_str in FP64 with a far cut, sources in spatial order (VPM_UJ_F64_FAR)
+int launch_estr_f64far(vpm_field f, int kernel, int transposed);
 int launch_reset(vpm_field f, int which);
 int launch_update_states(vpm_field f, double a, double b, double dt, const double Uinf[3],
                          double fr, double gr, double zeta0, int transposed);

@@ -55,6 +55,9 @@ enum { VPM_PREC_F32C = 0, VPM_PREC_F64 = 1 };
  *                  where the kernel's g is 1 and q g' is 0 to FP64 rounding;
  *                  near pairs take the reference's g and dg.  It agrees with
  *                  the reference arithmetic to FP64 rounding of the sums.
+ *                  With VPM_UJ_SFS, E_str is evaluated in FP64 over the
+ *                  pairs within q = 10 (Gaussian-erf) or q^2 = 14 (Gaussian),
+ *                  where zeta has fallen below 2e-22 of zeta(0).
  *                  Not taken by vpm_uj_direct_noalloc (VPM_EINVAL). */
 enum { VPM_UJ_RESET = 1, VPM_UJ_RESET_SFS = 2, VPM_UJ_SFS = 4,
        VPM_UJ_NO_SORT = 8, VPM_UJ_F64_FAR = 16 };

@@ -189,6 +189,56 @@ def test_f64far_with_sfs_and_accumulate(vpm):
     assert np.allclose(C2[:, 9:12], 2 * A[:, 9:12], rtol=1e-13, atol=0), np.abs(C2[:, 9:12] - 2 * A[:, 9:12]).max()
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("transposed", [True, False])
+def test_estr_f64far_vs_oracle(vpm, orc, kernel, transposed):
+    """E_str under the far form (UJ with sfs): FP64 pairs within q_cut only
+    (Gaussian-erf q = 10, Gaussian q^2 = 14, where zeta has fallen to 2e-22 of
+    zeta(0); Winckelmans every pair; singular the coincident pairs), in the
+    U/J kernels' spatial order, on a cloud with coincident and static
+    particles.  Against the oracle's Estr_direct! on the same J, per particle
+    to 1e-12 of the summation scale s_i = sum_j zeta |S_ij|, floored at 1e-6
+    of its largest value: a particle whose every pair lies near or past q_cut
+    (isolated in this sparse cloud: the Gaussian's zeta(3.7) is e^-52 of
+    zeta(0)) has an E_str of that size, which the cut drops."""
+    P = vpm.workloads.random_cloud(3000, seed=31)
+    P[100, 0:3] = P[7, 0:3]                          # coincident pairs
+    P[2000, 0:3] = P[7, 0:3]
+    P[[5, 50, 500], 42] = 1.0                        # static particles
+    A = P.copy()
+    vpm.UJ_hip(A, kernel=kernel, precision="f64far", sfs=True, reset_sfs=True, transposed=transposed)
+    R = A.copy()
+    R[:, 39:42] = 0.0
+    orc.estr_direct(R, kernel, transposed=transposed)
+    scale = orc.estr_scale(A, kernel, transposed=transposed)
+    e = np.linalg.norm(A[:, 39:42] - R[:, 39:42], axis=1)
+    ratio = e / np.maximum(scale, 1e-6 * scale.max()) if scale.max() > 0 else e
+    _say("estr f64far", kernel, transposed, "err/scale max %.2e" % ratio.max())
+    assert np.all(A[[5, 50, 500], 39:42] == 0)
+    assert ratio.max() < 1e-12, ratio.max()
+
+
+def test_estr_f64far_jet_column(vpm, orc):
+    """The jet column at 2^16 particles, Gaussian-erf (most source tiles
+    beyond q_cut of a block): E_str under the far form against the
+    reference-arithmetic FP64 E_str on the same J, per particle to 1e-12 of
+    the summation scale floored as above (oracle, on a sample of targets)."""
+    P = vpm.workloads.jet_column(1 << 16)
+    A = P.copy()
+    vpm.UJ_hip(A, kernel="gaussianerf", precision="f64far", sfs=True, reset_sfs=True)
+    B = A.copy()
+    B[:, 39:42] = 0.0
+    vpm.Estr_hip(B, kernel="gaussianerf", precision="f64", transposed=True)
+    idx = np.arange(0, P.shape[0], 61)
+    t = [np.asarray(idx, dtype=np.int64), np.asarray(idx + 1, dtype=np.int64),
+         np.zeros(len(idx), dtype=np.int64), np.full(len(idx), P.shape[0], dtype=np.int64)]
+    scale = orc.estr_scale_ranges(A, "gaussianerf", *t)[idx]
+    e = np.linalg.norm(A[idx, 39:42] - B[idx, 39:42], axis=1)
+    ratio = e / np.maximum(scale, 1e-6 * scale.max())
+    _say("estr f64far jet 2^16: err/scale max %.2e" % ratio.max())
+    assert ratio.max() < 1e-12
+
+
 def test_f64far_argument_checks(vpm):
     """VPM_UJ_F64_FAR needs VPM_PREC_F64, in vpm_uj_direct and in the step's
     uj_flags (which take no other bit), and the no-allocation entry point does
