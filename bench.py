@@ -270,15 +270,15 @@ def cfg1_cloud(vpm, n=65536, reps=5, cpu=True):
 def cfg2_rollout(vpm, nsteps=10, dt=0.01, cpu=True):
     """Two leapfrogging rings, N = 262,144: nsteps rk3 steps of rVPM with the
     dynamic SFS model (SFS_Cd_twolevel_nobackscatter), core spreading
-    (nu = 1e-4) and Pedrizzetti relaxation every step; FP32c pair passes
-    against the FP64 rollout."""
+    (nu = 1e-4) and Pedrizzetti relaxation every step; FP32c pair passes, and
+    FP64 with the far form, against the FP64 rollout."""
     api = vpm.api
     P = vpm.workloads.leapfrog_cfg2()
     n = P.shape[0]
     sg0 = float(P[0, 6])
     res = {"N": n, "steps": nsteps, "dt": dt}
     finals = {}
-    for prec in ("f32c", "f64"):
+    for prec in ("f32c", "f64", "f64far"):
         pf = api.ParticleField(n, kernel="gaussianerf", formulation=api.rVPM, relaxation=api.pedrizzetti,
                                SFS=api.SFS_Cd_twolevel_nobackscatter, viscous=api.CoreSpreading(1e-4, sg0),
                                precision=prec)
@@ -293,7 +293,10 @@ def cfg2_rollout(vpm, nsteps=10, dt=0.01, cpu=True):
     res["error_f32c_vs_f64"] = {name: vpm.metrics.err_stats(A[:, sl], R[:, sl])
                                 for name, sl in (("X", slice(0, 3)), ("Gamma", slice(3, 6)),
                                                  ("sigma", slice(6, 7)))}
-    log("bench: cfg2 errors %s" % res["error_f32c_vs_f64"])
+    res["error_f64far_vs_f64"] = {name: vpm.metrics.err_stats(finals["f64far"][:, sl], R[:, sl])
+                                  for name, sl in (("X", slice(0, 3)), ("Gamma", slice(3, 6)),
+                                                   ("sigma", slice(6, 7)))}
+    log("bench: cfg2 errors %s; far form %.2f s" % (res["error_f32c_vs_f64"], res["f64far_s"]))
     if cpu:
         # The reference's step with this model evaluates U+J 5 times (3
         # stages, the test filter of stage 1, the relaxation's UJ) and E_str
