@@ -218,6 +218,24 @@ def test_estr_f64far_vs_oracle(vpm, orc, kernel, transposed):
     assert ratio.max() < 1e-12, ratio.max()
 
 
+def test_estr_f64far_edge_sizes(vpm, orc):
+    """The far-cut E_str at sizes around the wave and tile widths."""
+    for n in (1, 2, 63, 65, 257):
+        P = vpm.workloads.random_cloud(n, seed=40 + n)
+        P[:, 6] *= 3.0                                   # overlapping cores
+        A = P.copy()
+        vpm.UJ_hip(A, kernel="gaussianerf", precision="f64far", sfs=True, reset_sfs=True)
+        R = A.copy()
+        R[:, 39:42] = 0.0
+        orc.estr_direct(R, "gaussianerf", transposed=True)
+        scale = orc.estr_scale(A, "gaussianerf", transposed=True)
+        e = np.linalg.norm(A[:, 39:42] - R[:, 39:42], axis=1)
+        if n == 1:
+            assert np.all(A[:, 39:42] == 0)
+            continue
+        assert (e / np.maximum(scale, 1e-6 * scale.max())).max() < 1e-12, n
+
+
 def test_estr_f64far_jet_column(vpm, orc):
     """The jet column at 2^16 particles, Gaussian-erf (most source tiles
     beyond q_cut of a block): E_str under the far form against the
