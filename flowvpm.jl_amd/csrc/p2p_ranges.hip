@@ -374,15 +374,17 @@ __global__ __launch_bounds__(kBlock, R > 1 ? 6 : 4) void ranges_wave_f32c_kernel
 }
 
 #pragma clang fp contract(off)
-template <int K>
-__global__ __launch_bounds__(kBlock) void ranges_f64_kernel(const RangeItem* __restrict__ items,
+// NT lanes per item: kBlock, or 64 (one wave) when every item of the plan
+// has at most 64 targets; each target sums its sources in list order either way.
+template <int K, int NT>
+__global__ __launch_bounds__(NT) void ranges_f64_kernel(const RangeItem* __restrict__ items,
                                                             const int2* __restrict__ lsrc,
                                                             const double* __restrict__ pos, int64_t ld,
                                                             const double4* __restrict__ g64,
                                                             double* __restrict__ P)
 {
-    __shared__ double4 lpos[kTile];
-    __shared__ double4 lgam[kTile];
+    __shared__ double4 lpos[NT];
+    __shared__ double4 lgam[NT];
     const RangeItem it = items[xcd_block(blockIdx.x, gridDim.x)];
     const int tid = threadIdx.x;
     const int64_t t = it.t_begin + tid;
@@ -395,8 +397,8 @@ __global__ __launch_bounds__(kBlock) void ranges_f64_kernel(const RangeItem* __r
     for (int64_t l = it.l_begin; l < it.l_end; ++l) {
         const int2 rs = lsrc[l];
         const int64_t sb = rs.x, se = rs.y;
-        for (int64_t base = sb; base < se; base += kTile) {
-            const int n = (int)((se - base) < kTile ? (se - base) : kTile);
+        for (int64_t base = sb; base < se; base += NT) {
+            const int n = (int)((se - base) < NT ? (se - base) : NT);
             __syncthreads();
             if (tid < n) {
                 const int64_t j = base + tid;
@@ -485,7 +487,8 @@ int make_dev_plan(vpm_field f, const int64_t* tb, const int64_t* te, const int64
 }
 
 // U/J of a planned list (F32C wave kernels for wave plans, the block kernels
-// otherwise; F64 always takes the block kernel, whatever the item size).
+// otherwise; F64 takes its block kernel, with one wave per item on wave plans,
+// whose items have at most 64 targets).
 int launch_p2p_plan(vpm_field f, int kernel, int precision, const DevPlan& dp)
 {
     if (dp.nitems == 0) return VPM_OK;
@@ -497,7 +500,13 @@ int launch_p2p_plan(vpm_field f, int kernel, int precision, const DevPlan& dp)
     timing_begin(f);
     if (precision == VPM_PREC_F64) {
         switch (kernel) {
-#define VPM_R64(KID) case KID: hipLaunchKernelGGL(ranges_f64_kernel<KID>, grid, dim3(kBlock), 0, f->stream, d_items, d_src, st.pos, st.ld, st.g64, f->P); break;
+#define VPM_R64(KID)                                                                                              \
+    case KID:                                                                                                     \
+        if (dp.wave) hipLaunchKernelGGL((ranges_f64_kernel<KID, 64>), grid, dim3(64), 0, f->stream, d_items, d_src, \
+                                        st.pos, st.ld, st.g64, f->P);                                             \
+        else hipLaunchKernelGGL((ranges_f64_kernel<KID, kBlock>), grid, dim3(kBlock), 0, f->stream, d_items, d_src, \
+                                st.pos, st.ld, st.g64, f->P);                                                     \
+        break;
             VPM_R64(VPM_KERNEL_SINGULAR) VPM_R64(VPM_KERNEL_GAUSSIAN) VPM_R64(VPM_KERNEL_GAUSSIANERF) VPM_R64(VPM_KERNEL_WINCKELMANS)
 #undef VPM_R64
         default: return fail(VPM_EINVAL, "unknown kernel id");
@@ -535,13 +544,15 @@ int launch_p2p_plan(vpm_field f, int kernel, int precision, const DevPlan& dp)
     return VPM_OK;
 }
 
-// F32C over short target intervals (FMM leaves, mean <= 128 targets): one
-// wave per <= 64 targets; otherwise one block per <= 256 targets.
+// Short target intervals (FMM leaves, mean <= 128 targets): one wave per
+// <= 64 / R targets (F32C) or <= 64 targets (FP64); otherwise one block per
+// <= 256 targets.
 int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb, const int64_t* te,
                       const int64_t* sb, const int64_t* se, int64_t npairs)
 {
     DevPlan dp;
-    if (int rc = make_dev_plan(f, tb, te, sb, se, npairs, precision != VPM_PREC_F64, 8, f->rng, dp)) return rc;
+    if (int rc = make_dev_plan(f, tb, te, sb, se, npairs, true, precision == VPM_PREC_F64 ? 1 : 8, f->rng, dp))
+        return rc;
     return launch_p2p_plan(f, kernel, precision, dp);
 }
 
@@ -1033,14 +1044,14 @@ __device__ __forceinline__ double zeta_ref_r(double r)
 
 // FP64: subfilterscale_models.jl:16-41 per pair, list order, accumulating
 // onto SFS directly as the reference does.
-template <int K>
-__global__ __launch_bounds__(kBlock) void estr_ranges_f64_kernel(const RangeItem* __restrict__ items,
+template <int K, int NT>
+__global__ __launch_bounds__(NT) void estr_ranges_f64_kernel(const RangeItem* __restrict__ items,
                                                                  const int2* __restrict__ lsrc,
                                                                  double* __restrict__ P, int transposed)
 {
-    __shared__ double lx[kTile][4];
-    __shared__ double lg[kTile][3];
-    __shared__ double lj[kTile][9];
+    __shared__ double lx[NT][4];
+    __shared__ double lg[NT][3];
+    __shared__ double lj[NT][9];
     const RangeItem it = items[xcd_block(blockIdx.x, gridDim.x)];
     const int tid = threadIdx.x;
     const int64_t t = it.t_begin + tid;
@@ -1053,8 +1064,8 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_f64_kernel(const RangeItem
     for (int64_t l = it.l_begin; l < it.l_end; ++l) {
         const int2 rs = lsrc[l];
         const int64_t sb = rs.x, se = rs.y;
-        for (int64_t base = sb; base < se; base += kTile) {
-            const int n = (int)((se - base) < kTile ? (se - base) : kTile);
+        for (int64_t base = sb; base < se; base += NT) {
+            const int n = (int)((se - base) < NT ? (se - base) : NT);
             __syncthreads();
             if (tid < n) {
                 const double* sr = P + (base + tid) * NF;
@@ -1096,8 +1107,8 @@ __global__ __launch_bounds__(kBlock) void estr_ranges_f64_kernel(const RangeItem
 #pragma clang fp contract(on)
 
 // E_str of a planned list: F32C wave kernels for wave plans, block kernels
-// otherwise; FP64 (and the singular kernel, whose zeta is a delta) always
-// takes the FP64 block kernel, whatever the item size.
+// otherwise; FP64 (and the singular kernel, whose zeta is a delta) takes the
+// FP64 block kernel, with one wave per item on wave plans.
 int launch_estr_plan(vpm_field f, int kernel, int precision, int transposed, const DevPlan& dp)
 {
     if (dp.nitems == 0) return VPM_OK;
@@ -1131,7 +1142,13 @@ int launch_estr_plan(vpm_field f, int kernel, int precision, int transposed, con
         }
     } else if (f64) {
         switch (kernel) {
-#define VPM_E64(KID) case KID: hipLaunchKernelGGL(estr_ranges_f64_kernel<KID>, grid, dim3(kBlock), 0, f->stream, d_items, d_src, f->P, transposed); break;
+#define VPM_E64(KID)                                                                                            \
+    case KID:                                                                                                   \
+        if (dp.wave) hipLaunchKernelGGL((estr_ranges_f64_kernel<KID, 64>), grid, dim3(64), 0, f->stream, d_items, \
+                                        d_src, f->P, transposed);                                               \
+        else hipLaunchKernelGGL((estr_ranges_f64_kernel<KID, kBlock>), grid, dim3(kBlock), 0, f->stream, d_items, \
+                                d_src, f->P, transposed);                                                       \
+        break;
             VPM_E64(VPM_KERNEL_SINGULAR) VPM_E64(VPM_KERNEL_GAUSSIAN) VPM_E64(VPM_KERNEL_GAUSSIANERF) VPM_E64(VPM_KERNEL_WINCKELMANS)
 #undef VPM_E64
         default: return fail(VPM_EINVAL, "unknown kernel id");
@@ -1155,7 +1172,7 @@ int launch_estr_ranges(vpm_field f, int kernel, int precision, int transposed, c
 {
     const bool f64 = precision == VPM_PREC_F64 || kernel == VPM_KERNEL_SINGULAR;
     DevPlan dp;
-    if (int rc = make_dev_plan(f, tb, te, sb, se, npairs, !f64, 8, f->rng, dp)) return rc;
+    if (int rc = make_dev_plan(f, tb, te, sb, se, npairs, true, f64 ? 1 : 8, f->rng, dp)) return rc;
     return launch_estr_plan(f, kernel, precision, transposed, dp);
 }
 
@@ -1365,13 +1382,13 @@ __global__ __launch_bounds__(kBlock) void zeta_ranges_wave_f32c_kernel(const Ran
 #pragma clang fp contract(off)
 // FP64: viscous.jl:545-554 per pair, list order, accumulating onto J[1:3]
 // as the reference does.
-template <int K>
-__global__ __launch_bounds__(kBlock) void zeta_ranges_f64_kernel(const RangeItem* __restrict__ items,
+template <int K, int NT>
+__global__ __launch_bounds__(NT) void zeta_ranges_f64_kernel(const RangeItem* __restrict__ items,
                                                                  const int2* __restrict__ lsrc,
                                                                  double* __restrict__ P)
 {
-    __shared__ double4 ls[kTile];   // x, y, z, sigma
-    __shared__ double4 lg[kTile];   // Gamma
+    __shared__ double4 ls[NT];   // x, y, z, sigma
+    __shared__ double4 lg[NT];   // Gamma
     const RangeItem it = items[xcd_block(blockIdx.x, gridDim.x)];
     const int tid = threadIdx.x;
     const int64_t t = it.t_begin + tid;
@@ -1382,8 +1399,8 @@ __global__ __launch_bounds__(kBlock) void zeta_ranges_f64_kernel(const RangeItem
     for (int64_t l = it.l_begin; l < it.l_end; ++l) {
         const int2 rs = lsrc[l];
         const int64_t sb = rs.x, se = rs.y;
-        for (int64_t base = sb; base < se; base += kTile) {
-            const int n = (int)((se - base) < kTile ? (se - base) : kTile);
+        for (int64_t base = sb; base < se; base += NT) {
+            const int n = (int)((se - base) < NT ? (se - base) : NT);
             __syncthreads();
             if (tid < n) {
                 const double* sr = P + (base + tid) * NF;
@@ -1411,7 +1428,7 @@ __global__ __launch_bounds__(kBlock) void zeta_ranges_f64_kernel(const RangeItem
 
 // zeta of a planned list: the F32C wave kernels (R > 1) or the 64-lane block
 // kernel for wave plans, 256-lane blocks otherwise; FP64 (and the singular
-// kernel) takes the FP64 block kernel, whatever the item size.
+// kernel) takes the FP64 block kernel, with one wave per item on wave plans.
 int launch_zeta_plan(vpm_field f, int kernel, int precision, const DevPlan& dp)
 {
     if (dp.nitems == 0) return VPM_OK;
@@ -1427,7 +1444,13 @@ int launch_zeta_plan(vpm_field f, int kernel, int precision, const DevPlan& dp)
     }
     if (f64) {
         switch (kernel) {
-#define VPM_Z64(KID) case KID: hipLaunchKernelGGL(zeta_ranges_f64_kernel<KID>, grid, dim3(kBlock), 0, f->stream, d_items, d_src, f->P); break;
+#define VPM_Z64(KID)                                                                                            \
+    case KID:                                                                                                   \
+        if (wave) hipLaunchKernelGGL((zeta_ranges_f64_kernel<KID, 64>), grid, dim3(64), 0, f->stream, d_items,   \
+                                     d_src, f->P);                                                              \
+        else hipLaunchKernelGGL((zeta_ranges_f64_kernel<KID, kBlock>), grid, dim3(kBlock), 0, f->stream, d_items, \
+                                d_src, f->P);                                                                   \
+        break;
             VPM_Z64(VPM_KERNEL_SINGULAR) VPM_Z64(VPM_KERNEL_GAUSSIAN) VPM_Z64(VPM_KERNEL_GAUSSIANERF) VPM_Z64(VPM_KERNEL_WINCKELMANS)
 #undef VPM_Z64
         default: return fail(VPM_EINVAL, "unknown kernel id");
@@ -1470,7 +1493,7 @@ int launch_zeta_ranges(vpm_field f, int kernel, int precision, const int64_t* tb
 {
     const bool f64 = precision == VPM_PREC_F64 || kernel == VPM_KERNEL_SINGULAR;
     DevPlan dp;
-    if (int rc = make_dev_plan(f, tb, te, sb, se, npairs, !f64, 8, f->rng, dp)) return rc;
+    if (int rc = make_dev_plan(f, tb, te, sb, se, npairs, true, f64 ? 1 : 8, f->rng, dp)) return rc;
     return launch_zeta_plan(f, kernel, precision, dp);
 }
 
