@@ -418,6 +418,66 @@ __global__ __launch_bounds__(NT) void ranges_f64_kernel(const RangeItem* __restr
         for (int c = 0; c < 9; ++c) rec[RJ + c] += acc[3 + c];
     }
 }
+// FP64 U/J over a wave plan whose items hold at most 64 / G targets: G items
+// per wave, each in its own group of 64 / G lanes with its own LDS rows,
+// streaming its own source ranges (groups run their loops under the exec
+// mask).  Every target sums its sources in list order, the order of the
+// block kernel, so the results are the same bit for bit.
+template <int K, int G>
+__global__ __launch_bounds__(kBlock) void ranges_f64_group_kernel(const RangeItem* __restrict__ items,
+                                                                  int64_t nitems, const int2* __restrict__ lsrc,
+                                                                  const double* __restrict__ pos, int64_t ld,
+                                                                  const double4* __restrict__ g64,
+                                                                  double* __restrict__ P)
+{
+    constexpr int NG = 64 / G;   // lanes and LDS rows per item
+    __shared__ double4 lpos_all[kBlock / 64][64];
+    __shared__ double4 lgam_all[kBlock / 64][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane / NG, k = lane % NG;
+    double4* lpos = lpos_all[wave] + g * NG;
+    double4* lgam = lgam_all[wave] + g * NG;
+    const int64_t item = (xcd_block(blockIdx.x, gridDim.x) * (kBlock / 64) + wave) * G + g;
+    const RangeItem it = item < nitems ? items[item] : RangeItem{0, 0, 0, 0};
+    const int64_t t = it.t_begin + k;
+    const bool active = t < it.t_end;
+    const int64_t tt = active ? t : 0;
+    const double tx = pos[tt], ty = pos[ld + tt], tz = pos[2 * ld + tt];
+    double acc[kAccF64];
+#pragma unroll
+    for (int c = 0; c < kAccF64; ++c) acc[c] = 0.0;
+    int64_t l = it.l_begin, base = 0, se = 0;
+    if (l < it.l_end) { const int2 rs = lsrc[l]; base = rs.x; se = rs.y; }
+    for (;;) {
+        while (l < it.l_end && base >= se) {   // the group's next nonempty range
+            if (++l < it.l_end) { const int2 rs = lsrc[l]; base = rs.x; se = rs.y; }
+        }
+        const bool work = l < it.l_end;
+        if (!__any(work)) break;
+        const int n = work ? (int)((se - base) < NG ? (se - base) : NG) : 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (k < n) {
+            const int64_t j = base + k;
+            const double4 gg = g64[j];
+            lpos[k] = make_double4(pos[j], pos[ld + j], pos[2 * ld + j], gg.w);
+            lgam[k] = make_double4(gg.x, gg.y, gg.z, 0.0);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int s = 0; s < n; ++s) pair_f64<K>(tx, ty, tz, lpos[s], lgam[s], acc);
+        base += n;
+    }
+    if (active) {
+        double* rec = P + t * NF;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) rec[RU + c] += acc[c];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) rec[RJ + c] += acc[3 + c];
+    }
+}
 #pragma clang fp contract(on)
 
 // ------------------------------------------------------------- device plans
@@ -500,15 +560,27 @@ int launch_p2p_plan(vpm_field f, int kernel, int precision, const DevPlan& dp)
     timing_begin(f);
     if (precision == VPM_PREC_F64) {
         switch (kernel) {
+#define VPM_RG64(KID, G)                                                                                          \
+    hipLaunchKernelGGL((ranges_f64_group_kernel<KID, G>),                                                         \
+                       dim3((unsigned)((dp.nitems + G * (kBlock / 64) - 1) / (G * (kBlock / 64)))), dim3(kBlock), \
+                       0, f->stream, d_items, dp.nitems, d_src, st.pos, st.ld, st.g64, f->P)
 #define VPM_R64(KID)                                                                                              \
     case KID:                                                                                                     \
-        if (dp.wave) hipLaunchKernelGGL((ranges_f64_kernel<KID, 64>), grid, dim3(64), 0, f->stream, d_items, d_src, \
-                                        st.pos, st.ld, st.g64, f->P);                                             \
-        else hipLaunchKernelGGL((ranges_f64_kernel<KID, kBlock>), grid, dim3(kBlock), 0, f->stream, d_items, d_src, \
-                                st.pos, st.ld, st.g64, f->P);                                                     \
+        if (dp.wave && dp.R > 1) {                                                                                \
+            if (dp.R == 8) VPM_RG64(KID, 8);                                                                      \
+            else if (dp.R == 4) VPM_RG64(KID, 4);                                                                 \
+            else VPM_RG64(KID, 2);                                                                                \
+        } else if (dp.wave) {                                                                                     \
+            hipLaunchKernelGGL((ranges_f64_kernel<KID, 64>), grid, dim3(64), 0, f->stream, d_items, d_src,        \
+                               st.pos, st.ld, st.g64, f->P);                                                      \
+        } else {                                                                                                  \
+            hipLaunchKernelGGL((ranges_f64_kernel<KID, kBlock>), grid, dim3(kBlock), 0, f->stream, d_items,       \
+                               d_src, st.pos, st.ld, st.g64, f->P);                                               \
+        }                                                                                                         \
         break;
             VPM_R64(VPM_KERNEL_SINGULAR) VPM_R64(VPM_KERNEL_GAUSSIAN) VPM_R64(VPM_KERNEL_GAUSSIANERF) VPM_R64(VPM_KERNEL_WINCKELMANS)
 #undef VPM_R64
+#undef VPM_RG64
         default: return fail(VPM_EINVAL, "unknown kernel id");
         }
     } else if (dp.wave) {
@@ -551,8 +623,7 @@ int launch_p2p_ranges(vpm_field f, int kernel, int precision, const int64_t* tb,
                       const int64_t* sb, const int64_t* se, int64_t npairs)
 {
     DevPlan dp;
-    if (int rc = make_dev_plan(f, tb, te, sb, se, npairs, true, precision == VPM_PREC_F64 ? 1 : 8, f->rng, dp))
-        return rc;
+    if (int rc = make_dev_plan(f, tb, te, sb, se, npairs, true, 8, f->rng, dp)) return rc;
     return launch_p2p_plan(f, kernel, precision, dp);
 }
 
@@ -1104,6 +1175,82 @@ __global__ __launch_bounds__(NT) void estr_ranges_f64_kernel(const RangeItem* __
         rec[RSFS + 2] = s3;
     }
 }
+// E_str of estr_ranges_f64_kernel with G items per wave (see
+// ranges_f64_group_kernel): the same pair expression in the same order.
+template <int K, int G>
+__global__ __launch_bounds__(kBlock) void estr_ranges_f64_group_kernel(const RangeItem* __restrict__ items,
+                                                                       int64_t nitems, const int2* __restrict__ lsrc,
+                                                                       double* __restrict__ P, int transposed)
+{
+    constexpr int NG = 64 / G;
+    __shared__ double lx_all[kBlock / 64][64][4];
+    __shared__ double lg_all[kBlock / 64][64][3];
+    __shared__ double lj_all[kBlock / 64][64][9];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane / NG, k = lane % NG;
+    double (*lx)[4] = lx_all[wave] + g * NG;
+    double (*lg)[3] = lg_all[wave] + g * NG;
+    double (*lj)[9] = lj_all[wave] + g * NG;
+    const int64_t item = (xcd_block(blockIdx.x, gridDim.x) * (kBlock / 64) + wave) * G + g;
+    const RangeItem it = item < nitems ? items[item] : RangeItem{0, 0, 0, 0};
+    const int64_t t = it.t_begin + k;
+    const bool active = t < it.t_end;
+    const double* tr = P + (active ? t : 0) * NF;
+    const double tx = tr[RX], ty = tr[RX + 1], tz = tr[RX + 2];
+    double JT[9];
+    for (int c = 0; c < 9; ++c) JT[c] = tr[RJ + c];
+    double s1 = tr[RSFS], s2 = tr[RSFS + 1], s3 = tr[RSFS + 2];
+    int64_t l = it.l_begin, base = 0, se = 0;
+    if (l < it.l_end) { const int2 rs = lsrc[l]; base = rs.x; se = rs.y; }
+    for (;;) {
+        while (l < it.l_end && base >= se) {
+            if (++l < it.l_end) { const int2 rs = lsrc[l]; base = rs.x; se = rs.y; }
+        }
+        const bool work = l < it.l_end;
+        if (!__any(work)) break;
+        const int n = work ? (int)((se - base) < NG ? (se - base) : NG) : 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (k < n) {
+            const double* sr = P + (base + k) * NF;
+            lx[k][0] = sr[RX]; lx[k][1] = sr[RX + 1]; lx[k][2] = sr[RX + 2]; lx[k][3] = sr[RSIG];
+            lg[k][0] = sr[RG]; lg[k][1] = sr[RG + 1]; lg[k][2] = sr[RG + 2];
+            for (int c = 0; c < 9; ++c) lj[k][c] = sr[RJ + c];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int s = 0; s < n; ++s) {
+            const double dx = lx[s][0] - tx, dy = lx[s][1] - ty, dz = lx[s][2] - tz;
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            const double* JS = lj[s];
+            const double* GS = lg[s];
+            double S1, S2, S3;
+            if (transposed) {
+                S1 = (JT[0] - JS[0]) * GS[0] + (JT[1] - JS[1]) * GS[1] + (JT[2] - JS[2]) * GS[2];
+                S2 = (JT[3] - JS[3]) * GS[0] + (JT[4] - JS[4]) * GS[1] + (JT[5] - JS[5]) * GS[2];
+                S3 = (JT[6] - JS[6]) * GS[0] + (JT[7] - JS[7]) * GS[1] + (JT[8] - JS[8]) * GS[2];
+            } else {
+                S1 = (JT[0] - JS[0]) * GS[0] + (JT[3] - JS[3]) * GS[1] + (JT[6] - JS[6]) * GS[2];
+                S2 = (JT[1] - JS[1]) * GS[0] + (JT[4] - JS[4]) * GS[1] + (JT[7] - JS[7]) * GS[2];
+                S3 = (JT[2] - JS[2]) * GS[0] + (JT[5] - JS[5]) * GS[1] + (JT[8] - JS[8]) * GS[2];
+            }
+            const double sigma_inv = 1.0 / lx[s][3];
+            const double zeta_sgm = zeta_ref_r<K>(r * sigma_inv) * sigma_inv * sigma_inv * sigma_inv;
+            s1 += zeta_sgm * S1;
+            s2 += zeta_sgm * S2;
+            s3 += zeta_sgm * S3;
+        }
+        base += n;
+    }
+    if (active) {
+        double* rec = P + t * NF;
+        rec[RSFS] = s1;
+        rec[RSFS + 1] = s2;
+        rec[RSFS + 2] = s3;
+    }
+}
 #pragma clang fp contract(on)
 
 // E_str of a planned list: F32C wave kernels for wave plans, block kernels
@@ -1142,15 +1289,27 @@ int launch_estr_plan(vpm_field f, int kernel, int precision, int transposed, con
         }
     } else if (f64) {
         switch (kernel) {
+#define VPM_EG64(KID, G)                                                                                        \
+    hipLaunchKernelGGL((estr_ranges_f64_group_kernel<KID, G>),                                                  \
+                       dim3((unsigned)((dp.nitems + G * (kBlock / 64) - 1) / (G * (kBlock / 64)))),             \
+                       dim3(kBlock), 0, f->stream, d_items, dp.nitems, d_src, f->P, transposed)
 #define VPM_E64(KID)                                                                                            \
     case KID:                                                                                                   \
-        if (dp.wave) hipLaunchKernelGGL((estr_ranges_f64_kernel<KID, 64>), grid, dim3(64), 0, f->stream, d_items, \
-                                        d_src, f->P, transposed);                                               \
-        else hipLaunchKernelGGL((estr_ranges_f64_kernel<KID, kBlock>), grid, dim3(kBlock), 0, f->stream, d_items, \
-                                d_src, f->P, transposed);                                                       \
+        if (dp.wave && dp.R > 1) {                                                                              \
+            if (dp.R == 8) VPM_EG64(KID, 8);                                                                    \
+            else if (dp.R == 4) VPM_EG64(KID, 4);                                                               \
+            else VPM_EG64(KID, 2);                                                                              \
+        } else if (dp.wave) {                                                                                   \
+            hipLaunchKernelGGL((estr_ranges_f64_kernel<KID, 64>), grid, dim3(64), 0, f->stream, d_items, d_src, \
+                               f->P, transposed);                                                               \
+        } else {                                                                                                \
+            hipLaunchKernelGGL((estr_ranges_f64_kernel<KID, kBlock>), grid, dim3(kBlock), 0, f->stream, d_items, \
+                               d_src, f->P, transposed);                                                        \
+        }                                                                                                       \
         break;
             VPM_E64(VPM_KERNEL_SINGULAR) VPM_E64(VPM_KERNEL_GAUSSIAN) VPM_E64(VPM_KERNEL_GAUSSIANERF) VPM_E64(VPM_KERNEL_WINCKELMANS)
 #undef VPM_E64
+#undef VPM_EG64
         default: return fail(VPM_EINVAL, "unknown kernel id");
         }
     } else {
@@ -1172,7 +1331,7 @@ int launch_estr_ranges(vpm_field f, int kernel, int precision, int transposed, c
 {
     const bool f64 = precision == VPM_PREC_F64 || kernel == VPM_KERNEL_SINGULAR;
     DevPlan dp;
-    if (int rc = make_dev_plan(f, tb, te, sb, se, npairs, true, f64 ? 1 : 8, f->rng, dp)) return rc;
+    if (int rc = make_dev_plan(f, tb, te, sb, se, npairs, true, 8, f->rng, dp)) return rc;
     return launch_estr_plan(f, kernel, precision, transposed, dp);
 }
 
@@ -1424,6 +1583,62 @@ __global__ __launch_bounds__(NT) void zeta_ranges_f64_kernel(const RangeItem* __
         for (int c = 0; c < 3; ++c) rec[RJ + c] = acc[c];
     }
 }
+// zeta_ranges_f64_kernel with G items per wave (see ranges_f64_group_kernel).
+template <int K, int G>
+__global__ __launch_bounds__(kBlock) void zeta_ranges_f64_group_kernel(const RangeItem* __restrict__ items,
+                                                                       int64_t nitems, const int2* __restrict__ lsrc,
+                                                                       double* __restrict__ P)
+{
+    constexpr int NG = 64 / G;
+    __shared__ double4 ls_all[kBlock / 64][64];
+    __shared__ double4 lg_all[kBlock / 64][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane / NG, k = lane % NG;
+    double4* ls = ls_all[wave] + g * NG;
+    double4* lg = lg_all[wave] + g * NG;
+    const int64_t item = (xcd_block(blockIdx.x, gridDim.x) * (kBlock / 64) + wave) * G + g;
+    const RangeItem it = item < nitems ? items[item] : RangeItem{0, 0, 0, 0};
+    const int64_t t = it.t_begin + k;
+    const bool active = t < it.t_end;
+    const double* tr = P + (active ? t : 0) * NF;
+    const double tx = tr[RX], ty = tr[RX + 1], tz = tr[RX + 2];
+    double acc[3] = {tr[RJ], tr[RJ + 1], tr[RJ + 2]};
+    int64_t l = it.l_begin, base = 0, se = 0;
+    if (l < it.l_end) { const int2 rs = lsrc[l]; base = rs.x; se = rs.y; }
+    for (;;) {
+        while (l < it.l_end && base >= se) {
+            if (++l < it.l_end) { const int2 rs = lsrc[l]; base = rs.x; se = rs.y; }
+        }
+        const bool work = l < it.l_end;
+        if (!__any(work)) break;
+        const int n = work ? (int)((se - base) < NG ? (se - base) : NG) : 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (k < n) {
+            const double* sr = P + (base + k) * NF;
+            ls[k] = make_double4(sr[RX], sr[RX + 1], sr[RX + 2], sr[RSIG]);
+            lg[k] = make_double4(sr[RG], sr[RG + 1], sr[RG + 2], 0.0);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int s = 0; s < n; ++s) {
+            const double4 a = ls[s];
+            const double dX1 = tx - a.x, dX2 = ty - a.y, dX3 = tz - a.z;
+            const double r = sqrt(dX1 * dX1 + dX2 * dX2 + dX3 * dX3);
+            const double zeta_sgm = 1 / (a.w * a.w * a.w) * zeta_ref_r<K>(r / a.w);
+            acc[0] += lg[s].x * zeta_sgm;
+            acc[1] += lg[s].y * zeta_sgm;
+            acc[2] += lg[s].z * zeta_sgm;
+        }
+        base += n;
+    }
+    if (active) {
+        double* rec = P + t * NF;
+        for (int c = 0; c < 3; ++c) rec[RJ + c] = acc[c];
+    }
+}
 #pragma clang fp contract(on)
 
 // zeta of a planned list: the F32C wave kernels (R > 1) or the 64-lane block
@@ -1444,15 +1659,27 @@ int launch_zeta_plan(vpm_field f, int kernel, int precision, const DevPlan& dp)
     }
     if (f64) {
         switch (kernel) {
+#define VPM_ZG64(KID, G)                                                                                        \
+    hipLaunchKernelGGL((zeta_ranges_f64_group_kernel<KID, G>),                                                  \
+                       dim3((unsigned)((dp.nitems + G * (kBlock / 64) - 1) / (G * (kBlock / 64)))),             \
+                       dim3(kBlock), 0, f->stream, d_items, dp.nitems, d_src, f->P)
 #define VPM_Z64(KID)                                                                                            \
     case KID:                                                                                                   \
-        if (wave) hipLaunchKernelGGL((zeta_ranges_f64_kernel<KID, 64>), grid, dim3(64), 0, f->stream, d_items,   \
-                                     d_src, f->P);                                                              \
-        else hipLaunchKernelGGL((zeta_ranges_f64_kernel<KID, kBlock>), grid, dim3(kBlock), 0, f->stream, d_items, \
-                                d_src, f->P);                                                                   \
+        if (wave && dp.R > 1) {                                                                                 \
+            if (dp.R == 8) VPM_ZG64(KID, 8);                                                                    \
+            else if (dp.R == 4) VPM_ZG64(KID, 4);                                                               \
+            else VPM_ZG64(KID, 2);                                                                              \
+        } else if (wave) {                                                                                      \
+            hipLaunchKernelGGL((zeta_ranges_f64_kernel<KID, 64>), grid, dim3(64), 0, f->stream, d_items, d_src,  \
+                               f->P);                                                                           \
+        } else {                                                                                                \
+            hipLaunchKernelGGL((zeta_ranges_f64_kernel<KID, kBlock>), grid, dim3(kBlock), 0, f->stream, d_items, \
+                               d_src, f->P);                                                                    \
+        }                                                                                                       \
         break;
             VPM_Z64(VPM_KERNEL_SINGULAR) VPM_Z64(VPM_KERNEL_GAUSSIAN) VPM_Z64(VPM_KERNEL_GAUSSIANERF) VPM_Z64(VPM_KERNEL_WINCKELMANS)
 #undef VPM_Z64
+#undef VPM_ZG64
         default: return fail(VPM_EINVAL, "unknown kernel id");
         }
     } else if (wave) {
@@ -1493,7 +1720,7 @@ int launch_zeta_ranges(vpm_field f, int kernel, int precision, const int64_t* tb
 {
     const bool f64 = precision == VPM_PREC_F64 || kernel == VPM_KERNEL_SINGULAR;
     DevPlan dp;
-    if (int rc = make_dev_plan(f, tb, te, sb, se, npairs, true, f64 ? 1 : 8, f->rng, dp)) return rc;
+    if (int rc = make_dev_plan(f, tb, te, sb, se, npairs, true, 8, f->rng, dp)) return rc;
     return launch_zeta_plan(f, kernel, precision, dp);
 }
 
