@@ -201,6 +201,18 @@ def leaf_list(vpm, P, kernel, allpairs_rate, leaf=64, reps=3):
         call = 1e3 * (time.perf_counter() - t0) / reps
         kms, kc = f.timing(which)
         planned[name] = {"call_ms": call, "kernel_ms": kms / max(kc, 1), "call_over_kernel": call / (kms / max(kc, 1))}
+    # the reference's own precision on the same plan: FP64 pair arithmetic,
+    # each target summing its list in order (one kernel call each)
+    planned_f64 = {}
+    for name, fn, which in (("uj", lambda: plan.p2p(kernel, "f64"), 1),
+                            ("estr", lambda: plan.estr(kernel, "f64", True), 2),
+                            ("zeta", lambda: plan.zeta(kernel, "f64"), 4)):
+        fn()
+        f.profile(True)
+        fn()
+        kms, kc = f.timing(which)
+        f.profile(False)
+        planned_f64[name] = {"kernel_ms": kms / max(kc, 1), "pairs_per_s": pairs / (kms / max(kc, 1) / 1e3)}
     plan.close()
     f.close()
     out = {"N": n, "leaf": leaf, "list_pairs": len(tb), "particle_pairs": pairs, "sources_per_target": pairs / n,
@@ -210,7 +222,7 @@ def leaf_list(vpm, P, kernel, allpairs_rate, leaf=64, reps=3):
            "estr_pairs_per_s_kernel": pairs / (ems / ecnt / 1e3),
            "zeta_call_ms": 1e3 * zwall, "zeta_kernel_ms": zms / zcnt,
            "zeta_pairs_per_s_kernel": pairs / (zms / zcnt / 1e3),
-           "planned": dict(planned, plan_ms=plan_ms)}
+           "planned": dict(planned, plan_ms=plan_ms), "planned_f64": planned_f64}
     log("bench: leaf list %s" % out)
     return out
 
